@@ -1,0 +1,229 @@
+"""ctypes binding of libfrostspin_amd.so (the C-ABI declared in include/frostspin_amd.h).
+
+The library is loaded at import. A missing library is a hard error: there is no
+host fallback for any device operation.
+"""
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libfrostspin_amd.so"
+
+FS_NORM_PARTIALS = 1024
+
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_i8p = ctypes.POINTER(ctypes.c_int8)
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_dblp = ctypes.POINTER(ctypes.c_double)
+
+
+class GemmGroup(ctypes.Structure):
+    _fields_ = [
+        ("m", c_i64),
+        ("n", c_i64),
+        ("k", c_i64),
+        ("a_off", c_i64),
+        ("a_rs", c_i64),
+        ("a_cs", c_i64),
+        ("b_off", c_i64),
+        ("b_rs", c_i64),
+        ("b_cs", c_i64),
+        ("c_off", c_i64),
+        ("c_ld", c_i64),
+    ]
+
+
+class SvdDesc(ctypes.Structure):
+    _fields_ = [
+        ("m", c_i64),
+        ("n", c_i64),
+        ("a_off", c_i64),
+        ("a_rs", c_i64),
+        ("a_cs", c_i64),
+        ("u_off", c_i64),
+        ("s_off", c_i64),
+        ("vt_off", c_i64),
+        ("work_off", c_i64),
+    ]
+
+
+class Copy2dDesc(ctypes.Structure):
+    _fields_ = [
+        ("rows", c_i64),
+        ("cols", c_i64),
+        ("s_off", c_i64),
+        ("s_rs", c_i64),
+        ("s_cs", c_i64),
+        ("d_off", c_i64),
+        ("d_rs", c_i64),
+        ("d_cs", c_i64),
+        ("rw_off", c_i64),
+        ("cw_off", c_i64),
+        ("ci_off", c_i64),
+        ("alpha", c_dbl),
+        ("beta", c_dbl),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point
+_SIGNATURES = {
+    "fs_last_error": (ctypes.c_char_p, []),
+    "fs_abi_version": (c_int, []),
+    "fs_device_count": (c_int, [ctypes.POINTER(c_int)]),
+    "fs_u1_fuse": (c_int, [c_int, ctypes.POINTER(c_i8p), c_i64p, c_u8p, c_i8p]),
+    "fs_find_block_cuts": (
+        c_int,
+        [c_int, ctypes.POINTER(c_dblp), c_i64p, c_i64, c_i64p, c_dbl, c_dbl, c_i64p],
+    ),
+    "fs_abelian_permute_plan_create": (
+        c_int,
+        [
+            c_int,
+            c_i64p,
+            ctypes.POINTER(c_i8p),
+            c_u8p,
+            c_int,
+            c_int,
+            c_i8p,
+            c_i64p,
+            c_int,
+            c_int,
+            c_i32p,
+            c_int,
+            c_int,
+            ctypes.POINTER(c_vp),
+        ],
+    ),
+    "fs_permute_plan_info": (c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
+    "fs_permute_plan_blocks": (c_int, [c_vp, c_i8p, c_i64p]),
+    "fs_permute_plan_tables": (
+        c_int,
+        [c_vp, c_i64p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i64p, c_i64p, c_i64p],
+    ),
+    "fs_permute_plan_execute_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "fs_permute_plan_execute_f32": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "fs_permute_plan_destroy": (c_int, [c_vp]),
+    "fs_grouped_gemm_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(GemmGroup), c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp],
+    ),
+    "fs_grouped_gemm_f64_split": (
+        c_int,
+        [c_int, ctypes.POINTER(GemmGroup), c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp],
+    ),
+    "fs_batched_svd_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(SvdDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
+    ),
+    "fs_weighted_norm2_f64": (c_int, [c_int, c_i64p, c_dblp, c_vp, c_vp, c_vp, c_vp]),
+    "fs_scale_f64": (c_int, [c_i64, c_dbl, c_vp, c_vp]),
+    "fs_scale_rsqrt_dev_f64": (c_int, [c_i64, c_vp, c_vp, c_vp]),
+    "fs_axpby_f64": (c_int, [c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
+    "fs_batched_copy2d_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(Copy2dDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    ),
+    "fs_weighted_trace_f64": (
+        c_int,
+        [c_int, c_i64p, c_i64p, c_i64p, c_dblp, c_vp, c_vp, c_vp],
+    ),
+    "fs_fill_normal_f64": (c_int, [c_i64, ctypes.c_uint64, c_vp, c_vp]),
+    "fs_convert_f64_to_f32": (c_int, [c_i64, c_vp, c_vp, c_vp]),
+    "fs_convert_f32_to_f64": (c_int, [c_i64, c_vp, c_vp, c_vp]),
+}
+
+EXPORTED_SYMBOLS = tuple(_SIGNATURES)
+
+
+def _load():
+    if not _LIB_PATH.exists():
+        msg = (
+            f"{_LIB_PATH.name} is not built: run `python -m frostspin_amd.csrc.build` "
+            "(frostspin_amd has no host fallback for device operations)"
+        )
+        raise ImportError(msg)
+    lib = ctypes.CDLL(os.fspath(_LIB_PATH))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class FrostspinError(RuntimeError):
+    """Error reported by the native library."""
+
+
+def check(status):
+    if status != 0:
+        msg = lib.fs_last_error().decode(errors="replace")
+        raise FrostspinError(msg or f"frostspin_amd error status {status}")
+
+
+# ----------------------------------------------------------------------------
+# small helpers to pass numpy arrays
+# ----------------------------------------------------------------------------
+def i64p(a):
+    return a.ctypes.data_as(c_i64p)
+
+
+def i32p(a):
+    return a.ctypes.data_as(c_i32p)
+
+
+def i8p(a):
+    return a.ctypes.data_as(c_i8p)
+
+
+def u8p(a):
+    return a.ctypes.data_as(c_u8p)
+
+
+def dblp(a):
+    return a.ctypes.data_as(c_dblp)
+
+
+def device_count():
+    n = c_int(0)
+    check(lib.fs_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def u1_fuse(reps, flips):
+    """Row-major Kronecker sum of int8 charges (int8 wrap-around)."""
+    reps = [np.ascontiguousarray(r, dtype=np.int8) for r in reps]
+    sizes = np.array([r.size for r in reps], dtype=np.int64)
+    flips = np.ascontiguousarray(flips, dtype=np.uint8)
+    out = np.empty((int(np.prod(sizes)),), dtype=np.int8)
+    ptrs = (c_i8p * len(reps))(*[i8p(r) for r in reps])
+    check(lib.fs_u1_fuse(len(reps), ptrs, i64p(sizes), u8p(flips), i8p(out)))
+    return out
+
+
+def find_block_cuts_native(block_values, cut_target, dims, cutoff, degen_ratio):
+    vals = [np.ascontiguousarray(v, dtype=np.float64) for v in block_values]
+    n = len(vals)
+    sizes = np.array([v.size for v in vals], dtype=np.int64)
+    dims = np.ascontiguousarray(dims, dtype=np.int64)
+    cuts = np.zeros((n,), dtype=np.int64)
+    if n == 0:
+        return cuts
+    ptrs = (c_dblp * n)(*[dblp(v) for v in vals])
+    check(
+        lib.fs_find_block_cuts(
+            n, ptrs, i64p(sizes), int(cut_target), i64p(dims), float(cutoff),
+            float(degen_ratio), i64p(cuts),
+        )
+    )
+    return cuts

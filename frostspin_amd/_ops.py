@@ -1,0 +1,318 @@
+"""Device operations: the only module that launches work through the C-ABI.
+
+Every function takes torch tensors that live on the HIP device (float64 unless
+stated) and enqueues the native kernel on torch's current stream. Tensors on
+any other device are rejected: there is no host fallback.
+"""
+
+import ctypes
+from collections import OrderedDict
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+DTYPE = torch.float64
+
+
+def device():
+    """Device that holds all block buffers (the current HIP device)."""
+    if not torch.cuda.is_available():
+        msg = "frostspin_amd device operations need a HIP GPU (torch.cuda is unavailable)"
+        raise RuntimeError(msg)
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        msg = "frostspin_amd kernels only accept device tensors"
+        raise RuntimeError(msg)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def empty(n, dtype=DTYPE):
+    return torch.empty((int(n),), dtype=dtype, device=device())
+
+
+def zeros(n, dtype=DTYPE):
+    return torch.zeros((int(n),), dtype=dtype, device=device())
+
+
+def upload(arr, dtype=DTYPE):
+    """Host array -> new 1-D device tensor (flattened in C order)."""
+    a = np.ascontiguousarray(arr).reshape(-1)
+    return torch.from_numpy(a).to(device=device(), dtype=dtype)
+
+
+def download(t):
+    """Device tensor -> host numpy array (synchronizes the stream)."""
+    return t.detach().to("cpu").numpy()
+
+
+# ---------------------------------------------------------------------------
+# permute plans
+# ---------------------------------------------------------------------------
+class PermutePlan:
+    """Owner of a native permute plan (host tables + lazily uploaded device copy)."""
+
+    __slots__ = ("_handle", "irreps", "new_ncoeff", "old_ncoeff", "shapes")
+
+    def __init__(self, handle):
+        self._handle = handle
+        nb = ctypes.c_int64()
+        nc = ctypes.c_int64()
+        oc = ctypes.c_int64()
+        check(lib.fs_permute_plan_info(handle, ctypes.byref(nb), ctypes.byref(nc), ctypes.byref(oc)))
+        self.irreps = np.empty((nb.value,), dtype=np.int8)
+        shapes = np.empty((nb.value, 2), dtype=np.int64)
+        check(lib.fs_permute_plan_blocks(handle, _lib.i8p(self.irreps), _lib.i64p(shapes)))
+        self.shapes = shapes
+        self.new_ncoeff = nc.value
+        self.old_ncoeff = oc.value
+
+    def tables(self):
+        """Host copies of the index tables (inspection and tests)."""
+        sizes = np.zeros((5,), dtype=np.int64)
+        check(lib.fs_permute_plan_tables(self._handle, _lib.i64p(sizes), *([None] * 9)))
+        nor, noc, nnr, nnc, nob = (int(x) for x in sizes)
+        t = {
+            "old_row_block": np.empty((nor,), dtype=np.int32),
+            "old_row_pos": np.empty((nor,), dtype=np.int32),
+            "old_col_block": np.empty((noc,), dtype=np.int32),
+            "old_col_pos": np.empty((noc,), dtype=np.int32),
+            "row_a": np.empty((2 * nnr,), dtype=np.int32),
+            "col_b": np.empty((2 * nnc,), dtype=np.int32),
+            "old_off": np.empty((nob,), dtype=np.int64),
+            "old_rs": np.empty((nob,), dtype=np.int64),
+            "old_cs": np.empty((nob,), dtype=np.int64),
+        }
+        names32 = ("old_row_block", "old_row_pos", "old_col_block", "old_col_pos", "row_a", "col_b")
+        check(
+            lib.fs_permute_plan_tables(
+                self._handle,
+                _lib.i64p(sizes),
+                *(_lib.i32p(t[k]) for k in names32),
+                *(_lib.i64p(t[k]) for k in ("old_off", "old_rs", "old_cs")),
+            )
+        )
+        return t
+
+    def execute(self, src, dst):
+        if dst.dtype == torch.float64:
+            check(lib.fs_permute_plan_execute_f64(self._handle, _ptr(src), _ptr(dst), _stream()))
+        elif dst.dtype == torch.float32:
+            check(lib.fs_permute_plan_execute_f32(self._handle, _ptr(src), _ptr(dst), _stream()))
+        else:
+            msg = f"unsupported dtype {dst.dtype}"
+            raise TypeError(msg)
+
+    def __del__(self):
+        h = self._handle
+        if h is not None and lib is not None:
+            lib.fs_permute_plan_destroy(h)
+            self._handle = None
+
+
+_PLAN_CACHE = OrderedDict()
+_PLAN_CACHE_SIZE = 2048
+
+
+def abelian_permute_plan(
+    shape,
+    leg_charges,
+    signature,
+    old_nrr,
+    old_block_irreps,
+    old_offsets,
+    old_layout_f,
+    axes,
+    new_nrr,
+    *,
+    old_dense=False,
+    new_dense=False,
+):
+    """Cached native permute plan (keyed on structure only, never on data)."""
+    shape = np.ascontiguousarray(shape, dtype=np.int64)
+    sig = np.ascontiguousarray(signature, dtype=np.uint8)
+    irr = np.ascontiguousarray(old_block_irreps, dtype=np.int8)
+    offs = None if old_offsets is None else np.ascontiguousarray(old_offsets, dtype=np.int64)
+    ax = np.ascontiguousarray(axes, dtype=np.int32)
+    charges = [np.ascontiguousarray(c, dtype=np.int8) for c in leg_charges]
+    key = (
+        shape.tobytes(),
+        b"|".join(c.tobytes() for c in charges),
+        sig.tobytes(),
+        int(old_nrr),
+        irr.tobytes(),
+        None if offs is None else offs.tobytes(),
+        bool(old_layout_f),
+        ax.tobytes(),
+        int(new_nrr),
+        bool(old_dense),
+        bool(new_dense),
+    )
+    plan = _PLAN_CACHE.get(key)
+    if plan is not None:
+        _PLAN_CACHE.move_to_end(key)
+        return plan
+    ptrs = (_lib.c_i8p * len(charges))(*[_lib.i8p(c) for c in charges])
+    handle = ctypes.c_void_p()
+    check(
+        lib.fs_abelian_permute_plan_create(
+            len(charges),
+            _lib.i64p(shape),
+            ptrs,
+            _lib.u8p(sig),
+            int(old_nrr),
+            int(irr.size),
+            _lib.i8p(irr),
+            None if offs is None else _lib.i64p(offs),
+            int(bool(old_layout_f)),
+            int(bool(old_dense)),
+            _lib.i32p(ax),
+            int(new_nrr),
+            int(bool(new_dense)),
+            ctypes.byref(handle),
+        )
+    )
+    plan = PermutePlan(handle.value)
+    _PLAN_CACHE[key] = plan
+    if len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+        _PLAN_CACHE.popitem(last=False)
+    return plan
+
+
+def permute(plan, src, dtype=DTYPE):
+    out = empty(plan.new_ncoeff, dtype=dtype)
+    if plan.new_ncoeff:
+        plan.execute(src, out)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# grouped GEMM
+# ---------------------------------------------------------------------------
+def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
+    """groups: sequence of 11-tuples (m, n, k, a_off, a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld)."""
+    n = len(groups)
+    if n == 0:
+        return
+    arr = (_lib.GemmGroup * n)(*[_lib.GemmGroup(*map(int, g)) for g in groups])
+    fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
+    check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
+
+
+# ---------------------------------------------------------------------------
+# batched SVD
+# ---------------------------------------------------------------------------
+SVD_MAX_MIN_DIM = 1024
+
+
+def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
+    """descs: sequence of 9-tuples (m, n, a_off, a_rs, a_cs, u_off, s_off, vt_off, work_off)."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.SvdDesc * n)(*[_lib.SvdDesc(*map(int, d)) for d in descs])
+    check(
+        lib.fs_batched_svd_f64(
+            n, arr, _ptr(A), _ptr(U), _ptr(S), _ptr(VT), _ptr(work), int(max_sweeps), _stream()
+        )
+    )
+
+
+# ---------------------------------------------------------------------------
+# elementwise helpers
+# ---------------------------------------------------------------------------
+def weighted_norm2(x, seg_off, weights):
+    """Device scalar tensor sum_b w_b ||x[seg_b]||^2 (segments contiguous, ascending)."""
+    seg = np.ascontiguousarray(seg_off, dtype=np.int64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    out = empty(1)
+    partial = empty(_lib.FS_NORM_PARTIALS)
+    check(
+        lib.fs_weighted_norm2_f64(
+            int(w.size), _lib.i64p(seg), _lib.dblp(w), _ptr(x) if x.numel() else None,
+            _ptr(partial), _ptr(out), _stream(),
+        )
+    )
+    return out
+
+
+def scale_(x, alpha):
+    if x.numel():
+        check(lib.fs_scale_f64(int(x.numel()), float(alpha), _ptr(x), _stream()))
+
+
+def scale_rsqrt_dev_(x, d):
+    if x.numel():
+        check(lib.fs_scale_rsqrt_dev_f64(int(x.numel()), _ptr(d), _ptr(x), _stream()))
+
+
+def axpby_(alpha, x, beta, y):
+    if y.numel():
+        check(lib.fs_axpby_f64(int(y.numel()), float(alpha), _ptr(x), float(beta), _ptr(y), _stream()))
+
+
+def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
+    """descs: sequence of 13-tuples
+    (rows, cols, s_off, s_rs, s_cs, d_off, d_rs, d_cs, rw_off, cw_off, ci_off, alpha, beta)."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.Copy2dDesc * n)(
+        *[_lib.Copy2dDesc(*(int(v) for v in d[:11]), float(d[11]), float(d[12])) for d in descs]
+    )
+    check(
+        lib.fs_batched_copy2d_f64(
+            n, arr, _ptr(src), _ptr(dst), _ptr(row_w), _ptr(col_w), _ptr(col_idx), _stream()
+        )
+    )
+
+
+def weighted_trace(x, offs, ns, lds, weights):
+    """Device scalar sum_b w_b trace(block_b)."""
+    offs = np.ascontiguousarray(offs, dtype=np.int64)
+    ns = np.ascontiguousarray(ns, dtype=np.int64)
+    lds = np.ascontiguousarray(lds, dtype=np.int64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    out = empty(1)
+    total = zeros(1)
+    for s in range(0, max(len(w), 1), 64):
+        sl = slice(s, s + 64)
+        check(
+            lib.fs_weighted_trace_f64(
+                int(w[sl].size), _lib.i64p(offs[sl].copy()), _lib.i64p(ns[sl].copy()),
+                _lib.i64p(lds[sl].copy()), _lib.dblp(w[sl].copy()),
+                _ptr(x) if x.numel() else None, _ptr(out), _stream(),
+            )
+        )
+        axpby_(1.0, out, 1.0, total)
+    return total
+
+
+def fill_normal_(x, seed):
+    if x.numel():
+        check(lib.fs_fill_normal_f64(int(x.numel()), int(seed) & 0xFFFFFFFFFFFFFFFF, _ptr(x), _stream()))
+
+
+def to_f32(x):
+    y = empty(x.numel(), dtype=torch.float32)
+    if x.numel():
+        check(lib.fs_convert_f64_to_f32(int(x.numel()), _ptr(x), _ptr(y), _stream()))
+    return y
+
+
+def to_f64(x):
+    y = empty(x.numel())
+    if x.numel():
+        check(lib.fs_convert_f32_to_f64(int(x.numel()), _ptr(x), _ptr(y), _stream()))
+    return y

@@ -1,0 +1,233 @@
+// Batched FP64 SVD by one-sided (Hestenes) Jacobi, one workgroup per matrix.
+//
+// X (p x q, column-major, p = max(m,n), q = min(m,n)) is A or A^T. Pairs of
+// columns are orthogonalised with the Rutishauser rotation; a round-robin
+// tournament gives q/2 disjoint pairs per round so the 4 wavefronts of the
+// workgroup rotate different pairs concurrently (each pair: one wavefront,
+// lanes striding over the column length). Sweeps repeat until no pair is
+// rotated. One-sided Jacobi computes small singular values to high relative
+// accuracy, which the s^-1/2 projector scaling of CTMRG relies on.
+// Singular values are written sorted non-increasing like LAPACK gesdd
+// (robust_svd, misc_tools/svd_tools.py:116-151); singular-vector signs are
+// not canonical (neither are LAPACK's), tests compare gauge-invariant data.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fs_hip.h"
+
+namespace {
+
+constexpr int kMaxMat = 24;
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+
+struct SvdDev {
+  int32_t m, n;
+  int32_t a_rs, a_cs;
+  int64_t a_off, u_off, s_off, vt_off, work_off;
+};
+
+struct SvdBatch {
+  const double *A;
+  double *U, *S, *VT, *work;
+  int32_t nmat, max_sweeps;
+  SvdDev d[kMaxMat];
+};
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
+  const SvdDev D = bt.d[blockIdx.x];
+  const int m = D.m, n = D.n;
+  const bool tr = m < n;  // work on A^T when A is wide
+  const int p = tr ? n : m;
+  const int q = tr ? m : n;
+  double *X = bt.work + D.work_off;             // p x q, column-major
+  double *V = X + static_cast<int64_t>(p) * q;  // q x q, column-major
+  const double *A = bt.A + D.a_off;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+
+  // load X = A or A^T, V = I
+  for (int64_t e = tid; e < static_cast<int64_t>(p) * q; e += kThreads) {
+    const int col = static_cast<int>(e / p);
+    const int row = static_cast<int>(e - static_cast<int64_t>(col) * p);
+    const int i = tr ? col : row;  // row index in A
+    const int j = tr ? row : col;  // col index in A
+    X[e] = A[static_cast<int64_t>(i) * D.a_rs + static_cast<int64_t>(j) * D.a_cs];
+  }
+  for (int64_t e = tid; e < static_cast<int64_t>(q) * q; e += kThreads) {
+    const int col = static_cast<int>(e / q);
+    const int row = static_cast<int>(e - static_cast<int64_t>(col) * q);
+    V[e] = (row == col) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+
+  __shared__ int rotated;
+  const int Q = (q + 1) & ~1;  // even number of players (one dummy if q odd)
+  const double tol = 4.0e-16;
+  for (int sweep = 0; sweep < bt.max_sweeps && q > 1; ++sweep) {
+    if (tid == 0) rotated = 0;
+    __syncthreads();
+    int any = 0;
+    for (int r = 0; r < Q - 1; ++r) {
+      // circle method: player Q-1 fixed, others rotate
+      for (int k = wave; k < Q / 2; k += kWaves) {
+        int i, j;
+        if (k == 0) {
+          i = Q - 1;
+          j = r;
+        } else {
+          i = (r + k) % (Q - 1);
+          j = (r - k + (Q - 1)) % (Q - 1);
+        }
+        if (i >= q || j >= q) continue;
+        if (i > j) { const int s = i; i = j; j = s; }
+        double *xi = X + static_cast<int64_t>(i) * p;
+        double *xj = X + static_cast<int64_t>(j) * p;
+        double a = 0.0, b = 0.0, g = 0.0;
+        for (int t = lane; t < p; t += 64) {
+          const double u = xi[t], v = xj[t];
+          a += u * u;
+          b += v * v;
+          g += u * v;
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        g = wave_sum(g);
+        if (g != 0.0 && fabs(g) > tol * sqrt(a) * sqrt(b)) {
+          const double zeta = (b - a) / (2.0 * g);
+          const double t_ = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          const double c = 1.0 / sqrt(1.0 + t_ * t_);
+          const double s = c * t_;
+          for (int t = lane; t < p; t += 64) {
+            const double u = xi[t], v = xj[t];
+            xi[t] = c * u - s * v;
+            xj[t] = s * u + c * v;
+          }
+          double *vi = V + static_cast<int64_t>(i) * q;
+          double *vj = V + static_cast<int64_t>(j) * q;
+          for (int t = lane; t < q; t += 64) {
+            const double u = vi[t], v = vj[t];
+            vi[t] = c * u - s * v;
+            vj[t] = s * u + c * v;
+          }
+          any = 1;
+        }
+      }
+      __syncthreads();
+    }
+    if (any && lane == 0) atomicOr(&rotated, 1);
+    __syncthreads();
+    const int done = (rotated == 0);
+    __syncthreads();
+    if (done) break;
+  }
+
+  // singular values and their rank (descending, ties by index)
+  __shared__ double sv[1024];
+  const bool lds_sv = q <= 1024;
+  for (int k = wave; k < q; k += kWaves) {
+    const double *xk = X + static_cast<int64_t>(k) * p;
+    double a = 0.0;
+    for (int t = lane; t < p; t += 64) a += xk[t] * xk[t];
+    a = wave_sum(a);
+    if (lane == 0) {
+      if (lds_sv) sv[k] = sqrt(a);
+      bt.S[D.s_off + k] = sqrt(a);  // unsorted scratch, overwritten below
+    }
+  }
+  __syncthreads();
+  const double *svp = lds_sv ? sv : (bt.S + D.s_off);
+  // each wave handles columns k: compute rank, write outputs
+  double *U = bt.U + D.u_off;
+  double *VT = bt.VT + D.vt_off;
+  for (int k = wave; k < q; k += kWaves) {
+    const double sk = svp[k];
+    int rank = 0;
+    for (int l = lane; l < q; l += 64) {
+      const double sl = svp[l];
+      rank += (sl > sk || (sl == sk && l < k)) ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 64);
+    const double inv = sk > 0.0 ? 1.0 / sk : 0.0;
+    const double *xk = X + static_cast<int64_t>(k) * p;
+    const double *vk = V + static_cast<int64_t>(k) * q;
+    if (!tr) {
+      // A = U S V^T: U[:, rank] = x_k / s_k (m = p rows), VT[rank, :] = v_k
+      for (int t = lane; t < p; t += 64) U[static_cast<int64_t>(t) * q + rank] = xk[t] * inv;
+      for (int t = lane; t < q; t += 64) VT[static_cast<int64_t>(rank) * n + t] = vk[t];
+    } else {
+      // A^T = X: A = V S (x/s)^T: U[:, rank] = v_k (m = q rows), VT[rank, :] = x_k / s_k
+      for (int t = lane; t < q; t += 64) U[static_cast<int64_t>(t) * q + rank] = vk[t];
+      for (int t = lane; t < p; t += 64) VT[static_cast<int64_t>(rank) * n + t] = xk[t] * inv;
+    }
+  }
+  __syncthreads();
+  // sorted singular values (all waves finished reading the unsorted scratch)
+  if (lds_sv) {
+    for (int k = tid; k < q; k += kThreads) {
+      const double sk = sv[k];
+      int rank = 0;
+      for (int l = 0; l < q; ++l) {
+        const double sl = sv[l];
+        rank += (sl > sk || (sl == sk && l < k)) ? 1 : 0;
+      }
+      bt.S[D.s_off + rank] = sk;
+    }
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int fs_batched_svd_f64(int nmat, const fs_svd_desc *descs, const double *A, double *U,
+                       double *S, double *VT, double *work, int max_sweeps, void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(nmat >= 0, "batched svd: negative count");
+    FS_CHECK_ARG(nmat == 0 || (descs && A && U && S && VT && work), "batched svd: null argument");
+    FS_CHECK_ARG(max_sweeps >= 1, "batched svd: max_sweeps must be >= 1");
+    int i0 = 0;
+    while (i0 < nmat) {
+      SvdBatch bt;
+      std::memset(&bt, 0, sizeof(bt));
+      bt.A = A;
+      bt.U = U;
+      bt.S = S;
+      bt.VT = VT;
+      bt.work = work;
+      bt.max_sweeps = max_sweeps;
+      int nb = 0;
+      for (; i0 < nmat && nb < kMaxMat; ++i0) {
+        const fs_svd_desc &q = descs[i0];
+        FS_CHECK_ARG(q.m >= 1 && q.n >= 1, "batched svd: empty matrix");
+        FS_CHECK_ARG(q.m < (1ll << 31) && q.n < (1ll << 31), "batched svd: matrix too large");
+        FS_CHECK_ARG(std::min(q.m, q.n) <= 1024, "batched svd: min(m, n) must be <= 1024");
+        SvdDev &d = bt.d[nb++];
+        d.m = static_cast<int32_t>(q.m);
+        d.n = static_cast<int32_t>(q.n);
+        d.a_rs = static_cast<int32_t>(q.a_rs);
+        d.a_cs = static_cast<int32_t>(q.a_cs);
+        d.a_off = q.a_off;
+        d.u_off = q.u_off;
+        d.s_off = q.s_off;
+        d.vt_off = q.vt_off;
+        d.work_off = q.work_off;
+      }
+      bt.nmat = nb;
+      hipLaunchKernelGGL(jacobi_svd_kernel, dim3(nb), dim3(kThreads), 0, fs::as_stream(stream), bt);
+      FS_LAUNCH_CHECK("jacobi_svd_kernel");
+    }
+    return FS_OK;
+  });
+}
+
+} // extern "C"

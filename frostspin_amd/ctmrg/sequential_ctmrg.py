@@ -1,0 +1,128 @@
+"""Sequential CTMRG schedule (ctmrg/sequential_ctmrg.py:19-211).
+
+One iterate() = up, right, down, left moves. Each move builds the projectors
+of every inequivalent site from the four enlarged corners (freeing the two
+corners it renormalizes), then renormalizes C, T, C of every site, then swaps
+the new tensors in. The move table below fixes, for each direction, which
+corners feed construct_projectors, which tensors are renormalized and the
+coordinate offsets, exactly as in the reference so environments agree leg by
+leg.
+"""
+
+from .abstract_ctmrg import AbstractCTMRG
+from .ctm_renormalize import (
+    construct_projectors,
+    renormalize_C1_left,
+    renormalize_C1_up,
+    renormalize_C2_right,
+    renormalize_C2_up,
+    renormalize_C3_down,
+    renormalize_C3_right,
+    renormalize_C4_down,
+    renormalize_C4_left,
+    renormalize_T1,
+    renormalize_T2,
+    renormalize_T3,
+    renormalize_T4,
+)
+
+
+class SequentialCTMRG(AbstractCTMRG):
+    def __repr__(self):
+        return (
+            f"{self.symmetry()} symmetric SequentialCTMRG with Dmax = {self.Dmax}"
+            f" and chi_target = {self.chi_target}"
+        )
+
+    def iterate(self):
+        self.up_move()
+        self.right_move()
+        self.down_move()
+        self.left_move()
+
+    def _projectors(self, c1, c2, c3, c4, last):
+        return construct_projectors(
+            c1, c2, c3, c4, self.chi_target, last,
+            block_chi_ratio=self.block_chi_ratio, ncv_ratio=self.ncv_ratio,
+            rtol=self.cutoff, degen_ratio=self.degen_ratio,
+        )
+
+    def up_move(self):
+        e = self._env
+        for x, y in self._site_coords:
+            P, Pt = self._projectors(
+                self.construct_enlarged_dr(x, y),
+                self.construct_enlarged_ur(x, y, free_memory=True),
+                self.construct_enlarged_ul(x, y, free_memory=True),
+                self.construct_enlarged_dl(x, y),
+                e.get_C2(x + 2, y + 1),
+            )
+            e.store_up_projectors(x + 2, y, P, Pt)
+        for x, y in self._site_coords:
+            P = e.get_up_P(x + 1, y)
+            Pt = e.get_up_Pt(x, y)
+            nC1 = renormalize_C1_up(e.get_C1(x, y), e.get_T4(x, y + 1), P)
+            nT1 = renormalize_T1(Pt, e.get_T1(x, y), e.get_A(x, y + 1), P)
+            nC2 = renormalize_C2_up(e.get_C2(x, y), e.get_T2(x, y + 1), Pt)
+            e.store_renormalized_tensors(x, y + 1, nC1, nT1, nC2)
+        e.set_renormalized_tensors_up()
+
+    def right_move(self):
+        e = self._env
+        for x, y in self._site_coords:
+            P, Pt = self._projectors(
+                self.construct_enlarged_dl(x, y),
+                self.construct_enlarged_dr(x, y, free_memory=True),
+                self.construct_enlarged_ur(x, y, free_memory=True),
+                self.construct_enlarged_ul(x, y),
+                e.get_C3(x + 2, y + 2).transpose(),
+            )
+            e.store_right_projectors(x + 3, y + 2, P, Pt)
+        for x, y in self._site_coords:
+            P = e.get_right_P(x, y + 1)
+            Pt = e.get_right_Pt(x, y)
+            nC2 = renormalize_C2_right(e.get_C2(x, y), e.get_T1(x - 1, y), P)
+            nT2 = renormalize_T2(Pt, e.get_T2(x, y), e.get_A(x - 1, y), P)
+            nC3 = renormalize_C3_right(e.get_C3(x, y), e.get_T3(x - 1, y), Pt)
+            e.store_renormalized_tensors(x - 1, y, nC2, nT2, nC3)
+        e.set_renormalized_tensors_right()
+
+    def down_move(self):
+        e = self._env
+        for x, y in self._site_coords:
+            P, Pt = self._projectors(
+                self.construct_enlarged_ul(x, y),
+                self.construct_enlarged_dl(x, y, free_memory=True),
+                self.construct_enlarged_dr(x, y, free_memory=True),
+                self.construct_enlarged_ur(x, y),
+                e.get_C4(x + 1, y + 2),
+            )
+            e.store_down_projectors(x + 1, y + 3, P, Pt)
+        for x, y in self._site_coords:
+            P = e.get_down_P(x - 1, y)
+            Pt = e.get_down_Pt(x, y)
+            nC3 = renormalize_C3_down(e.get_C3(x, y), e.get_T2(x, y - 1), P)
+            nT3 = renormalize_T3(Pt, e.get_T3(x, y), e.get_A(x, y - 1), P)
+            nC4 = renormalize_C4_down(e.get_C4(x, y), e.get_T4(x, y - 1), Pt)
+            e.store_renormalized_tensors(x, y - 1, nC3, nT3, nC4)
+        e.set_renormalized_tensors_down()
+
+    def left_move(self):
+        e = self._env
+        for x, y in self._site_coords:
+            P, Pt = self._projectors(
+                self.construct_enlarged_ur(x, y),
+                self.construct_enlarged_ul(x, y, free_memory=True),
+                self.construct_enlarged_dl(x, y, free_memory=True),
+                self.construct_enlarged_dr(x, y),
+                e.get_C1(x + 1, y + 1),
+            )
+            e.store_left_projectors(x, y + 1, P, Pt)
+        for x, y in self._site_coords:
+            P = e.get_left_P(x, y - 1)
+            Pt = e.get_left_Pt(x, y)
+            nC4 = renormalize_C4_left(e.get_C4(x, y), e.get_T3(x + 1, y), P)
+            nT4 = renormalize_T4(Pt, e.get_T4(x, y), e.get_A(x + 1, y), P)
+            nC1 = renormalize_C1_left(e.get_C1(x, y), e.get_T1(x + 1, y), Pt)
+            e.store_renormalized_tensors(x + 1, y, nC4, nT4, nC1)
+        e.set_renormalized_tensors_left()

@@ -1,0 +1,147 @@
+"""Batched linear algebra on lists of device matrices (one launch per step).
+
+A `Mats` is a list of matrices living in device buffers, each described by
+(buffer, offset, rows, cols, row stride, col stride). All blocks of a
+symmetry-sector loop go through the same launch: one grouped GEMM, one batched
+Jacobi SVD. This is the device counterpart of the per-sector Python loops of
+symmetric_tensor.py:857-864 and ctm_renormalize.py:253-285.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _ops
+
+
+@dataclass
+class Mats:
+    buf: torch.Tensor
+    off: np.ndarray
+    rows: np.ndarray
+    cols: np.ndarray
+    rs: np.ndarray
+    cs: np.ndarray
+
+    def __len__(self):
+        return len(self.off)
+
+    @classmethod
+    def packed(cls, rows, cols, buf=None):
+        """Row-major matrices packed back to back in a (new) buffer."""
+        rows = np.asarray(rows, dtype=np.int64).reshape(-1)
+        cols = np.asarray(cols, dtype=np.int64).reshape(-1)
+        sizes = rows * cols
+        off = np.zeros_like(sizes)
+        if sizes.size > 1:
+            off[1:] = np.cumsum(sizes[:-1])
+        if buf is None:
+            buf = _ops.empty(int(sizes.sum()))
+        return cls(buf, off, rows, cols, cols.copy(), np.ones_like(cols))
+
+    @classmethod
+    def from_tensor_blocks(cls, st, ids):
+        ids = list(ids)
+        off = np.array([st._boffs[b] for b in ids], dtype=np.int64)
+        rows = np.array([st._bshapes[b, 0] for b in ids], dtype=np.int64)
+        cols = np.array([st._bshapes[b, 1] for b in ids], dtype=np.int64)
+        strides = np.array([st._strides(b) for b in ids], dtype=np.int64).reshape(-1, 2)
+        return cls(st._data, off, rows, cols, strides[:, 0].copy(), strides[:, 1].copy())
+
+    def T(self):
+        """Transposed views (no copy)."""
+        return Mats(self.buf, self.off, self.cols, self.rows, self.cs, self.rs)
+
+    def select(self, idx):
+        idx = list(idx)
+        return Mats(self.buf, self.off[idx], self.rows[idx], self.cols[idx], self.rs[idx], self.cs[idx])
+
+
+def gemm(A, B, *, alpha=1.0, split=False):
+    """C_i = alpha * A_i @ B_i for all i in one grouped launch; C packed row-major."""
+    assert len(A) == len(B)
+    assert np.array_equal(A.cols, B.rows)
+    C = Mats.packed(A.rows, B.cols)
+    groups = [
+        (A.rows[i], B.cols[i], A.cols[i], A.off[i], A.rs[i], A.cs[i], B.off[i], B.rs[i], B.cs[i],
+         C.off[i], C.cols[i])
+        for i in range(len(A))
+    ]
+    if A.buf is B.buf or A.buf.data_ptr() == B.buf.data_ptr():
+        _ops.grouped_gemm(groups, A.buf, A.buf, C.buf, alpha=alpha, split=split)
+    else:
+        _ops.grouped_gemm(groups, A.buf, B.buf, C.buf, alpha=alpha, split=split)
+    return C
+
+
+def scale_slice(A, nrows, ncols, *, row_w=None, col_w=None, w_off=None):
+    """Copy the leading (nrows_i x ncols_i) corner of each A_i, optionally scaling rows
+    or columns by a device weight vector at w_off[i]; output packed row-major."""
+    out = Mats.packed(nrows, ncols)
+    descs = []
+    for i in range(len(A)):
+        wo = 0 if w_off is None else int(w_off[i])
+        descs.append(
+            (nrows[i], ncols[i], A.off[i], A.rs[i], A.cs[i], out.off[i], out.cols[i], 1,
+             wo if row_w is not None else 0, wo if col_w is not None else 0, 0, 1.0, 0.0)
+        )
+    _ops.copy2d(descs, A.buf, out.buf, row_w=row_w, col_w=col_w)
+    return out
+
+
+def svd(A, *, max_sweeps=60):
+    """Thin SVD of every A_i (batched one-sided Jacobi). Returns U (packed m x q),
+    S (device, packed), host singular values list, VT (packed q x n)."""
+    q = np.minimum(A.rows, A.cols)
+    if q.size and q.max() > _ops.SVD_MAX_MIN_DIM:
+        msg = "dense device SVD limited to min(m, n) <= 1024; use rsvd"
+        raise NotImplementedError(msg)
+    U = Mats.packed(A.rows, q)
+    VT = Mats.packed(q, A.cols)
+    s_off = np.zeros_like(q)
+    if q.size > 1:
+        s_off[1:] = np.cumsum(q[:-1])
+    w_size = q * (np.maximum(A.rows, A.cols) + q)
+    w_off = np.zeros_like(w_size)
+    if q.size > 1:
+        w_off[1:] = np.cumsum(w_size[:-1])
+    S = _ops.empty(int(q.sum()))
+    work = _ops.empty(int(w_size.sum()))
+    descs = [
+        (A.rows[i], A.cols[i], A.off[i], A.rs[i], A.cs[i], U.off[i], s_off[i], VT.off[i], w_off[i])
+        for i in range(len(A))
+    ]
+    _ops.batched_svd(descs, A.buf, U.buf, S, VT.buf, work, max_sweeps=max_sweeps)
+    s_host = _ops.download(S) if S.numel() else np.zeros((0,))
+    s_list = [s_host[s_off[i] : s_off[i] + q[i]] for i in range(len(A))]
+    return U, S, s_off, s_list, VT
+
+
+def _orthonormal_columns(Y):
+    """Orthonormal basis of the column space of each Y_i (U factor of its SVD)."""
+    U, _, _, _, _ = svd(Y)
+    return U
+
+
+def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):
+    """Randomized truncated SVD (range finder + power iterations, Halko et al.,
+    SIAM Rev. 53 (2011) Alg. 4.4/5.1) of every A_i, keeping l_i = k_i + p_i
+    vectors. Returns U (m x l), S, s_off, s_list, VT (l x n) like svd()."""
+    ks = np.asarray(ks, dtype=np.int64)
+    mins = np.minimum(A.rows, A.cols)
+    p = np.maximum(10, ks // 2) if oversampling is None else np.full_like(ks, oversampling)
+    ls = np.minimum(ks + p, mins)
+    omega = Mats.packed(A.cols, ls)
+    _ops.fill_normal_(omega.buf, seed)
+    Y = gemm(A, omega)
+    for _ in range(power_iters):
+        Q = _orthonormal_columns(Y)
+        Z = gemm(A.T(), Q)
+        Qz = _orthonormal_columns(Z)
+        Y = gemm(A, Qz)
+    Q = _orthonormal_columns(Y)
+    B = gemm(Q.T(), A)  # l x n
+    Ub, S, s_off, s_list, VT = svd(B)
+    U = gemm(Q, Ub)  # m x l
+    return U, S, s_off, s_list, VT

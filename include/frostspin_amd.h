@@ -1,0 +1,272 @@
+/*
+ * frostspin_amd C-ABI: block-sparse SymmetricTensor engine for MI355X (gfx950).
+ *
+ * Every entry point returns an int status (FS_OK on success); on failure the
+ * message is available from fs_last_error() (thread-local). No C++ exception
+ * crosses this boundary. Device pointers are plain `double*` into memory the
+ * caller owns; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * The library never allocates or frees caller tensors. Plans own the device
+ * tables they upload and release them in fs_permute_plan_destroy.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * file:line relative to the frostspin source tree.
+ */
+#ifndef FROSTSPIN_AMD_H
+#define FROSTSPIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_OK 0
+#define FS_ERR_INVALID 1
+#define FS_ERR_HIP 2
+#define FS_ERR_NOMEM 3
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* Last error message of the calling thread ("" if none). */
+const char *fs_last_error(void);
+
+/* ABI version of this header (bumped on any signature change). */
+int fs_abi_version(void);
+
+/* Number of visible HIP devices (0 with no GPU; never fails on a CPU host). */
+int fs_device_count(int *count);
+
+/* ------------------------------------------------------------------------- */
+/* Host planner (no GPU needed)                                              */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * U(1) charge fusion: out = Kronecker sum over legs of (flip[l] ? -c_l : c_l),
+ * row-major (last leg fastest), with int8 wrap-around (Z2 as {0,-128}).
+ * out has prod(sizes) entries.
+ * Replaces _numba_combine_U1 / U1SymmetricTensor.combine_representations
+ * (symmetric_tensor/u1_symmetric_tensor.py:8-19, 35-40).
+ */
+int fs_u1_fuse(int nlegs, const int8_t *const *legs, const int64_t *sizes,
+               const uint8_t *flip, int8_t *out);
+
+/*
+ * Greedy multiplet-aware truncation over blockwise non-increasing values.
+ * values[b] has sizes[b] entries; dims[b] is the irrep dimension counted per
+ * kept value; cutoff = atol + rtol * max value (already combined by caller).
+ * cuts[b] receives the number of values kept in block b.
+ * Replaces _numba_find_block_cuts (misc_tools/svd_tools.py:57-74).
+ */
+int fs_find_block_cuts(int nblocks, const double *const *values,
+                       const int64_t *sizes, int64_t cut_target,
+                       const int64_t *dims, double cutoff, double degen_ratio,
+                       int64_t *cuts);
+
+/* ------------------------------------------------------------------------- */
+/* Abelian permute plan                                                      */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fs_permute_plan fs_permute_plan;
+
+/*
+ * Build the index maps of an abelian (U(1), Z2-as-int8 or trivial) permute.
+ *   ndim, shape[ndim]       old tensor shape (leg dimensions)
+ *   leg_charges[ndim]       int8 charges of each old leg (shape[l] entries)
+ *   signature[ndim]         old signature (1 = flipped charge)
+ *   old_nrr                 number of old row legs
+ *   old_nblocks, old_block_irreps   blocks present in the old tensor
+ *                           (sorted; missing allowed blocks read as zero)
+ *   old_block_offsets       start of each old block in the source buffer
+ *                           (NULL: packed in block order)
+ *   old_layout_f            1 if old blocks are stored column-major
+ *   old_dense               1: old data is one dense row-major matrix
+ *                           (prod row dims x prod col dims); charges only
+ *                           shape the new side (from_array)
+ *   axes[ndim], new_nrr     permutation (new leg i = old leg axes[i])
+ *   new_dense               1: write one dense row-major matrix (toarray)
+ * Replaces AbelianSymmetricTensor._permute_data + _numba_abelian_transpose
+ * (symmetric_tensor/abelian_symmetric_tensor.py:395-430, 103-250),
+ * _blocks_from_dense/_numba_reduce_to_blocks (:363-373, 19-40) and
+ * _tomatrix/_numba_blocks_to_array (:375-383, 63-75).
+ */
+int fs_abelian_permute_plan_create(int ndim, const int64_t *shape,
+                                   const int8_t *const *leg_charges,
+                                   const uint8_t *signature, int old_nrr,
+                                   int old_nblocks,
+                                   const int8_t *old_block_irreps,
+                                   const int64_t *old_block_offsets,
+                                   int old_layout_f, int old_dense,
+                                   const int32_t *axes, int new_nrr,
+                                   int new_dense, fs_permute_plan **plan);
+
+/* Number of output blocks and stored coefficients. */
+int fs_permute_plan_info(const fs_permute_plan *plan, int64_t *new_nblocks,
+                         int64_t *new_ncoeff, int64_t *old_ncoeff);
+
+/* Output block irreps (sorted) and shapes (nblocks x 2, row-major). */
+int fs_permute_plan_blocks(const fs_permute_plan *plan, int8_t *irreps,
+                           int64_t *shapes);
+
+/*
+ * Host copies of the planner tables (test/inspection): sizes first, then data.
+ * sizes[0..4] = n_old_rows, n_old_cols, n_new_rows_total, n_new_cols_total,
+ * n_old_blocks. Any data pointer may be NULL. old_row_block/old_row_pos:
+ * n_old_rows; old_col_block/old_col_pos: n_old_cols; row_a (2 per new
+ * in-block row: old-row and old-col contributions), col_b (2 per new in-block
+ * col); old_off/old_rs/old_cs: start, row stride and column stride of each
+ * old block in the source buffer.
+ */
+int fs_permute_plan_tables(const fs_permute_plan *plan, int64_t *sizes,
+                           int32_t *old_row_block, int32_t *old_row_pos,
+                           int32_t *old_col_block, int32_t *old_col_pos,
+                           int32_t *row_a, int32_t *col_b, int64_t *old_off,
+                           int64_t *old_rs, int64_t *old_cs);
+
+/* Gather old blocks (packed, in block order) into the new packed blocks. */
+int fs_permute_plan_execute_f64(fs_permute_plan *plan, const double *src,
+                                double *dst, void *stream);
+
+/* Same map, FP32 storage (mixed-precision form). */
+int fs_permute_plan_execute_f32(fs_permute_plan *plan, const float *src,
+                                float *dst, void *stream);
+
+int fs_permute_plan_destroy(fs_permute_plan *plan);
+
+/* ------------------------------------------------------------------------- */
+/* Grouped GEMM over uneven blocks                                           */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * One group: C[m x n] = alpha * A[m x k] * B[k x n] + beta * C.
+ * Element (i,j) of A is at A + a_off + i*a_rs + j*a_cs (same for B);
+ * C is row-major with leading dimension c_ld (C + c_off + i*c_ld + j).
+ * Replaces the per-block numpy matmul loop of SymmetricTensor.__matmul__
+ * (symmetric_tensor/symmetric_tensor.py:453-489) and the sector products of
+ * construct_projectors (ctmrg/ctm_renormalize.py:92-94, 122-123).
+ */
+typedef struct fs_gemm_group {
+  int64_t m, n, k;
+  int64_t a_off, a_rs, a_cs;
+  int64_t b_off, b_rs, b_cs;
+  int64_t c_off, c_ld;
+} fs_gemm_group;
+
+/* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 accumulation. */
+int fs_grouped_gemm_f64(int ngroups, const fs_gemm_group *groups,
+                        const double *A, const double *B, double *C,
+                        double alpha, double beta, void *stream);
+
+/*
+ * Compensated FP32 form: A and B are split into hi/lo FP32 parts on the fly
+ * (x = hi + lo), the three products hi*hi + hi*lo + lo*hi run on FP32 MFMA
+ * with FP32 accumulation per K chunk and FP64 accumulation across chunks.
+ * Inputs and outputs stay FP64.
+ */
+int fs_grouped_gemm_f64_split(int ngroups, const fs_gemm_group *groups,
+                              const double *A, const double *B, double *C,
+                              double alpha, double beta, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Batched FP64 SVD (one-sided Jacobi)                                       */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * A_b (m x n, element (i,j) at A + a_off + i*a_rs + j*a_cs) = U S V^T with
+ * q = min(m,n): U (m x q) row-major at U + u_off, S (q) at S + s_off sorted
+ * non-increasing, VT (q x n) row-major at VT + vt_off. work must hold
+ * work_off .. work_off + q*(max(m,n)+q) doubles per matrix.
+ * Replaces robust_svd (misc_tools/svd_tools.py:116-151) per block in
+ * SymmetricTensor.svd/truncated_svd (symmetric_tensor.py:691-710, 835-877)
+ * and construct_projectors (ctmrg/ctm_renormalize.py:96-103).
+ */
+typedef struct fs_svd_desc {
+  int64_t m, n;
+  int64_t a_off, a_rs, a_cs;
+  int64_t u_off, s_off, vt_off, work_off;
+} fs_svd_desc;
+
+int fs_batched_svd_f64(int nmat, const fs_svd_desc *descs, const double *A,
+                       double *U, double *S, double *VT, double *work,
+                       int max_sweeps, void *stream);
+
+/* ------------------------------------------------------------------------- */
+/* Elementwise / reduction helpers on packed block buffers                   */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Per-segment weighted sum of squares: out[0] = sum_b w[b] * ||x[off_b:off_b+len_b]||^2
+ * computed in a fixed order (deterministic for a given segment list).
+ * seg_off (host, nseg+1 entries) and w (host, nseg) are copied into kernel
+ * arguments. partial is device scratch of FS_NORM_PARTIALS doubles; out is a
+ * device pointer (1 double). Replaces SymmetricTensor.norm
+ * (symmetric_tensor.py:624-631).
+ */
+#define FS_NORM_PARTIALS 1024
+int fs_weighted_norm2_f64(int nseg, const int64_t *seg_off, const double *w,
+                          const double *x, double *partial, double *out,
+                          void *stream);
+
+/* x[i] *= alpha for i < n (in place). Replaces __imul__/__itruediv__
+ * (symmetric_tensor.py:441-451). */
+int fs_scale_f64(int64_t n, double alpha, double *x, void *stream);
+
+/* x[i] *= 1 / sqrt(d[0]) where d is a device scalar (norm^2): normalises a
+ * tensor without a host round trip (nT /= nT.norm(), ctm_renormalize.py:327). */
+int fs_scale_rsqrt_dev_f64(int64_t n, const double *d, double *x, void *stream);
+
+/* y[i] = alpha * x[i] + beta * y[i]. Used by __add__/__sub__ (:349-375). */
+int fs_axpby_f64(int64_t n, double alpha, const double *x, double beta,
+                 double *y, void *stream);
+
+/*
+ * Batched strided 2-D copy with optional scaling, column gather and
+ * accumulation:
+ *   dst[d_off + i*d_rs + j*d_cs] = alpha * src[s_off + i*s_rs + jj*s_cs]
+ *                                  * (row_w ? row_w[rw_off+i] : 1)
+ *                                  * (col_w ? col_w[cw_off+j] : 1)
+ *                                  + beta * dst[...]
+ * with jj = col_idx ? col_idx[ci_off+j] : j. row_w, col_w and col_idx are
+ * device arrays (may be NULL); beta == 0 never reads dst. Used for
+ * transposes, slicing U[:, :k] / V[:k, :], diagonal products (_diagonal_mul,
+ * symmetric_tensor.py:1189-1248), block sums (__add__, :349-372), reordering
+ * eigenvectors (_sparse_eig, :1316-1322) and projector scaling s^-1/2
+ * (ctm_renormalize.py:283-285).
+ */
+typedef struct fs_copy2d_desc {
+  int64_t rows, cols;
+  int64_t s_off, s_rs, s_cs;
+  int64_t d_off, d_rs, d_cs;
+  int64_t rw_off, cw_off, ci_off;
+  double alpha, beta;
+} fs_copy2d_desc;
+
+int fs_batched_copy2d_f64(int ndesc, const fs_copy2d_desc *descs,
+                          const double *src, double *dst, const double *row_w,
+                          const double *col_w, const int64_t *col_idx,
+                          void *stream);
+
+/*
+ * Weighted sum of the diagonals: out[0] = sum_b w[b] * trace(block_b) where
+ * block b is n_b x n_b at off_b with row stride ld_b (trace, :633-637).
+ */
+int fs_weighted_trace_f64(int nseg, const int64_t *off, const int64_t *n,
+                          const int64_t *ld, const double *w, const double *x,
+                          double *out, void *stream);
+
+/*
+ * Deterministic counter-based N(0,1) fill (Box-Muller on a 64-bit hash of
+ * (seed, index)); used for randomized range finders and synthetic inputs.
+ */
+int fs_fill_normal_f64(int64_t n, uint64_t seed, double *x, void *stream);
+
+/* FP64 <-> FP32 conversion for the mixed-precision storage form. */
+int fs_convert_f64_to_f32(int64_t n, const double *x, float *y, void *stream);
+int fs_convert_f32_to_f64(int64_t n, const float *x, double *y, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FROSTSPIN_AMD_H */

@@ -1,0 +1,152 @@
+"""Test-only emulation of the device entry points wrapped by frostspin_amd._ops.
+
+CPU tests install it (fixture `emulated_device`) to run the product's host logic
+(block bookkeeping, offsets, strides, plan tables, GEMM/SVD descriptors) without
+a GPU: every `_ops` device function is replaced by a numpy implementation of the
+kernel's documented semantics, operating on torch CPU tensors. The product never
+imports this module; GPU tests run the real kernels.
+"""
+
+import numpy as np
+import torch
+
+from frostspin_amd import _ops
+
+
+def _np(t):
+    return t.numpy()
+
+
+def _permute_execute(plan, src, dst):
+    t = plan.tables()
+    s = _np(src) if src is not None and src.numel() else np.zeros((0,))
+    d = _np(dst)
+    shapes = plan.shapes
+    row_a = t["row_a"].reshape(-1, 2).astype(np.int64)
+    col_b = t["col_b"].reshape(-1, 2).astype(np.int64)
+    rstart = cstart = off = 0
+    for b in range(shapes.shape[0]):
+        nr, nc = (int(x) for x in shapes[b])
+        ra = row_a[rstart : rstart + nr]
+        cb = col_b[cstart : cstart + nc]
+        R = ra[:, 0][:, None] + cb[:, 0][None, :]
+        C = ra[:, 1][:, None] + cb[:, 1][None, :]
+        ob = t["old_row_block"][R]
+        ok = (ob >= 0) & (ob == t["old_col_block"][C])
+        obc = np.where(ok, ob, 0)
+        src_idx = (
+            t["old_off"][obc]
+            + t["old_row_pos"][R].astype(np.int64) * t["old_rs"][obc]
+            + t["old_col_pos"][C].astype(np.int64) * t["old_cs"][obc]
+        )
+        vals = np.zeros((nr, nc))
+        if s.size:
+            vals[ok] = s[src_idx[ok]]
+        d[off : off + nr * nc] = vals.reshape(-1)
+        rstart += nr
+        cstart += nc
+        off += nr * nc
+
+
+def _view(buf, off, rows, cols, rs, cs):
+    # the same bounds the kernel relies on: every addressed element is in the buffer
+    last = int(off) + (int(rows) - 1) * int(rs) + (int(cols) - 1) * int(cs)
+    assert int(off) >= 0 and min(int(rs), int(cs)) >= 0 and last < buf.size, "out of bounds"
+    return np.lib.stride_tricks.as_strided(
+        buf[int(off) :], shape=(int(rows), int(cols)), strides=(int(rs) * 8, int(cs) * 8)
+    )
+
+
+def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
+    a, b, c = _np(A), _np(B), _np(C)
+    for m, n, k, ao, ars, acs, bo, brs, bcs, co, cld in groups:
+        if m == 0 or n == 0:
+            continue
+        am = _view(a, ao, m, k, ars, acs)
+        bm = _view(b, bo, k, n, brs, bcs)
+        cm = _view(c, co, m, n, cld, 1)
+        res = alpha * (am @ bm)
+        cm[...] = res if beta == 0.0 else res + beta * cm
+
+
+def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
+    a, u, s, vt = _np(A), _np(U), _np(S), _np(VT)
+    for m, n, ao, ars, acs, uo, so, vo, _wo in descs:
+        q = min(m, n)
+        am = _view(a, ao, m, n, ars, acs).copy()
+        uu, ss, vv = np.linalg.svd(am, full_matrices=False)
+        u[uo : uo + m * q] = uu.reshape(-1)
+        s[so : so + q] = ss
+        vt[vo : vo + q * n] = vv.reshape(-1)
+
+
+def weighted_norm2(x, seg_off, weights):
+    xs = _np(x)
+    tot = 0.0
+    for i, w in enumerate(weights):
+        seg = xs[int(seg_off[i]) : int(seg_off[i + 1])]
+        tot += w * float(seg @ seg)
+    return torch.tensor([tot], dtype=torch.float64)
+
+
+def scale_(x, alpha):
+    x.mul_(alpha)
+
+
+def scale_rsqrt_dev_(x, d):
+    x.mul_(1.0 / float(np.sqrt(float(d[0]))))
+
+
+def axpby_(alpha, x, beta, y):
+    if beta == 0.0:
+        y.copy_(alpha * x)
+    else:
+        y.copy_(alpha * x + beta * y)
+
+
+def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
+    s, d = _np(src), _np(dst)
+    rw = None if row_w is None else _np(row_w)
+    cw = None if col_w is None else _np(col_w)
+    ci = None if col_idx is None else _np(col_idx)
+    for rows, cols, so, srs, scs, do, drs, dcs, rwo, cwo, cio, alpha, beta in descs:
+        if rows == 0 or cols == 0:
+            continue
+        i = np.arange(rows)[:, None]
+        j = np.arange(cols)[None, :]
+        jj = j if ci is None else ci[int(cio) + j]
+        v = alpha * s[so + i * srs + jj * scs]
+        if rw is not None:
+            v = v * rw[rwo + i]
+        if cw is not None:
+            v = v * cw[cwo + j]
+        di = do + i * drs + j * dcs
+        d[di] = v if beta == 0.0 else v + beta * d[di]
+
+
+def weighted_trace(x, offs, ns, lds, weights):
+    xs = _np(x)
+    tot = 0.0
+    for o, n, ld, w in zip(offs, ns, lds, weights, strict=True):
+        tot += w * xs[int(o) + np.arange(int(n)) * (int(ld) + 1)].sum()
+    return torch.tensor([tot], dtype=torch.float64)
+
+
+def fill_normal_(x, seed):
+    rng = np.random.default_rng(int(seed))
+    x.copy_(torch.from_numpy(rng.standard_normal(x.numel())))
+
+
+def _execute(self, src, dst):
+    _permute_execute(self, src, dst)
+
+
+def install(monkeypatch):
+    """Route frostspin_amd._ops device calls to the numpy emulation (CPU tensors)."""
+    monkeypatch.setattr(_ops, "device", lambda: torch.device("cpu"))
+    monkeypatch.setattr(_ops.PermutePlan, "execute", _execute)
+    for name in (
+        "grouped_gemm", "batched_svd", "weighted_norm2", "scale_", "scale_rsqrt_dev_",
+        "axpby_", "copy2d", "weighted_trace", "fill_normal_",
+    ):
+        monkeypatch.setattr(_ops, name, globals()[name])

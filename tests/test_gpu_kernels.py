@@ -1,0 +1,148 @@
+"""Native kernels on the GPU against numpy/dense oracles (bit-exact for data
+movement, ~1e-13 relative for FP64 arithmetic)."""
+
+import numpy as np
+import pytest
+import torch
+
+from frostspin_amd import _ops
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return _ops.upload(np.asarray(a, dtype=np.float64))
+
+
+def test_mfma_gemm_layouts_and_uneven_groups(gpu):
+    rng = np.random.default_rng(0)
+    sizes = [(1, 1, 1), (3, 70, 5), (64, 64, 16), (65, 63, 17), (130, 7, 200), (17, 129, 33)]
+    for a_f in (False, True):
+        for b_f in (False, True):
+            A_parts, B_parts, groups, refs = [], [], [], []
+            ao = bo = co = 0
+            for m, n, k in sizes:
+                a = rng.standard_normal((m, k))
+                b = rng.standard_normal((k, n))
+                A_parts.append((a.T if a_f else a).reshape(-1))
+                B_parts.append((b.T if b_f else b).reshape(-1))
+                ars, acs = (1, m) if a_f else (k, 1)
+                brs, bcs = (1, k) if b_f else (n, 1)
+                groups.append((m, n, k, ao, ars, acs, bo, brs, bcs, co, n))
+                refs.append((co, a @ b))
+                ao += m * k
+                bo += k * n
+                co += m * n
+            A = _dev(np.concatenate(A_parts))
+            B = _dev(np.concatenate(B_parts))
+            C = _ops.upload(np.full(co, 7.0))
+            _ops.grouped_gemm(groups, A, B, C, alpha=2.0, beta=0.5)
+            c = _ops.download(C)
+            for (off, ref), (m, n, _) in zip(refs, sizes, strict=True):
+                got = c[off : off + m * n].reshape(m, n)
+                expect = 2.0 * ref + 3.5
+                err = float(np.abs(got - expect).max() / max(1.0, np.abs(expect).max()))
+                assert err <= 1e-12, f"group {(m, n)} a_f={a_f} b_f={b_f} rel err {err:.2e}"
+
+
+def test_gemm_many_groups_chunked(gpu):
+    rng = np.random.default_rng(1)
+    groups, refs = [], []
+    A_parts, B_parts = [], []
+    ao = bo = co = 0
+    for _ in range(75):  # > 32 groups per launch
+        m, n, k = rng.integers(1, 40, size=3)
+        a = rng.standard_normal((m, k))
+        b = rng.standard_normal((k, n))
+        A_parts.append(a.reshape(-1))
+        B_parts.append(b.reshape(-1))
+        groups.append((m, n, k, ao, k, 1, bo, n, 1, co, n))
+        refs.append((co, m, n, a @ b))
+        ao += m * k
+        bo += k * n
+        co += m * n
+    C = _ops.empty(co)
+    _ops.grouped_gemm(groups, _dev(np.concatenate(A_parts)), _dev(np.concatenate(B_parts)), C)
+    c = _ops.download(C)
+    for off, m, n, ref in refs:
+        err = float(np.abs(c[off : off + m * n].reshape(m, n) - ref).max())
+        assert err < 1e-12, f"group {(m, n)} err {err:.2e}"
+
+
+def test_split_gemm_precision(gpu):
+    rng = np.random.default_rng(2)
+    m, n, k = 96, 80, 300
+    a = rng.standard_normal((m, k))
+    b = rng.standard_normal((k, n))
+    C = _ops.empty(m * n)
+    _ops.grouped_gemm([(m, n, k, 0, k, 1, 0, n, 1, 0, n)], _dev(a), _dev(b), C, split=True)
+    got = _ops.download(C).reshape(m, n)
+    err = float(np.abs(got - a @ b).max() / np.abs(a @ b).max())
+    assert err < 1e-5, f"split gemm rel err {err:.2e}"  # FP32 accumulation inside each K chunk
+
+
+def test_jacobi_svd_batched(gpu):
+    rng = np.random.default_rng(3)
+    shapes = [(1, 1), (5, 3), (3, 5), (40, 40), (100, 37), (20, 130), (200, 200)]
+    parts, descs = [], []
+    ao = uo = so = vo = wo = 0
+    mats = []
+    for m, n in shapes:
+        a = rng.standard_normal((m, n))
+        if m == 40:
+            a[:, 5] = a[:, 6]  # rank deficient
+        mats.append(a)
+        q = min(m, n)
+        parts.append(a.reshape(-1))
+        descs.append((m, n, ao, n, 1, uo, so, vo, wo))
+        ao += m * n
+        uo += m * q
+        so += q
+        vo += q * n
+        wo += q * (max(m, n) + q)
+    A = _dev(np.concatenate(parts))
+    U, S, VT, W = _ops.empty(uo), _ops.empty(so), _ops.empty(vo), _ops.empty(wo)
+    _ops.batched_svd(descs, A, U, S, VT, W)
+    u, s, vt = _ops.download(U), _ops.download(S), _ops.download(VT)
+    for (m, n, _, _, _, uo_, so_, vo_, _), a in zip(descs, mats, strict=True):
+        q = min(m, n)
+        uu = u[uo_ : uo_ + m * q].reshape(m, q)
+        ss = s[so_ : so_ + q]
+        vv = vt[vo_ : vo_ + q * n].reshape(q, n)
+        ref = np.linalg.svd(a, compute_uv=False)
+        tag = f"shape {(m, n)}"
+        assert np.all(np.diff(ss) <= 0), tag
+        e_s = float(np.abs(ss - ref).max() / ref[0])
+        e_r = float(np.abs(uu * ss @ vv - a).max() / ref[0])
+        nz = ss > 1e-10 * ss[0]
+        e_u = float(np.abs(uu[:, nz].T @ uu[:, nz] - np.eye(nz.sum())).max())
+        e_v = float(np.abs(vv @ vv.T - np.eye(q)).max())
+        assert e_s < 1e-13 and e_r < 1e-12 and e_u < 1e-12 and e_v < 1e-12, (tag, e_s, e_r, e_u, e_v)
+
+
+def test_norm_trace_copy_fill(gpu):
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(100_003)
+    X = _dev(x)
+    seg = np.array([0, 10, 5000, 100_003])
+    w = np.array([1.0, 3.0, 2.0])
+    got = float(_ops.download(_ops.weighted_norm2(X, seg, w))[0])
+    ref = sum(wi * np.sum(x[seg[i] : seg[i + 1]] ** 2) for i, wi in enumerate(w))
+    assert abs(got - ref) <= 1e-13 * ref
+    m = rng.standard_normal((7, 7))
+    tr = float(_ops.download(_ops.weighted_trace(_dev(m), [0], [7], [7], [2.0]))[0])
+    assert abs(tr - 2 * np.trace(m)) < 1e-13
+    src = rng.standard_normal((6, 9))
+    dst = _ops.zeros(54)
+    cw = _dev(np.arange(9.0))
+    idx = _ops.upload(np.array([8, 0, 3], dtype=np.int64), dtype=torch.int64)
+    _ops.copy2d([(6, 3, 0, 9, 1, 0, 3, 1, 0, 2, 0, 1.5, 0.0)], _dev(src), dst, col_w=cw, col_idx=idx)
+    got = _ops.download(dst)[:18].reshape(6, 3)
+    assert np.array_equal(got, 1.5 * src[:, [8, 0, 3]] * np.arange(2.0, 5.0))
+    y = _ops.empty(1001)
+    _ops.fill_normal_(y, 5)
+    yy = _ops.download(y)
+    assert abs(yy.mean()) < 0.15 and abs(yy.std() - 1) < 0.1
+    y2 = _ops.empty(1001)
+    _ops.fill_normal_(y2, 5)
+    assert np.array_equal(yy, _ops.download(y2))
