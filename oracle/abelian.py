@@ -211,6 +211,44 @@ class Tensor:
                 kept.append(irr)
         return Tensor(self.row_reps, self.col_reps, blocks, kept, self.signature)
 
+    def __add__(self, other):
+        """Blockwise sum keeping blocks missing on one side (symmetric_tensor.py:349-372)."""
+        d = {int(r): b.copy() for r, b in zip(self.block_irreps, self.blocks, strict=True)}
+        for r, b in zip(other.block_irreps, other.blocks, strict=True):
+            d[int(r)] = d[int(r)] + b if int(r) in d else b.copy()
+        irr = sorted(d)
+        return Tensor(self.row_reps, self.col_reps, [d[r] for r in irr], irr, self.signature)
+
+    def trace(self):
+        return float(sum(np.trace(b) for b in self.blocks))
+
+    def truncated_eigh(self, nvals, atol=0.0, rtol=None, degen_ratio=1.0):
+        """Dense per-block eigh, keep the nvals largest |eigenvalues| with multiplets
+        (symmetric_tensor.py:1087-1132, 1250-1434, svd_tools.py:77-113). Returns
+        (value blocks, irreps, U)."""
+        irr_all, shapes = self.block_sizes(self.row_reps, self.col_reps, self.signature)
+        have = {int(r): b for r, b in zip(self.block_irreps, self.blocks, strict=True)}
+        vals, absv, vecs = [], [], []
+        for r in irr_all:
+            k = nvals + 1
+            if int(r) not in have:
+                vals.append(np.zeros((0,)))
+                absv.append(np.zeros((1,)))
+                vecs.append(None)
+                continue
+            w, v = lg.eigh(have[int(r)], driver="evd" if have[int(r)].size > 1 else "evr")
+            so = np.abs(w).argsort()[: -k - 1 : -1]
+            vals.append(w[so])
+            absv.append(np.abs(w[so]))
+            vecs.append(v[:, so])
+        cuts = find_block_cuts(absv, nvals, atol=atol, rtol=rtol, degen_ratio=degen_ratio)
+        keep = [i for i in range(len(cuts)) if cuts[i] > 0 and vecs[i] is not None]
+        irr = irr_all[keep]
+        mid = np.repeat(irr, cuts[keep]).astype(np.int8)
+        U = Tensor(self.row_reps, (mid,), [vecs[i][:, : cuts[i]] for i in keep], irr,
+                   np.concatenate([self.signature[: self.nrr], [True]]))
+        return [vals[i][: cuts[i]] for i in keep], irr, U
+
     def truncated_svd(self, cut, atol=0.0, rtol=None, degen_ratio=1.0):
         """Dense per-block SVD + find_block_cuts (symmetric_tensor.py:835-877,
         misc_tools/svd_tools.py:116-151). Returns (U, s_blocks, s_irreps, V)."""

@@ -1,0 +1,115 @@
+"""CTMRG, SimpleUpdate and the classical Ising driver on device tensors.
+
+Parity checks: the reference's own cross-symmetry invariants
+(tests/test_ctmrg.py:56-124 of frostspin: U(1) and trivial environments give the
+same rdm), the simple-update bond representations (test_simple_update.py:51-58),
+and the numpy oracle of the classical Ising driver at the plain-method
+configuration (beta = 0.44, chi = 32, 200 iterations, Z2)."""
+
+import numpy as np
+import pytest
+
+from frostspin_amd import AsymmetricTensor, U1SymmetricTensor
+from frostspin_amd.ctmrg import SequentialCTMRG
+from frostspin_amd.ctmrg import classical_ising as ci
+from frostspin_amd.simple_update import SimpleUpdate
+from oracle import ising as oracle_ising
+
+BACKENDS = [
+    pytest.param("emulated", id="emulated"),
+    pytest.param("gpu", id="gpu", marks=pytest.mark.gpu),
+]
+
+
+@pytest.fixture(params=BACKENDS)
+def backend(request):
+    if request.param == "emulated":
+        request.getfixturevalue("emulated_device")
+    else:
+        request.getfixturevalue("gpu")
+    return request.param
+
+
+def _cell_tensors(rng):
+    rp = np.array([1, -1], dtype=np.int8)
+    ra = np.array([1, -1, 2], dtype=np.int8)
+    ru = np.array([1, 1, -1, 0, 2, -2], dtype=np.int8)
+    rr = np.array([2, 0, 1, -1, 0, 1, -1], dtype=np.int8)
+    rd = np.array([1, 1, -1, -2], dtype=np.int8)
+    rl = np.array([1, -2, 1, 0, 0], dtype=np.int8)
+    sA = np.array([False, False, True, True, True, True])
+    sB = np.array([False, False, False, False, False, False])
+    reps_A = (rp, ra, ru, rr, rd, rl)
+    reps_B = (rp, ra, rd, rl, ru, rr)
+    A = U1SymmetricTensor.random(reps_A[:2], reps_A[2:], rng=rng, signature=sA)
+    B = U1SymmetricTensor.random(reps_B[:2], reps_B[2:], rng=rng, signature=sB)
+    dA = tuple(np.array([t.size]) for t in reps_A)
+    dB = tuple(np.array([t.size]) for t in reps_B)
+    A_as = AsymmetricTensor.from_array(A.toarray(), dA[:2], dA[2:], signature=sA)
+    B_as = AsymmetricTensor.from_array(B.toarray(), dB[:2], dB[2:], signature=sB)
+    return (A, B), (A_as, B_as)
+
+
+def test_ctmrg_u1_vs_trivial(backend, tmp_path):
+    u1, tr = _cell_tensors(np.random.default_rng(42))
+    ctm_u1 = SequentialCTMRG.from_elementary_tensors("AB\nBA", u1, 20)
+    ctm_tr = SequentialCTMRG.from_elementary_tensors("AB\nBA", tr, 20)
+    assert ctm_u1.Lx == 2 and ctm_u1.n_sites == 2 and ctm_u1.chi_values == [1]
+    assert ctm_u1.tiling == "AB\nBA"
+    r12u, r21u = ctm_u1.compute_rdm_1st_neighbor_cell()
+    r12t, r21t = ctm_tr.compute_rdm_1st_neighbor_cell()
+    for i in range(2):
+        assert r12u[i].shape == (2, 2, 2, 2)
+        assert (r12u[i].totrivial() - r12t[i]).norm() < 1e-13
+        assert (r12u[i] - r12u[i].dagger()).norm() < 1e-13
+        assert (r21u[i].totrivial() - r21t[i]).norm() < 1e-13
+    for _ in range(2):
+        ctm_u1.iterate()
+        ctm_tr.iterate()
+    r12u, r21u = ctm_u1.compute_rdm_1st_neighbor_cell()
+    r12t, r21t = ctm_tr.compute_rdm_1st_neighbor_cell()
+    for i in range(2):
+        assert (r12u[i].totrivial() - r12t[i]).norm() < 2e-4
+        assert (r21u[i].totrivial() - r21t[i]).norm() < 2e-4
+    assert abs(ctm_u1.compute_PEPS_norm_log() - ctm_tr.compute_PEPS_norm_log()) < 1e-4
+    f = tmp_path / "ctm.npz"
+    ctm_u1.save_to_file(f)
+    ctm2 = SequentialCTMRG.load_from_file(f)
+    for x, y in ctm_u1.site_coords:
+        for name in ("C1", "C2", "C3", "C4", "T1", "T2", "T3", "T4"):
+            t1 = getattr(ctm_u1, f"get_{name}")(x, y)
+            t2 = getattr(ctm2, f"get_{name}")(x, y)
+            assert t1.match_representations(t2)
+            assert (t1.toarray() == t2.toarray()).all()
+
+
+def test_simple_update_reference(backend):
+    sds = np.array(
+        [[0.25, 0, 0, 0], [0, -0.25, 0.5, 0], [0, 0.5, -0.25, 0], [0, 0, 0, 0.25]]
+    ).reshape(2, 2, 2, 2)
+    reps = (np.array([1, -1], dtype=np.int8),) * 2
+    h1 = U1SymmetricTensor.from_array(sds, reps, reps)
+    su = SimpleUpdate.square_lattice_first_neighbor(h1, 7, 0.02, degen_ratio=0.99, rcutoff=1e-8)
+    assert su.Dmax == 1 and su.beta == 0.0 and su.logZ == 0.0
+    assert su.n_bonds == 4 and su.n_tensors == 2 and su.symmetry() == "U1"
+    su.tau = 0.01
+    su.D = 4
+    su.evolve(0.04)
+    assert su.Dmax == 4
+    assert abs(su.beta - 0.04) < 1e-14
+    assert su.logZ > 0.0
+    assert all((r == np.array([-2, 0, 0, 2])).all() for r in su.get_bond_representations())
+    tensors = su.get_tensors()
+    assert len(tensors) == 2 and tensors[0].shape == (2, 2, 4, 4, 4, 4)
+
+
+@pytest.mark.parametrize(("beta", "chi", "n_iter", "sym"), [(0.44, 32, 200, "Z2"), (0.5, 16, 40, "trivial")])
+def test_classical_ising_vs_oracle(backend, beta, chi, n_iter, sym):
+    r = ci.run(beta=beta, chi=chi, n_iter=n_iter, symmetry=sym)
+    o = oracle_ising.run(beta=beta, chi=chi, n_iter=n_iter, symmetry=sym)
+    assert abs(r["energy"] - o["energy"]) < 1e-10 * abs(o["energy"])
+    assert abs(r["magnetization"] - o["magnetization"]) < 1e-9
+    got = np.sort(r["s"].toarray())
+    ref = np.sort(np.concatenate(o["s"].blocks))
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() < 1e-10
