@@ -55,6 +55,19 @@ class SvdDesc(ctypes.Structure):
     ]
 
 
+class QrDesc(ctypes.Structure):
+    _fields_ = [
+        ("m", c_i64),
+        ("n", c_i64),
+        ("a_off", c_i64),
+        ("a_rs", c_i64),
+        ("a_cs", c_i64),
+        ("q_off", c_i64),
+        ("r_off", c_i64),
+        ("work_off", c_i64),
+    ]
+
+
 class Copy2dDesc(ctypes.Structure):
     _fields_ = [
         ("rows", c_i64),
@@ -123,7 +136,11 @@ _SIGNATURES = {
         c_int,
         [c_int, ctypes.POINTER(SvdDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_vp],
     ),
-    "fs_weighted_norm2_f64": (c_int, [c_int, c_i64p, c_dblp, c_vp, c_vp, c_vp, c_vp]),
+    "fs_batched_qr_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(QrDesc), c_vp, c_vp, c_vp, c_vp, c_vp],
+    ),
+    "fs_weighted_norm2_f64":(c_int, [c_int, c_i64p, c_dblp, c_vp, c_vp, c_vp, c_vp]),
     "fs_scale_f64": (c_int, [c_i64, c_dbl, c_vp, c_vp]),
     "fs_scale_rsqrt_dev_f64": (c_int, [c_i64, c_vp, c_vp, c_vp]),
     "fs_axpby_f64": (c_int, [c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
@@ -146,7 +163,7 @@ EXPORTED_SYMBOLS = tuple(_SIGNATURES)
 def _load():
     if not _LIB_PATH.exists():
         msg = (
-            f"{_LIB_PATH.name} is not built: run `python -m frostspin_amd.csrc.build` "
+            f"{_LIB_PATH.name} is not built: run `python frostspin_amd/csrc/build.py` "
             "(frostspin_amd has no host fallback for device operations)"
         )
         raise ImportError(msg)

@@ -229,6 +229,16 @@ def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
     )
 
 
+def batched_qr(descs, A, Q, R, work):
+    """descs: sequence of 8-tuples (m, n, a_off, a_rs, a_cs, q_off, r_off, work_off);
+    R may be None (only Q is formed)."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.QrDesc * n)(*[_lib.QrDesc(*map(int, d)) for d in descs])
+    check(lib.fs_batched_qr_f64(n, arr, _ptr(A), _ptr(Q), _ptr(R), _ptr(work), _stream()))
+
+
 # ---------------------------------------------------------------------------
 # elementwise helpers
 # ---------------------------------------------------------------------------

@@ -1,6 +1,6 @@
 """Build libfrostspin_amd.so for gfx950 with hipcc (in-tree, no JIT cache).
 
-Usage: python -m frostspin_amd.csrc.build   (or call build())
+Usage: python frostspin_amd/csrc/build.py   (or __graft_entry__.build())
 The shared library lands next to the package (frostspin_amd/libfrostspin_amd.so)
 so it travels with the source tree.
 """
@@ -23,6 +23,7 @@ SOURCES = [
     "fs_permute.hip",
     "fs_gemm.hip",
     "fs_svd.hip",
+    "fs_qr.hip",
     "fs_elementwise.hip",
 ]
 ARCH = os.environ.get("FS_OFFLOAD_ARCH", "gfx950")

@@ -2,11 +2,13 @@
 //
 // X (p x q, column-major, p = max(m,n), q = min(m,n)) is A or A^T. Pairs of
 // columns are orthogonalised with the Rutishauser rotation; a round-robin
-// tournament gives q/2 disjoint pairs per round so the 4 wavefronts of the
+// tournament gives q/2 disjoint pairs per round so the wavefronts of the
 // workgroup rotate different pairs concurrently (each pair: one wavefront,
 // lanes striding over the column length). Sweeps repeat until no pair is
-// rotated. One-sided Jacobi computes small singular values to high relative
-// accuracy, which the s^-1/2 projector scaling of CTMRG relies on.
+// rotated. When X and V (q x q) fit in LDS the whole iteration runs out of LDS;
+// otherwise they live in the caller's work buffer (served by L2).
+// One-sided Jacobi computes small singular values to high relative accuracy,
+// which the s^-1/2 projector scaling of CTMRG relies on.
 // Singular values are written sorted non-increasing like LAPACK gesdd
 // (robust_svd, misc_tools/svd_tools.py:116-151); singular-vector signs are
 // not canonical (neither are LAPACK's), tests compare gauge-invariant data.
@@ -19,8 +21,7 @@
 namespace {
 
 constexpr int kMaxMat = 24;
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
+constexpr int kMaxQ = 1024;
 
 struct SvdDev {
   int32_t m, n;
@@ -32,6 +33,8 @@ struct SvdBatch {
   const double *A;
   double *U, *S, *VT, *work;
   int32_t nmat, max_sweeps;
+  int32_t lds_doubles;  // >0: matrices with p*q+q*q <= lds_doubles run in LDS
+  int32_t pad;
   SvdDev d[kMaxMat];
 };
 
@@ -41,35 +44,40 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-__global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
+__global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
+  extern __shared__ double lds[];
+  __shared__ double sv[kMaxQ];
+  __shared__ int rotated;
   const SvdDev D = bt.d[blockIdx.x];
   const int m = D.m, n = D.n;
   const bool tr = m < n;  // work on A^T when A is wide
   const int p = tr ? n : m;
   const int q = tr ? m : n;
-  double *X = bt.work + D.work_off;             // p x q, column-major
-  double *V = X + static_cast<int64_t>(p) * q;  // q x q, column-major
+  const int64_t pq = static_cast<int64_t>(p) * q;
+  const bool in_lds = pq + static_cast<int64_t>(q) * q <= bt.lds_doubles;
+  double *X = in_lds ? lds : bt.work + D.work_off;  // p x q, column-major
+  double *V = X + pq;                                // q x q, column-major
   const double *A = bt.A + D.a_off;
+  const int nthreads = blockDim.x;
+  const int nwaves = nthreads >> 6;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
 
-  // load X = A or A^T, V = I
-  for (int64_t e = tid; e < static_cast<int64_t>(p) * q; e += kThreads) {
+  for (int64_t e = tid; e < pq; e += nthreads) {
     const int col = static_cast<int>(e / p);
     const int row = static_cast<int>(e - static_cast<int64_t>(col) * p);
     const int i = tr ? col : row;  // row index in A
     const int j = tr ? row : col;  // col index in A
     X[e] = A[static_cast<int64_t>(i) * D.a_rs + static_cast<int64_t>(j) * D.a_cs];
   }
-  for (int64_t e = tid; e < static_cast<int64_t>(q) * q; e += kThreads) {
+  for (int64_t e = tid; e < static_cast<int64_t>(q) * q; e += nthreads) {
     const int col = static_cast<int>(e / q);
     const int row = static_cast<int>(e - static_cast<int64_t>(col) * q);
     V[e] = (row == col) ? 1.0 : 0.0;
   }
   __syncthreads();
 
-  __shared__ int rotated;
   const int Q = (q + 1) & ~1;  // even number of players (one dummy if q odd)
   const double tol = 4.0e-16;
   for (int sweep = 0; sweep < bt.max_sweeps && q > 1; ++sweep) {
@@ -77,8 +85,8 @@ __global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
     __syncthreads();
     int any = 0;
     for (int r = 0; r < Q - 1; ++r) {
-      // circle method: player Q-1 fixed, others rotate
-      for (int k = wave; k < Q / 2; k += kWaves) {
+      // circle method: player Q-1 fixed, the others rotate
+      for (int k = wave; k < Q / 2; k += nwaves) {
         int i, j;
         if (k == 0) {
           i = Q - 1;
@@ -88,7 +96,11 @@ __global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
           j = (r - k + (Q - 1)) % (Q - 1);
         }
         if (i >= q || j >= q) continue;
-        if (i > j) { const int s = i; i = j; j = s; }
+        if (i > j) {
+          const int s = i;
+          i = j;
+          j = s;
+        }
         double *xi = X + static_cast<int64_t>(i) * p;
         double *xj = X + static_cast<int64_t>(j) * p;
         double a = 0.0, b = 0.0, g = 0.0;
@@ -98,6 +110,7 @@ __global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
           b += v * v;
           g += u * v;
         }
+        // xor butterflies give bit-identical sums on every lane
         a = wave_sum(a);
         b = wave_sum(b);
         g = wave_sum(g);
@@ -131,28 +144,21 @@ __global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
   }
 
   // singular values and their rank (descending, ties by index)
-  __shared__ double sv[1024];
-  const bool lds_sv = q <= 1024;
-  for (int k = wave; k < q; k += kWaves) {
+  for (int k = wave; k < q; k += nwaves) {
     const double *xk = X + static_cast<int64_t>(k) * p;
     double a = 0.0;
     for (int t = lane; t < p; t += 64) a += xk[t] * xk[t];
     a = wave_sum(a);
-    if (lane == 0) {
-      if (lds_sv) sv[k] = sqrt(a);
-      bt.S[D.s_off + k] = sqrt(a);  // unsorted scratch, overwritten below
-    }
+    if (lane == 0) sv[k] = sqrt(a);
   }
   __syncthreads();
-  const double *svp = lds_sv ? sv : (bt.S + D.s_off);
-  // each wave handles columns k: compute rank, write outputs
   double *U = bt.U + D.u_off;
   double *VT = bt.VT + D.vt_off;
-  for (int k = wave; k < q; k += kWaves) {
-    const double sk = svp[k];
+  for (int k = wave; k < q; k += nwaves) {
+    const double sk = sv[k];
     int rank = 0;
     for (int l = lane; l < q; l += 64) {
-      const double sl = svp[l];
+      const double sl = sv[l];
       rank += (sl > sk || (sl == sk && l < k)) ? 1 : 0;
     }
 #pragma unroll
@@ -169,20 +175,27 @@ __global__ void __launch_bounds__(kThreads) jacobi_svd_kernel(SvdBatch bt) {
       for (int t = lane; t < q; t += 64) U[static_cast<int64_t>(t) * q + rank] = vk[t];
       for (int t = lane; t < p; t += 64) VT[static_cast<int64_t>(rank) * n + t] = xk[t] * inv;
     }
+    if (lane == 0) bt.S[D.s_off + rank] = sk;
   }
-  __syncthreads();
-  // sorted singular values (all waves finished reading the unsorted scratch)
-  if (lds_sv) {
-    for (int k = tid; k < q; k += kThreads) {
-      const double sk = sv[k];
-      int rank = 0;
-      for (int l = 0; l < q; ++l) {
-        const double sl = sv[l];
-        rank += (sl > sk || (sl == sk && l < k)) ? 1 : 0;
-      }
-      bt.S[D.s_off + rank] = sk;
-    }
+}
+
+int lds_limit_doubles() {
+  static int cached = -1;
+  if (cached >= 0) return cached;
+  int dev = 0, bytes = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    bytes = 65536;
+  // keep room for the static arrays (sv, rotated)
+  const int usable = std::max(0, bytes - (kMaxQ * 8 + 256));
+  if (hipFuncSetAttribute(reinterpret_cast<const void *>(jacobi_svd_kernel),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, usable) != hipSuccess) {
+    (void)hipGetLastError();
+    cached = 0;
+    return cached;
   }
+  cached = usable / 8;
+  return cached;
 }
 
 } // namespace
@@ -193,8 +206,9 @@ int fs_batched_svd_f64(int nmat, const fs_svd_desc *descs, const double *A, doub
                        double *S, double *VT, double *work, int max_sweeps, void *stream) {
   return fs::guarded([&]() -> int {
     FS_CHECK_ARG(nmat >= 0, "batched svd: negative count");
-    FS_CHECK_ARG(nmat == 0 || (descs && A && U && S && VT && work), "batched svd: null argument");
+    FS_CHECK_ARG(nmat == 0 || (descs && A && U && S && VT), "batched svd: null argument");
     FS_CHECK_ARG(max_sweeps >= 1, "batched svd: max_sweeps must be >= 1");
+    const int lds_cap = nmat ? lds_limit_doubles() : 0;
     int i0 = 0;
     while (i0 < nmat) {
       SvdBatch bt;
@@ -206,24 +220,36 @@ int fs_batched_svd_f64(int nmat, const fs_svd_desc *descs, const double *A, doub
       bt.work = work;
       bt.max_sweeps = max_sweeps;
       int nb = 0;
+      int64_t lds_need = 0;
+      int qmax = 1;
+      bool needs_work = false;
       for (; i0 < nmat && nb < kMaxMat; ++i0) {
-        const fs_svd_desc &q = descs[i0];
-        FS_CHECK_ARG(q.m >= 1 && q.n >= 1, "batched svd: empty matrix");
-        FS_CHECK_ARG(q.m < (1ll << 31) && q.n < (1ll << 31), "batched svd: matrix too large");
-        FS_CHECK_ARG(std::min(q.m, q.n) <= 1024, "batched svd: min(m, n) must be <= 1024");
+        const fs_svd_desc &s = descs[i0];
+        FS_CHECK_ARG(s.m >= 1 && s.n >= 1, "batched svd: empty matrix");
+        FS_CHECK_ARG(s.m < (1ll << 31) && s.n < (1ll << 31), "batched svd: matrix too large");
+        const int64_t q = std::min(s.m, s.n), p = std::max(s.m, s.n);
+        FS_CHECK_ARG(q <= kMaxQ, "batched svd: min(m, n) must be <= 1024");
+        const int64_t need = p * q + q * q;
+        if (need <= lds_cap) lds_need = std::max(lds_need, need);
+        else needs_work = true;
+        qmax = static_cast<int>(std::max<int64_t>(qmax, q));
         SvdDev &d = bt.d[nb++];
-        d.m = static_cast<int32_t>(q.m);
-        d.n = static_cast<int32_t>(q.n);
-        d.a_rs = static_cast<int32_t>(q.a_rs);
-        d.a_cs = static_cast<int32_t>(q.a_cs);
-        d.a_off = q.a_off;
-        d.u_off = q.u_off;
-        d.s_off = q.s_off;
-        d.vt_off = q.vt_off;
-        d.work_off = q.work_off;
+        d.m = static_cast<int32_t>(s.m);
+        d.n = static_cast<int32_t>(s.n);
+        d.a_rs = static_cast<int32_t>(s.a_rs);
+        d.a_cs = static_cast<int32_t>(s.a_cs);
+        d.a_off = s.a_off;
+        d.u_off = s.u_off;
+        d.s_off = s.s_off;
+        d.vt_off = s.vt_off;
+        d.work_off = s.work_off;
       }
+      FS_CHECK_ARG(!needs_work || work, "batched svd: work buffer required for large matrices");
       bt.nmat = nb;
-      hipLaunchKernelGGL(jacobi_svd_kernel, dim3(nb), dim3(kThreads), 0, fs::as_stream(stream), bt);
+      bt.lds_doubles = static_cast<int32_t>(lds_need);
+      const int threads = qmax >= 96 ? 1024 : (qmax >= 24 ? 512 : 256);
+      hipLaunchKernelGGL(jacobi_svd_kernel, dim3(nb), dim3(threads),
+                         static_cast<size_t>(lds_need) * 8, fs::as_stream(stream), bt);
       FS_LAUNCH_CHECK("jacobi_svd_kernel");
     }
     return FS_OK;

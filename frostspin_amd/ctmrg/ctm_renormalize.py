@@ -69,8 +69,10 @@ def construct_projectors(
     ncv = (ncv_ratio * block_chi).astype(int) + 10
     # the reference uses ARPACK (block_chi values) where min(shape) >= max(100, 4 ncv)
     sparse_ref = mins >= np.maximum(100, 4 * ncv)
-    dense_ids = [i for i in range(n) if mins[i] <= DENSE_SVD_LIMIT]
-    rand_ids = [i for i in range(n) if mins[i] > DENSE_SVD_LIMIT]
+    # exact dense SVD where the reference uses LAPACK (and for anything small),
+    # randomized range finder where it uses ARPACK for block_chi values
+    dense_ids = [i for i in range(n) if not sparse_ref[i] and mins[i] <= DENSE_SVD_LIMIT]
+    rand_ids = [i for i in range(n) if sparse_ref[i] or mins[i] > DENSE_SVD_LIMIT]
     U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
     if dense_ids:
         U, S, s_off, s_list, VT = bl.svd(M.select(dense_ids))

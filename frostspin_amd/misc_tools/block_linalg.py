@@ -118,10 +118,30 @@ def svd(A, *, max_sweeps=60):
     return U, S, s_off, s_list, VT
 
 
+def qr(A, *, want_r=True):
+    """Economic Householder QR of every A_i (one batched launch): Q packed
+    (m x q), R packed (q x n) upper triangular (or None)."""
+    q = np.minimum(A.rows, A.cols)
+    Q = Mats.packed(A.rows, q)
+    R = Mats.packed(q, A.cols) if want_r else None
+    w_size = A.rows * A.cols + q + A.rows * q
+    w_off = np.zeros_like(w_size)
+    if q.size > 1:
+        w_off[1:] = np.cumsum(w_size[:-1])
+    work = _ops.empty(int(w_size.sum()))
+    descs = [
+        (A.rows[i], A.cols[i], A.off[i], A.rs[i], A.cs[i], Q.off[i],
+         R.off[i] if R is not None else 0, w_off[i])
+        for i in range(len(A))
+    ]
+    _ops.batched_qr(descs, A.buf, Q.buf, None if R is None else R.buf, work)
+    return Q, R
+
+
 def _orthonormal_columns(Y):
-    """Orthonormal basis of the column space of each Y_i (U factor of its SVD)."""
-    U, _, _, _, _ = svd(Y)
-    return U
+    """Orthonormal basis of the column space of each Y_i (Householder Q factor)."""
+    Q, _ = qr(Y, want_r=False)
+    return Q
 
 
 def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):
@@ -142,6 +162,10 @@ def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):
         Y = gemm(A, Qz)
     Q = _orthonormal_columns(Y)
     B = gemm(Q.T(), A)  # l x n
-    Ub, S, s_off, s_list, VT = svd(B)
-    U = gemm(Q, Ub)  # m x l
+    # B^T = Q2 R2 (Householder), then the small l x l SVD of R2^T = Ur S Vr^T:
+    # B = Ur S (Q2 Vr)^T
+    Q2, R2 = qr(B.T())
+    Ur, S, s_off, s_list, Vrt = svd(R2.T())
+    U = gemm(Q, Ur)  # m x l
+    VT = gemm(Vrt, Q2.T())  # l x n
     return U, S, s_off, s_list, VT

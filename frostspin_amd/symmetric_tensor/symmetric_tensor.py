@@ -718,16 +718,33 @@ class SymmetricTensor:
         return v.dagger() * (1 / s) @ u.dagger()
 
     def qr(self):
-        """Q R factorisation per block through the device SVD: Q = U (orthonormal
-        columns), R = S V^T = Q^T A. R is not triangular (the reference's LAPACK R
-        is); Q R reproduces the tensor and Q spans the same column space."""
-        u, s, v = self.svd()
-        return u, s * v
+        """Economic QR of every block in one batched Householder launch, LAPACK
+        conventions (symmetric_tensor.py:655-671): Q (rows | mid), R (mid | cols)."""
+        from ..misc_tools import block_linalg as bl
+
+        A = bl.Mats.from_tensor_blocks(self, range(self._nblocks))
+        Q, R = bl.qr(A)
+        degen = np.minimum(self._bshapes[:, 0], self._bshapes[:, 1])
+        mid_rep = self.init_representation(degen, self._block_irreps)
+        qsign = np.ones((self._nrr + 1,), dtype=bool)
+        qsign[: self._nrr] = self._signature[: self._nrr]
+        rsign = np.zeros((self._ndim - self._nrr + 1,), dtype=bool)
+        rsign[1:] = self._signature[self._nrr :]
+        Qt = type(self)._build(
+            self._row_reps, (mid_rep,), Q.buf, np.stack([Q.rows, Q.cols], 1), Q.off,
+            self._block_irreps, qsign,
+        )
+        Rt = type(self)._build(
+            (mid_rep,), self._col_reps, R.buf, np.stack([R.rows, R.cols], 1), R.off,
+            self._block_irreps, rsign,
+        )
+        return Qt, Rt
 
     def rq(self):
-        """R Q per block through the device SVD: R = U S, Q = V^T (orthonormal rows)."""
-        u, s, v = self.svd()
-        return u * s, v
+        """A = R Q with Q orthonormal rows, from the QR of the transpose: R is lower
+        triangular here (LAPACK's rq returns an upper triangular R)."""
+        q, r = self.transpose().qr()
+        return r.transpose(), q.transpose()
 
     def _shifted_svd(self):
         """SVD of every block shifted by sigma*I with sigma = ||A||_F, which makes each

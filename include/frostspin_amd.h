@@ -192,6 +192,27 @@ int fs_batched_svd_f64(int nmat, const fs_svd_desc *descs, const double *A,
                        int max_sweeps, void *stream);
 
 /* ------------------------------------------------------------------------- */
+/* Batched FP64 Householder QR                                               */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * A_b (m x n, strided as in fs_svd_desc) = Q R with q = min(m,n): Q (m x q)
+ * row-major at Q + q_off, R (q x n) row-major upper triangular at R + r_off
+ * (R may be NULL). LAPACK geqrf/orgqr sign conventions. work must hold
+ * m*n + q + m*q doubles per matrix from work_off.
+ * Replaces the per-block scipy.linalg.qr(mode="economic") of
+ * SymmetricTensor.qr (symmetric_tensor.py:655-671).
+ */
+typedef struct fs_qr_desc {
+  int64_t m, n;
+  int64_t a_off, a_rs, a_cs;
+  int64_t q_off, r_off, work_off;
+} fs_qr_desc;
+
+int fs_batched_qr_f64(int nmat, const fs_qr_desc *descs, const double *A,
+                      double *Q, double *R, double *work, void *stream);
+
+/* ------------------------------------------------------------------------- */
 /* Elementwise / reduction helpers on packed block buffers                   */
 /* ------------------------------------------------------------------------- */
 

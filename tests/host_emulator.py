@@ -80,6 +80,17 @@ def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
         vt[vo : vo + q * n] = vv.reshape(-1)
 
 
+def batched_qr(descs, A, Q, R, work):
+    a, qb = _np(A), _np(Q)
+    rb = None if R is None else _np(R)
+    for m, n, ao, ars, acs, qo, ro, _wo in descs:
+        q = min(m, n)
+        qq, rr = np.linalg.qr(_view(a, ao, m, n, ars, acs).copy(), mode="reduced")
+        qb[qo : qo + m * q] = qq.reshape(-1)
+        if rb is not None:
+            rb[ro : ro + q * n] = rr.reshape(-1)
+
+
 def weighted_norm2(x, seg_off, weights):
     xs = _np(x)
     tot = 0.0
@@ -146,7 +157,7 @@ def install(monkeypatch):
     monkeypatch.setattr(_ops, "device", lambda: torch.device("cpu"))
     monkeypatch.setattr(_ops.PermutePlan, "execute", _execute)
     for name in (
-        "grouped_gemm", "batched_svd", "weighted_norm2", "scale_", "scale_rsqrt_dev_",
+        "grouped_gemm", "batched_svd", "batched_qr", "weighted_norm2", "scale_", "scale_rsqrt_dev_",
         "axpby_", "copy2d", "weighted_trace", "fill_normal_",
     ):
         monkeypatch.setattr(_ops, name, globals()[name])
