@@ -29,6 +29,57 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+class KernelTimer:
+    """Optional per-launch timing: HIP events recorded on the launch stream around
+    every native call, with the algorithmic work of that call (flop for GEMMs,
+    bytes moved for permutes). Used by the benchmark to price kernels against the
+    roofline; off by default (no events, no synchronization)."""
+
+    active = None
+
+    def __init__(self):
+        self.records = []  # (kind, work, start_event, end_event)
+
+    def __enter__(self):
+        KernelTimer.active = self
+        return self
+
+    def __exit__(self, *exc):
+        KernelTimer.active = None
+
+    def summary(self):
+        torch.cuda.synchronize()
+        out = {}
+        for kind, work, e0, e1 in self.records:
+            ms = e0.elapsed_time(e1)
+            d = out.setdefault(kind, {"launches": 0, "ms": 0.0, "work": 0.0})
+            d["launches"] += 1
+            d["ms"] += ms
+            d["work"] += work
+        return out
+
+
+class _timed:
+    __slots__ = ("kind", "work", "e0")
+
+    def __init__(self, kind, work):
+        self.kind = kind
+        self.work = work
+        self.e0 = None
+
+    def __enter__(self):
+        if KernelTimer.active is not None:
+            self.e0 = torch.cuda.Event(enable_timing=True)
+            self.e0.record()
+        return self
+
+    def __exit__(self, *exc):
+        if self.e0 is not None and KernelTimer.active is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            KernelTimer.active.records.append((self.kind, self.work, self.e0, e1))
+
+
 def _ptr(t):
     if t is None:
         return None
@@ -106,13 +157,14 @@ class PermutePlan:
         return t
 
     def execute(self, src, dst):
-        if dst.dtype == torch.float64:
-            check(lib.fs_permute_plan_execute_f64(self._handle, _ptr(src), _ptr(dst), _stream()))
-        elif dst.dtype == torch.float32:
-            check(lib.fs_permute_plan_execute_f32(self._handle, _ptr(src), _ptr(dst), _stream()))
-        else:
-            msg = f"unsupported dtype {dst.dtype}"
-            raise TypeError(msg)
+        with _timed("permute", 2.0 * dst.element_size() * self.new_ncoeff):
+            if dst.dtype == torch.float64:
+                check(lib.fs_permute_plan_execute_f64(self._handle, _ptr(src), _ptr(dst), _stream()))
+            elif dst.dtype == torch.float32:
+                check(lib.fs_permute_plan_execute_f32(self._handle, _ptr(src), _ptr(dst), _stream()))
+            else:
+                msg = f"unsupported dtype {dst.dtype}"
+                raise TypeError(msg)
 
     def __del__(self):
         h = self._handle
@@ -207,7 +259,9 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
         return
     arr = (_lib.GemmGroup * n)(*[_lib.GemmGroup(*map(int, g)) for g in groups])
     fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
-    check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
+    flop = 2.0 * sum(float(g[0]) * float(g[1]) * float(g[2]) for g in groups)
+    with _timed("gemm_split" if split else "gemm", flop):
+        check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
 
 
 # ---------------------------------------------------------------------------
@@ -222,11 +276,14 @@ def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
     if n == 0:
         return
     arr = (_lib.SvdDesc * n)(*[_lib.SvdDesc(*map(int, d)) for d in descs])
-    check(
-        lib.fs_batched_svd_f64(
-            n, arr, _ptr(A), _ptr(U), _ptr(S), _ptr(VT), _ptr(work), int(max_sweeps), _stream()
+    # work model: one Jacobi sweep ~ 3 q^2 p flop (dot products + rotations)
+    flop = sum(3.0 * min(d[0], d[1]) ** 2 * max(d[0], d[1]) for d in descs)
+    with _timed("svd", flop):
+        check(
+            lib.fs_batched_svd_f64(
+                n, arr, _ptr(A), _ptr(U), _ptr(S), _ptr(VT), _ptr(work), int(max_sweeps), _stream()
+            )
         )
-    )
 
 
 def batched_qr(descs, A, Q, R, work):
@@ -236,7 +293,9 @@ def batched_qr(descs, A, Q, R, work):
     if n == 0:
         return
     arr = (_lib.QrDesc * n)(*[_lib.QrDesc(*map(int, d)) for d in descs])
-    check(lib.fs_batched_qr_f64(n, arr, _ptr(A), _ptr(Q), _ptr(R), _ptr(work), _stream()))
+    flop = sum(4.0 * d[0] * min(d[0], d[1]) ** 2 for d in descs)  # geqrf + orgqr, m >= n
+    with _timed("qr", flop):
+        check(lib.fs_batched_qr_f64(n, arr, _ptr(A), _ptr(Q), _ptr(R), _ptr(work), _stream()))
 
 
 # ---------------------------------------------------------------------------
@@ -281,11 +340,13 @@ def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
     arr = (_lib.Copy2dDesc * n)(
         *[_lib.Copy2dDesc(*(int(v) for v in d[:11]), float(d[11]), float(d[12])) for d in descs]
     )
-    check(
-        lib.fs_batched_copy2d_f64(
-            n, arr, _ptr(src), _ptr(dst), _ptr(row_w), _ptr(col_w), _ptr(col_idx), _stream()
+    nbytes = sum((16.0 if d[12] == 0.0 else 24.0) * d[0] * d[1] for d in descs)
+    with _timed("copy2d", nbytes):
+        check(
+            lib.fs_batched_copy2d_f64(
+                n, arr, _ptr(src), _ptr(dst), _ptr(row_w), _ptr(col_w), _ptr(col_idx), _stream()
+            )
         )
-    )
 
 
 def weighted_trace(x, offs, ns, lds, weights):

@@ -76,10 +76,26 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
     const int row = static_cast<int>(e - static_cast<int64_t>(col) * q);
     V[e] = (row == col) ? 1.0 : 0.0;
   }
+  __shared__ double fro2_red[16];
+  {
+    double f = 0.0;
+    for (int64_t e = tid; e < pq; e += nthreads) f += X[e] * X[e];
+    f = wave_sum(f);
+    if (lane == 0) fro2_red[wave] = f;
+  }
   __syncthreads();
+  double fro2 = 0.0;
+  for (int w = 0; w < nwaves; ++w) fro2 += fro2_red[w];
+  // columns whose squared norm is below (eps ||A||_F)^2 are numerical zeros of a
+  // rank-deficient matrix: rotating them against each other only reshuffles
+  // rounding noise and would keep the sweep loop from terminating
+  const double negligible = fro2 * 4.930380657631324e-32;
 
   const int Q = (q + 1) & ~1;  // even number of players (one dummy if q odd)
-  const double tol = 4.0e-16;
+  // rotate only if |cos angle(x_i, x_j)| > sqrt(p) * eps (LAPACK dgesvj criterion);
+  // a smaller threshold sits below the rounding noise of the dot products and
+  // never converges
+  const double tol = sqrt(static_cast<double>(p)) * 2.220446049250313e-16;
   for (int sweep = 0; sweep < bt.max_sweeps && q > 1; ++sweep) {
     if (tid == 0) rotated = 0;
     __syncthreads();
@@ -114,7 +130,7 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
         a = wave_sum(a);
         b = wave_sum(b);
         g = wave_sum(g);
-        if (g != 0.0 && fabs(g) > tol * sqrt(a) * sqrt(b)) {
+        if (g != 0.0 && a > negligible && b > negligible && fabs(g) > tol * sqrt(a) * sqrt(b)) {
           const double zeta = (b - a) / (2.0 * g);
           const double t_ = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double c = 1.0 / sqrt(1.0 + t_ * t_);
