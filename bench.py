@@ -288,6 +288,7 @@ def main():
     ap.add_argument("--D", type=int, default=6)
     ap.add_argument("--beta", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--debug-checks", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -301,6 +302,11 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP GPU", file=sys.stderr)
         sys.exit(1)
+    from frostspin_amd.symmetric_tensor.tools import config as fs_config
+
+    # the timed path runs like the reference under python -O (README.md:46):
+    # no debug norm/structure checks (each costs a device reduction + sync)
+    fs_config["debug_checks"] = args.debug_checks
     if args.workload == "ctmrg":
         res = run_ctmrg(args)
     else:

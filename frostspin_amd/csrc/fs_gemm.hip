@@ -12,6 +12,7 @@
 // without a copy: the global->LDS load order follows whichever dimension is
 // contiguous.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -60,11 +61,59 @@ __device__ inline int xcd_logical_tile(int bid, int nblocks) {
   return (bid % nx) * per + bid / nx;
 }
 
+// Global -> register tile loads: the index walk follows whichever dimension of
+// the operand is contiguous so every wavefront load is coalesced.
+template <int BM, int BK, int PER>
+__device__ inline void load_a(double (&r)[PER], const double *__restrict__ A, const GroupDev &G,
+                              int m0, int k0, bool kcontig) {
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = threadIdx.x + kThreads * q;
+    const int i = kcontig ? idx / BK : idx % BM;
+    const int kk = kcontig ? idx % BK : idx / BM;
+    const int gi = m0 + i, gk = k0 + kk;
+    r[q] = (gi < G.m && gk < G.k)
+               ? A[static_cast<int64_t>(gi) * G.a_rs + static_cast<int64_t>(gk) * G.a_cs]
+               : 0.0;
+  }
+}
+
+template <int BN, int BK, int PER>
+__device__ inline void load_b(double (&r)[PER], const double *__restrict__ B, const GroupDev &G,
+                              int n0, int k0, bool ncontig) {
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int idx = threadIdx.x + kThreads * q;
+    const int j = ncontig ? idx % BN : idx / BK;
+    const int kk = ncontig ? idx / BN : idx % BK;
+    const int gj = n0 + j, gk = k0 + kk;
+    r[q] = (gj < G.n && gk < G.k)
+               ? B[static_cast<int64_t>(gk) * G.b_rs + static_cast<int64_t>(gj) * G.b_cs]
+               : 0.0;
+  }
+}
+
+// FP64 MFMA tile kernel. BM x BN output tile per workgroup (4 wavefronts in a
+// 2x2 arrangement, each (BM/2) x (BN/2) = TI x TJ MFMA 16x16 tiles), BK = 16.
+// Software pipeline: the next K panel is fetched into registers while the
+// current one is consumed from LDS; LDS is double buffered so each K step needs
+// one workgroup barrier.
+//
+// LDS layout follows the source layout so that both the stores and the MFMA
+// operand reads are bank-conflict free (64 banks x 4 B = 32 doubles):
+//  * dimension-contiguous source (col-major A / row-major B): k-major rows
+//    [k][i] with row stride BM+16 (= 16 mod 32): the 4 k rows read by one MFMA
+//    operand land in disjoint bank halves, stores are unit stride;
+//  * k-contiguous source (row-major A / col-major B): i-major rows [i][k] with
+//    odd stride BK+1: the 16 rows of an operand read map to distinct banks and
+//    the 4 k columns shift them by one bank.
 template <int BM, int BN, int BK>
 __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
-  // LDS tiles, k-major: As[k][i], Bs[k][j]; +1 pad breaks power-of-two strides.
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN + 1];
+  constexpr int PAD = 16;
+  constexpr int A_SZ = (BK * (BM + PAD) > BM * (BK + 1)) ? BK * (BM + PAD) : BM * (BK + 1);
+  constexpr int B_SZ = (BK * (BN + PAD) > BN * (BK + 1)) ? BK * (BN + PAD) : BN * (BK + 1);
+  __shared__ double As[2][A_SZ];
+  __shared__ double Bs[2][B_SZ];
 
   const int tile = bt.xcd_remap ? xcd_logical_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const int gi = find_group(bt, tile);
@@ -79,7 +128,6 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  // 4 waves in a 2x2 arrangement; each wave owns (BM/2) x (BN/2)
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int TI = WM / 16, TJ = WN / 16;
   const int wm = wave >> 1, wn = wave & 1;
@@ -94,59 +142,58 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
   const bool b_ncontig = (G.b_cs == 1);
   constexpr int A_PER = BM * BK / kThreads;
   constexpr int B_PER = BN * BK / kThreads;
+  double ra[A_PER], rb[B_PER];
+  // LDS element strides (k, i) of A and (k, j) of B for the chosen layouts
+  const int a_ks = a_kcontig ? 1 : BM + PAD, a_is = a_kcontig ? BK + 1 : 1;
+  const int b_ks = b_ncontig ? BN + PAD : 1, b_js = b_ncontig ? 1 : BK + 1;
 
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const int idx = threadIdx.x + kThreads * q;
+      const int i = a_kcontig ? idx / BK : idx % BM;
+      const int kk = a_kcontig ? idx % BK : idx / BM;
+      As[buf][kk * a_ks + i * a_is] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const int idx = threadIdx.x + kThreads * q;
+      const int j = b_ncontig ? idx % BN : idx / BK;
+      const int kk = b_ncontig ? idx / BN : idx % BK;
+      Bs[buf][kk * b_ks + j * b_js] = rb[q];
+    }
+  };
+
+  load_a<BM, BK, A_PER>(ra, A, G, m0, 0, a_kcontig);
+  load_b<BN, BK, B_PER>(rb, B, G, n0, 0, b_ncontig);
+  store(0);
+  __syncthreads();
+  int cur = 0;
   for (int k0 = 0; k0 < G.k; k0 += BK) {
-    double ra[A_PER], rb[B_PER];
-#pragma unroll
-    for (int r = 0; r < A_PER; ++r) {
-      const int idx = threadIdx.x + kThreads * r;
-      const int i = a_kcontig ? idx / BK : idx % BM;
-      const int kk = a_kcontig ? idx % BK : idx / BM;
-      const int gi_ = m0 + i, gk = k0 + kk;
-      ra[r] = (gi_ < G.m && gk < G.k)
-                  ? A[static_cast<int64_t>(gi_) * G.a_rs + static_cast<int64_t>(gk) * G.a_cs]
-                  : 0.0;
+    const bool next = k0 + BK < G.k;
+    if (next) {
+      load_a<BM, BK, A_PER>(ra, A, G, m0, k0 + BK, a_kcontig);
+      load_b<BN, BK, B_PER>(rb, B, G, n0, k0 + BK, b_ncontig);
     }
-#pragma unroll
-    for (int r = 0; r < B_PER; ++r) {
-      const int idx = threadIdx.x + kThreads * r;
-      const int j = b_ncontig ? idx % BN : idx / BK;
-      const int kk = b_ncontig ? idx / BN : idx % BK;
-      const int gj = n0 + j, gk = k0 + kk;
-      rb[r] = (gj < G.n && gk < G.k)
-                  ? B[static_cast<int64_t>(gk) * G.b_rs + static_cast<int64_t>(gj) * G.b_cs]
-                  : 0.0;
-    }
-    __syncthreads();  // previous tile fully consumed
-#pragma unroll
-    for (int r = 0; r < A_PER; ++r) {
-      const int idx = threadIdx.x + kThreads * r;
-      const int i = a_kcontig ? idx / BK : idx % BM;
-      const int kk = a_kcontig ? idx % BK : idx / BM;
-      As[kk][i] = ra[r];
-    }
-#pragma unroll
-    for (int r = 0; r < B_PER; ++r) {
-      const int idx = threadIdx.x + kThreads * r;
-      const int j = b_ncontig ? idx % BN : idx / BK;
-      const int kk = b_ncontig ? idx / BN : idx % BK;
-      Bs[kk][j] = rb[r];
-    }
-    __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4);
       double av[TI], bv[TJ];
 #pragma unroll
-      for (int a = 0; a < TI; ++a) av[a] = As[kr][wm * WM + a * 16 + (lane & 15)];
+      for (int a = 0; a < TI; ++a)
+        av[a] = As[cur][kr * a_ks + (wm * WM + a * 16 + (lane & 15)) * a_is];
 #pragma unroll
-      for (int b = 0; b < TJ; ++b) bv[b] = Bs[kr][wn * WN + b * 16 + (lane & 15)];
+      for (int b = 0; b < TJ; ++b)
+        bv[b] = Bs[cur][kr * b_ks + (wn * WN + b * 16 + (lane & 15)) * b_js];
 #pragma unroll
       for (int a = 0; a < TI; ++a)
 #pragma unroll
         for (int b = 0; b < TJ; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
+    if (next) store(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 
   // epilogue: for v_mfma_f64_16x16x4f64 lane l holds D[l/16 + 4*r][l%16] of
@@ -303,7 +350,19 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(Batch bt) {
       }
 }
 
-constexpr int kBM = 64, kBN = 64, kBK = 16;
+constexpr int kBK = 16;
+
+// Tile shape for one launch. Measured on MI355X (FP64, square m=n=k): 64x64
+// reaches 42-47 TFLOP/s from 2048 up for every operand layout, 128x128 (one
+// workgroup per CU at 398 registers) 30-39 TFLOP/s and far less on thin or
+// small shapes, so 64x64 is the default; FS_GEMM_TILE=128 forces the other.
+inline int pick_tile(int, const fs_gemm_group *) {
+  static const int forced = [] {
+    const char *e = std::getenv("FS_GEMM_TILE");
+    return e ? std::atoi(e) : 0;
+  }();
+  return forced == 128 ? 128 : 64;
+}
 
 template <bool SPLIT>
 int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, const double *B,
@@ -312,6 +371,8 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
   FS_CHECK_ARG(ngroups == 0 || groups, "grouped gemm: null groups");
   int g0 = 0;
   while (g0 < ngroups) {
+    const int g_end = std::min(ngroups, g0 + kMaxGroups);
+    const int tsz = SPLIT ? 64 : pick_tile(g_end - g0, groups + g0);
     Batch bt;
     std::memset(&bt, 0, sizeof(bt));
     bt.A = A;
@@ -322,7 +383,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
     int ng = 0;
     int64_t tiles = 0;
     bt.tile_prefix[0] = 0;
-    for (; g0 < ngroups && ng < kMaxGroups; ++g0) {
+    for (; g0 < g_end; ++g0) {
       const fs_gemm_group &q = groups[g0];
       FS_CHECK_ARG(q.m >= 0 && q.n >= 0 && q.k >= 0, "grouped gemm: negative dimension");
       if (q.m == 0 || q.n == 0) continue;
@@ -333,7 +394,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
       d.m = static_cast<int32_t>(q.m);
       d.n = static_cast<int32_t>(q.n);
       d.k = static_cast<int32_t>(q.k);
-      d.tiles_n = static_cast<int32_t>(fs::ceil_div(q.n, kBN));
+      d.tiles_n = static_cast<int32_t>(fs::ceil_div(q.n, tsz));
       d.a_rs = static_cast<int32_t>(q.a_rs);
       d.a_cs = static_cast<int32_t>(q.a_cs);
       d.b_rs = static_cast<int32_t>(q.b_rs);
@@ -342,7 +403,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
       d.a_off = q.a_off;
       d.b_off = q.b_off;
       d.c_off = q.c_off;
-      tiles += fs::ceil_div(q.m, kBM) * d.tiles_n;
+      tiles += fs::ceil_div(q.m, tsz) * d.tiles_n;
       FS_CHECK_ARG(tiles < (1ll << 31), "grouped gemm: too many tiles");
       bt.tile_prefix[ng + 1] = static_cast<int32_t>(tiles);
       ++ng;
@@ -350,12 +411,16 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
     if (ng == 0) continue;
     bt.ngroups = ng;
     bt.xcd_remap = tiles >= 64 ? 1 : 0;
+    const dim3 grid(static_cast<unsigned>(tiles));
     if (SPLIT)
-      hipLaunchKernelGGL((gemm_split_kernel<kBM, kBN, kBK>), dim3(static_cast<unsigned>(tiles)),
-                         dim3(kThreads), 0, fs::as_stream(stream), bt);
+      hipLaunchKernelGGL((gemm_split_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,
+                         fs::as_stream(stream), bt);
+    else if (tsz == 128)
+      hipLaunchKernelGGL((gemm_f64_kernel<128, 128, kBK>), grid, dim3(kThreads), 0,
+                         fs::as_stream(stream), bt);
     else
-      hipLaunchKernelGGL((gemm_f64_kernel<kBM, kBN, kBK>), dim3(static_cast<unsigned>(tiles)),
-                         dim3(kThreads), 0, fs::as_stream(stream), bt);
+      hipLaunchKernelGGL((gemm_f64_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,
+                         fs::as_stream(stream), bt);
     FS_LAUNCH_CHECK(SPLIT ? "gemm_split_kernel" : "gemm_f64_kernel");
   }
   return FS_OK;

@@ -98,9 +98,11 @@ __global__ void __launch_bounds__(kThreads) householder_qr_kernel(QrBatch bt) {
       for (int c = j + 1 + wave; c < n; c += kWaves) {
         double *xc = X + static_cast<int64_t>(c) * m;
         double d = (lane == 0) ? xc[j] : 0.0;
+#pragma unroll 8
         for (int i = j + 1 + lane; i < m; i += 64) d += xj[i] * xc[i];
         d = wave_sum(d) * t;
         if (lane == 0) xc[j] -= d;
+#pragma unroll 8
         for (int i = j + 1 + lane; i < m; i += 64) xc[i] -= d * xj[i];
       }
     }
@@ -121,9 +123,11 @@ __global__ void __launch_bounds__(kThreads) householder_qr_kernel(QrBatch bt) {
     for (int c = j + wave; c < q; c += kWaves) {
       double *yc = Y + static_cast<int64_t>(c) * m;
       double d = (lane == 0) ? yc[j] : 0.0;
+#pragma unroll 8
       for (int i = j + 1 + lane; i < m; i += 64) d += xj[i] * yc[i];
       d = wave_sum(d) * t;
       if (lane == 0) yc[j] -= d;
+#pragma unroll 8
       for (int i = j + 1 + lane; i < m; i += 64) yc[i] -= d * xj[i];
     }
     __syncthreads();

@@ -44,6 +44,13 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// Sum over an aligned group of 16 lanes (xor offsets < 16 never leave the group).
+__device__ inline double group_sum16(double v) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
   extern __shared__ double lds[];
   __shared__ double sv[kMaxQ];
@@ -63,6 +70,9 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
+  const int group = tid >> 4;  // 16-lane pair groups
+  const int sub = tid & 15;
+  const int ngroups = nthreads >> 4;
 
   for (int64_t e = tid; e < pq; e += nthreads) {
     const int col = static_cast<int>(e / p);
@@ -101,8 +111,10 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
     __syncthreads();
     int any = 0;
     for (int r = 0; r < Q - 1; ++r) {
-      // circle method: player Q-1 fixed, the others rotate
-      for (int k = wave; k < Q / 2; k += nwaves) {
+      // circle method: player Q-1 fixed, the others rotate. Each 16-lane group
+      // of a wavefront owns one pair, so a wavefront rotates 4 pairs at once and
+      // the dot-product reductions are 4 butterfly steps deep.
+      for (int k = group; k < Q / 2; k += ngroups) {
         int i, j;
         if (k == 0) {
           i = Q - 1;
@@ -120,29 +132,32 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
         double *xi = X + static_cast<int64_t>(i) * p;
         double *xj = X + static_cast<int64_t>(j) * p;
         double a = 0.0, b = 0.0, g = 0.0;
-        for (int t = lane; t < p; t += 64) {
+#pragma unroll 4
+        for (int t = sub; t < p; t += 16) {
           const double u = xi[t], v = xj[t];
           a += u * u;
           b += v * v;
           g += u * v;
         }
-        // xor butterflies give bit-identical sums on every lane
-        a = wave_sum(a);
-        b = wave_sum(b);
-        g = wave_sum(g);
+        // xor butterflies inside the 16-lane group: bit-identical sums on its lanes
+        a = group_sum16(a);
+        b = group_sum16(b);
+        g = group_sum16(g);
         if (g != 0.0 && a > negligible && b > negligible && fabs(g) > tol * sqrt(a) * sqrt(b)) {
           const double zeta = (b - a) / (2.0 * g);
           const double t_ = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           const double c = 1.0 / sqrt(1.0 + t_ * t_);
           const double s = c * t_;
-          for (int t = lane; t < p; t += 64) {
+#pragma unroll 4
+          for (int t = sub; t < p; t += 16) {
             const double u = xi[t], v = xj[t];
             xi[t] = c * u - s * v;
             xj[t] = s * u + c * v;
           }
           double *vi = V + static_cast<int64_t>(i) * q;
           double *vj = V + static_cast<int64_t>(j) * q;
-          for (int t = lane; t < q; t += 64) {
+#pragma unroll 4
+          for (int t = sub; t < q; t += 16) {
             const double u = vi[t], v = vj[t];
             vi[t] = c * u - s * v;
             vj[t] = s * u + c * v;
@@ -152,7 +167,7 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
       }
       __syncthreads();
     }
-    if (any && lane == 0) atomicOr(&rotated, 1);
+    if (any && sub == 0) atomicOr(&rotated, 1);
     __syncthreads();
     const int done = (rotated == 0);
     __syncthreads();
@@ -263,7 +278,7 @@ int fs_batched_svd_f64(int nmat, const fs_svd_desc *descs, const double *A, doub
       FS_CHECK_ARG(!needs_work || work, "batched svd: work buffer required for large matrices");
       bt.nmat = nb;
       bt.lds_doubles = static_cast<int32_t>(lds_need);
-      const int threads = qmax >= 96 ? 1024 : (qmax >= 24 ? 512 : 256);
+      const int threads = qmax >= 64 ? 1024 : (qmax >= 32 ? 512 : 256);
       hipLaunchKernelGGL(jacobi_svd_kernel, dim3(nb), dim3(threads),
                          static_cast<size_t>(lds_need) * 8, fs::as_stream(stream), bt);
       FS_LAUNCH_CHECK("jacobi_svd_kernel");

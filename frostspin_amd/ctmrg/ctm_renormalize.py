@@ -16,7 +16,9 @@ from ..misc_tools import block_linalg as bl
 from ..misc_tools.svd_tools import find_block_cuts
 from ..symmetric_tensor.symmetric_tensor import _precision_split
 
-# Blocks whose smallest side exceeds this use the randomized truncated SVD.
+# Blocks up to SMALL_DENSE (smallest side) always use the exact Jacobi SVD; above
+# DENSE_SVD_LIMIT the randomized truncated SVD is the only option.
+SMALL_DENSE = 96
 DENSE_SVD_LIMIT = 1024
 
 
@@ -69,10 +71,16 @@ def construct_projectors(
     ncv = (ncv_ratio * block_chi).astype(int) + 10
     # the reference uses ARPACK (block_chi values) where min(shape) >= max(100, 4 ncv)
     sparse_ref = mins >= np.maximum(100, 4 * ncv)
-    # exact dense SVD where the reference uses LAPACK (and for anything small),
-    # randomized range finder where it uses ARPACK for block_chi values
-    dense_ids = [i for i in range(n) if not sparse_ref[i] and mins[i] <= DENSE_SVD_LIMIT]
-    rand_ids = [i for i in range(n) if sparse_ref[i] or mins[i] > DENSE_SVD_LIMIT]
+    # values needed per sector: block_chi, the reference's own estimate from the
+    # last renormalized corner (what it passes to ARPACK). Small blocks are
+    # decomposed exactly (all values, as the reference's LAPACK branch); larger
+    # ones go through the randomized range finder (GEMM-dominated) for block_chi
+    # values, which is what the reference's sparse branch computes.
+    k_need = np.minimum(block_chi, mins)
+    l_need = np.minimum(k_need + np.maximum(10, k_need // 2), mins)
+    dense = (mins <= SMALL_DENSE) | ((l_need >= 0.8 * mins) & (mins <= DENSE_SVD_LIMIT))
+    dense_ids = [i for i in range(n) if dense[i]]
+    rand_ids = [i for i in range(n) if not dense[i]]
     U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
     if dense_ids:
         U, S, s_off, s_list, VT = bl.svd(M.select(dense_ids))
@@ -81,13 +89,13 @@ def construct_projectors(
             VT_of[i] = (VT, j)
             s_of[i] = s_list[j]
     if rand_ids:
-        U, S, s_off, s_list, VT = bl.rsvd(M.select(rand_ids), block_chi[rand_ids], seed=rsvd_seed)
+        U, S, s_off, s_list, VT = bl.rsvd(M.select(rand_ids), k_need[rand_ids], seed=rsvd_seed)
         for j, i in enumerate(rand_ids):
             U_of[i] = (U, j)
             VT_of[i] = (VT, j)
             s_of[i] = s_list[j]
     for i in range(n):
-        if sparse_ref[i]:
+        if sparse_ref[i] or not dense[i]:
             s_of[i] = s_of[i][: block_chi[i]]
 
     dims = [corner2.irrep_dimension(r) for r in shared]

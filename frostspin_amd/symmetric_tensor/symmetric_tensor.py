@@ -24,6 +24,7 @@ from .. import _ops
 from ..misc_tools.svd_tools import find_block_cuts
 from .diagonal_tensor import DiagonalTensor
 from .tools import check_norm
+from .tools import config as _debug_config
 
 
 def validated_signature(signature0, n_row_reps, n_col_reps):
@@ -149,8 +150,9 @@ class SymmetricTensor:
         assert 0 < self._nrr < self._ndim
         assert self._signature.shape == (self._ndim,)
         assert self._block_irreps.size == self._nblocks
-        assert sorted(set(self._block_irreps.tolist())) == list(self._block_irreps)
-        assert self.check_blocks_fit_representations()
+        if _debug_config["debug_checks"]:
+            assert sorted(set(self._block_irreps.tolist())) == list(self._block_irreps)
+            assert self.check_blocks_fit_representations()
 
     @classmethod
     def _build(cls, row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran=False):
