@@ -68,6 +68,22 @@ class QrDesc(ctypes.Structure):
     ]
 
 
+class CholDesc(ctypes.Structure):
+    _fields_ = [("l", c_i64), ("g_off", c_i64), ("r_off", c_i64), ("shift_rel", c_dbl)]
+
+
+class TrsmDesc(ctypes.Structure):
+    _fields_ = [
+        ("m", c_i64),
+        ("l", c_i64),
+        ("y_off", c_i64),
+        ("y_rs", c_i64),
+        ("y_cs", c_i64),
+        ("r_off", c_i64),
+        ("o_off", c_i64),
+    ]
+
+
 class Copy2dDesc(ctypes.Structure):
     _fields_ = [
         ("rows", c_i64),
@@ -140,7 +156,15 @@ _SIGNATURES = {
         c_int,
         [c_int, ctypes.POINTER(QrDesc), c_vp, c_vp, c_vp, c_vp, c_vp],
     ),
-    "fs_weighted_norm2_f64":(c_int, [c_int, c_i64p, c_dblp, c_vp, c_vp, c_vp, c_vp]),
+    "fs_batched_cholesky_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(CholDesc), c_vp, c_vp, c_vp, c_vp],
+    ),
+    "fs_batched_trsm_right_upper_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(TrsmDesc), c_vp, c_vp, c_vp, c_vp],
+    ),
+    "fs_weighted_norm2_f64": (c_int, [c_int, c_i64p, c_dblp, c_vp, c_vp, c_vp, c_vp]),
     "fs_scale_f64": (c_int, [c_i64, c_dbl, c_vp, c_vp]),
     "fs_scale_rsqrt_dev_f64": (c_int, [c_i64, c_vp, c_vp, c_vp]),
     "fs_axpby_f64": (c_int, [c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),

@@ -298,6 +298,28 @@ def batched_qr(descs, A, Q, R, work):
         check(lib.fs_batched_qr_f64(n, arr, _ptr(A), _ptr(Q), _ptr(R), _ptr(work), _stream()))
 
 
+def batched_cholesky(descs, G, R, status):
+    """descs: sequence of (l, g_off, r_off, shift_rel); status: device int32 (len(descs))."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.CholDesc * n)(*[_lib.CholDesc(int(d[0]), int(d[1]), int(d[2]), float(d[3])) for d in descs])
+    flop = sum(d[0] ** 3 / 3.0 for d in descs)
+    with _timed("cholesky", flop):
+        check(lib.fs_batched_cholesky_f64(n, arr, _ptr(G), _ptr(R), _ptr(status), _stream()))
+
+
+def batched_trsm_right_upper(descs, Y, R, O):
+    """descs: sequence of (m, l, y_off, y_rs, y_cs, r_off, o_off): O = Y R^-1."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.TrsmDesc * n)(*[_lib.TrsmDesc(*map(int, d)) for d in descs])
+    flop = sum(float(d[0]) * d[1] * d[1] for d in descs)
+    with _timed("trsm", flop):
+        check(lib.fs_batched_trsm_right_upper_f64(n, arr, _ptr(Y), _ptr(R), _ptr(O), _stream()))
+
+
 # ---------------------------------------------------------------------------
 # elementwise helpers
 # ---------------------------------------------------------------------------

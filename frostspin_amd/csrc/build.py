@@ -24,6 +24,7 @@ SOURCES = [
     "fs_gemm.hip",
     "fs_svd.hip",
     "fs_qr.hip",
+    "fs_chol.hip",
     "fs_elementwise.hip",
 ]
 ARCH = os.environ.get("FS_OFFLOAD_ARCH", "gfx950")

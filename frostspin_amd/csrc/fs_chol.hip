@@ -1,0 +1,214 @@
+// Cholesky factorisation of small SPD Gram matrices and the right triangular
+// solve Y R^-1: the two non-GEMM steps of (shifted) CholeskyQR, which
+// orthonormalises the tall-skinny range-finder bases of the randomized SVD with
+// the bulk of its flops in grouped MFMA GEMMs (G = Y^T Y).
+//
+// Shifted CholeskyQR3 (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa,
+// SIAM J. Sci. Comput. 42 (2020) A477): the first pass factors
+// G + s I with s = shift_rel * trace(G) (shift_rel = 11 (m l + l (l+1)) u),
+// which keeps the factorisation defined for cond(Y) up to ~u^-1; two more
+// unshifted passes restore orthogonality to O(u).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fs_hip.h"
+
+namespace {
+
+constexpr int kMaxMat = 24;
+constexpr int kThreads = 512;
+
+struct CholDev {
+  int32_t l, pad;
+  int64_t g_off, r_off;
+  double shift_rel;
+};
+
+struct CholBatch {
+  const double *G;
+  double *R;
+  int32_t *status;
+  int32_t nmat, pad;
+  CholDev d[kMaxMat];
+};
+
+// Right-looking upper Cholesky G = R^T R, one workgroup per matrix, in place in
+// R (row-major l x l). status[b] = 0 on success, j+1 if pivot j is not positive.
+__global__ void __launch_bounds__(kThreads) cholesky_kernel(CholBatch bt) {
+  const CholDev D = bt.d[blockIdx.x];
+  const int l = D.l;
+  const double *G = bt.G + D.g_off;
+  double *R = bt.R + D.r_off;
+  const int tid = threadIdx.x;
+  __shared__ double red[kThreads / 64];
+  __shared__ int fail;
+  // trace for the shift
+  double tr = 0.0;
+  for (int i = tid; i < l; i += kThreads) tr += G[static_cast<int64_t>(i) * l + i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tr += __shfl_xor(tr, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = tr;
+  if (tid == 0) fail = 0;
+  __syncthreads();
+  double trace = 0.0;
+  for (int w = 0; w < kThreads / 64; ++w) trace += red[w];
+  const double shift = D.shift_rel * trace;
+  // symmetrise from the upper triangle (G comes from a GEMM; its two halves can
+  // differ in the last bit) and add the shift
+  for (int64_t e = tid; e < static_cast<int64_t>(l) * l; e += kThreads) {
+    const int i = static_cast<int>(e / l), c = static_cast<int>(e - static_cast<int64_t>(i) * l);
+    const int a = min(i, c), b = max(i, c);
+    R[e] = G[static_cast<int64_t>(a) * l + b] + (i == c ? shift : 0.0);
+  }
+  __syncthreads();
+  for (int j = 0; j < l; ++j) {
+    const double piv = R[static_cast<int64_t>(j) * l + j];
+    if (!(piv > 0.0)) {  // also catches NaN
+      if (tid == 0) fail = j + 1;
+      break;  // piv is read by every thread after the same barrier: uniform
+    }
+    const double r = sqrt(piv);
+    const double inv = 1.0 / r;
+    __syncthreads();  // everyone has read the pivot before it is overwritten
+    if (tid == 0) R[static_cast<int64_t>(j) * l + j] = r;
+    for (int c = j + 1 + tid; c < l; c += kThreads) R[static_cast<int64_t>(j) * l + c] *= inv;
+    __syncthreads();
+    // trailing update of the upper triangle: R[i][c] -= R[j][i] R[j][c], j < i <= c
+    const int w = l - j - 1;
+    const int64_t cnt = static_cast<int64_t>(w) * (w + 1) / 2;
+    for (int64_t e = tid; e < cnt; e += kThreads) {
+      // map e to (ii, cc) with 0 <= ii <= cc < w (row-wise packing of the triangle)
+      int ii = static_cast<int>((2.0 * w + 1.0 - sqrt((2.0 * w + 1.0) * (2.0 * w + 1.0) - 8.0 * e)) / 2.0);
+      if (ii < 0) ii = 0;
+      while (ii > 0 && static_cast<int64_t>(ii) * (2 * w - ii + 1) / 2 > e) --ii;
+      while (static_cast<int64_t>(ii + 1) * (2 * w - ii) / 2 <= e) ++ii;
+      const int cc = ii + static_cast<int>(e - static_cast<int64_t>(ii) * (2 * w - ii + 1) / 2);
+      const int i = j + 1 + ii, c = j + 1 + cc;
+      R[static_cast<int64_t>(i) * l + c] -=
+          R[static_cast<int64_t>(j) * l + i] * R[static_cast<int64_t>(j) * l + c];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // zero the strict lower triangle
+  for (int64_t e = tid; e < static_cast<int64_t>(l) * l; e += kThreads) {
+    const int i = static_cast<int>(e / l), c = static_cast<int>(e - static_cast<int64_t>(i) * l);
+    if (c < i) R[e] = 0.0;
+  }
+  if (tid == 0) bt.status[blockIdx.x] = fail;
+}
+
+constexpr int kMaxSolve = 32;
+
+struct SolveDev {
+  int32_t m, l;
+  int64_t y_off, y_rs, y_cs, r_off, o_off;
+};
+
+struct SolveBatch {
+  const double *Y;
+  const double *R;
+  double *O;
+  int32_t n, pad;
+  int64_t block_prefix[kMaxSolve + 1];
+  SolveDev d[kMaxSolve];
+};
+
+// O = Y R^-1 (R upper triangular, row-major l x l): one thread per row of Y,
+// forward substitution x_j = (y_j - sum_{k<j} x_k R[k][j]) / R[j][j]. All lanes
+// read the same R entries (broadcast loads); x lives in the output row.
+__global__ void __launch_bounds__(256) trsm_rows_kernel(SolveBatch b) {
+  const int64_t blk = blockIdx.x;
+  int g = 0;
+  while (g + 1 < b.n && b.block_prefix[g + 1] <= blk) ++g;
+  const SolveDev D = b.d[g];
+  const int64_t row = (blk - b.block_prefix[g]) * 256 + threadIdx.x;
+  if (row >= D.m) return;
+  const double *y = b.Y + D.y_off + row * D.y_rs;
+  const double *R = b.R + D.r_off;
+  double *x = b.O + D.o_off + row * D.l;
+  for (int j = 0; j < D.l; ++j) {
+    double acc = y[static_cast<int64_t>(j) * D.y_cs];
+    for (int k = 0; k < j; ++k) acc -= x[k] * R[static_cast<int64_t>(k) * D.l + j];
+    x[j] = acc / R[static_cast<int64_t>(j) * D.l + j];
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int fs_batched_cholesky_f64(int nmat, const fs_chol_desc *descs, const double *G, double *R,
+                            int32_t *status, void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(nmat >= 0, "batched cholesky: negative count");
+    FS_CHECK_ARG(nmat == 0 || (descs && G && R && status), "batched cholesky: null argument");
+    int i0 = 0;
+    while (i0 < nmat) {
+      CholBatch bt;
+      std::memset(&bt, 0, sizeof(bt));
+      bt.G = G;
+      bt.R = R;
+      bt.status = status + i0;
+      int nb = 0;
+      for (; i0 < nmat && nb < kMaxMat; ++i0) {
+        const fs_chol_desc &s = descs[i0];
+        FS_CHECK_ARG(s.l >= 1 && s.l < (1 << 15), "batched cholesky: bad order");
+        FS_CHECK_ARG(s.shift_rel >= 0.0, "batched cholesky: negative shift");
+        CholDev &d = bt.d[nb++];
+        d.l = static_cast<int32_t>(s.l);
+        d.g_off = s.g_off;
+        d.r_off = s.r_off;
+        d.shift_rel = s.shift_rel;
+      }
+      bt.nmat = nb;
+      hipLaunchKernelGGL(cholesky_kernel, dim3(nb), dim3(kThreads), 0, fs::as_stream(stream), bt);
+      FS_LAUNCH_CHECK("cholesky_kernel");
+    }
+    return FS_OK;
+  });
+}
+
+int fs_batched_trsm_right_upper_f64(int nmat, const fs_trsm_desc *descs, const double *Y,
+                                    const double *R, double *O, void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(nmat >= 0, "batched trsm: negative count");
+    FS_CHECK_ARG(nmat == 0 || (descs && Y && R && O), "batched trsm: null argument");
+    int i0 = 0;
+    while (i0 < nmat) {
+      SolveBatch b;
+      std::memset(&b, 0, sizeof(b));
+      b.Y = Y;
+      b.R = R;
+      b.O = O;
+      int nb = 0;
+      int64_t blocks = 0;
+      for (; i0 < nmat && nb < kMaxSolve; ++i0) {
+        const fs_trsm_desc &s = descs[i0];
+        FS_CHECK_ARG(s.m >= 0 && s.l >= 1, "batched trsm: bad shape");
+        if (s.m == 0) continue;
+        SolveDev &d = b.d[nb];
+        d.m = static_cast<int32_t>(s.m);
+        d.l = static_cast<int32_t>(s.l);
+        d.y_off = s.y_off;
+        d.y_rs = s.y_rs;
+        d.y_cs = s.y_cs;
+        d.r_off = s.r_off;
+        d.o_off = s.o_off;
+        b.block_prefix[nb] = blocks;
+        blocks += fs::ceil_div(s.m, 256);
+        ++nb;
+      }
+      if (nb == 0) continue;
+      b.block_prefix[nb] = blocks;
+      b.n = nb;
+      hipLaunchKernelGGL(trsm_rows_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                         fs::as_stream(stream), b);
+      FS_LAUNCH_CHECK("trsm_rows_kernel");
+    }
+    return FS_OK;
+  });
+}
+
+} // extern "C"

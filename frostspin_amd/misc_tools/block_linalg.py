@@ -138,10 +138,50 @@ def qr(A, *, want_r=True):
     return Q, R
 
 
+def cholqr(Y, *, shifted=True, passes=3):
+    """Orthonormal basis of the column space of each Y_i (m x l, m >= l) by
+    (shifted) CholeskyQR: per pass G = Y^T Y (grouped GEMM), R = chol(G + s I),
+    Y <- Y R^-1. Shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 42
+    (2020) A477) stays defined up to cond(Y) ~ 1/eps. Matrices whose Cholesky
+    breaks down anyway are finished by Householder QR."""
+    eps = np.finfo(np.float64).eps
+    cur = Y
+    for p in range(passes):
+        G = gemm(cur.T(), cur)
+        R = Mats.packed(cur.cols, cur.cols)
+        status = _ops.zeros(len(cur), dtype=torch.int32)
+        shift = [
+            11.0 * (float(cur.rows[i]) * cur.cols[i] + cur.cols[i] * (cur.cols[i] + 1)) * eps
+            if (shifted and p == 0) else 0.0
+            for i in range(len(cur))
+        ]
+        _ops.batched_cholesky(
+            [(cur.cols[i], G.off[i], R.off[i], shift[i]) for i in range(len(cur))], G.buf, R.buf, status
+        )
+        out = Mats.packed(cur.rows, cur.cols)
+        _ops.batched_trsm_right_upper(
+            [(cur.rows[i], cur.cols[i], cur.off[i], cur.rs[i], cur.cs[i], R.off[i], out.off[i])
+             for i in range(len(cur))],
+            cur.buf, R.buf, out.buf,
+        )
+        bad = np.nonzero(_ops.download(status))[0]
+        if bad.size:
+            # breakdown (numerically rank-deficient beyond the shift): Householder
+            # on the original inputs of those matrices
+            Qh, _ = qr(Y.select(bad), want_r=False)
+            descs = [
+                (Qh.rows[j], Qh.cols[j], Qh.off[j], Qh.rs[j], Qh.cs[j], out.off[i], out.cols[i], 1,
+                 0, 0, 0, 1.0, 0.0)
+                for j, i in enumerate(bad)
+            ]
+            _ops.copy2d(descs, Qh.buf, out.buf)
+        cur = out
+    return cur
+
+
 def _orthonormal_columns(Y):
-    """Orthonormal basis of the column space of each Y_i (Householder Q factor)."""
-    Q, _ = qr(Y, want_r=False)
-    return Q
+    """Orthonormal basis of the column space of each Y_i."""
+    return cholqr(Y)
 
 
 def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):

@@ -213,6 +213,39 @@ int fs_batched_qr_f64(int nmat, const fs_qr_desc *descs, const double *A,
                       double *Q, double *R, double *work, void *stream);
 
 /* ------------------------------------------------------------------------- */
+/* CholeskyQR building blocks (randomized truncated SVD)                     */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Upper Cholesky G + s I = R^T R of symmetric positive (semi)definite
+ * l x l matrices (row-major at G + g_off, upper triangle read), with
+ * s = shift_rel * trace(G); R written row-major at R + r_off. status[b] (device
+ * int32) = 0 on success, j+1 if pivot j was not positive. These replace the
+ * orthonormalisation that ARPACK performs inside sparse_svd
+ * (misc_tools/svd_tools.py:154-197) for the randomized range finder.
+ */
+typedef struct fs_chol_desc {
+  int64_t l, g_off, r_off;
+  double shift_rel;
+} fs_chol_desc;
+
+int fs_batched_cholesky_f64(int nmat, const fs_chol_desc *descs,
+                            const double *G, double *R, int32_t *status,
+                            void *stream);
+
+/*
+ * O = Y R^-1 with R upper triangular (l x l row-major at R + r_off), Y (m x l)
+ * strided (y_rs, y_cs) at Y + y_off, O row-major (m x l) at O + o_off.
+ */
+typedef struct fs_trsm_desc {
+  int64_t m, l, y_off, y_rs, y_cs, r_off, o_off;
+} fs_trsm_desc;
+
+int fs_batched_trsm_right_upper_f64(int nmat, const fs_trsm_desc *descs,
+                                    const double *Y, const double *R, double *O,
+                                    void *stream);
+
+/* ------------------------------------------------------------------------- */
 /* Elementwise / reduction helpers on packed block buffers                   */
 /* ------------------------------------------------------------------------- */
 

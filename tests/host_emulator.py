@@ -91,6 +91,35 @@ def batched_qr(descs, A, Q, R, work):
             rb[ro : ro + q * n] = rr.reshape(-1)
 
 
+def batched_cholesky(descs, G, R, status):
+    g, r, st = _np(G), _np(R), _np(status)
+    for b, (l, go, ro, shift_rel) in enumerate(descs):
+        gm = g[go : go + l * l].reshape(l, l)
+        gm = np.triu(gm) + np.triu(gm, 1).T
+        gm = gm + shift_rel * np.trace(gm) * np.eye(l)
+        try:
+            rr = np.linalg.cholesky(gm).T
+            st[b] = 0
+        except np.linalg.LinAlgError:
+            rr = np.zeros((l, l))
+            st[b] = 1
+        r[ro : ro + l * l] = rr.reshape(-1)
+
+
+def batched_trsm_right_upper(descs, Y, R, O):
+    y, r, o = _np(Y), _np(R), _np(O)
+    for m, l, yo, yrs, ycs, ro, oo in descs:
+        if m == 0:
+            continue
+        ym = _view(y, yo, m, l, yrs, ycs)
+        rm = r[ro : ro + l * l].reshape(l, l)
+        with np.errstate(all="ignore"):  # like the kernel: a zero pivot gives inf/nan
+            x = np.zeros((m, l))
+            for j in range(l):
+                x[:, j] = (ym[:, j] - x[:, :j] @ rm[:j, j]) / rm[j, j]
+        o[oo : oo + m * l] = x.reshape(-1)
+
+
 def weighted_norm2(x, seg_off, weights):
     xs = _np(x)
     tot = 0.0
@@ -157,7 +186,8 @@ def install(monkeypatch):
     monkeypatch.setattr(_ops, "device", lambda: torch.device("cpu"))
     monkeypatch.setattr(_ops.PermutePlan, "execute", _execute)
     for name in (
-        "grouped_gemm", "batched_svd", "batched_qr", "weighted_norm2", "scale_", "scale_rsqrt_dev_",
+        "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_trsm_right_upper",
+        "weighted_norm2", "scale_", "scale_rsqrt_dev_",
         "axpby_", "copy2d", "weighted_trace", "fill_normal_",
     ):
         monkeypatch.setattr(_ops, name, globals()[name])
