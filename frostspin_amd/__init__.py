@@ -9,6 +9,7 @@ torch provides device buffers and streams only.
 from .symmetric_tensor import (
     AsymmetricTensor,
     DiagonalTensor,
+    SU2SymmetricTensor,
     U1SymmetricTensor,
     get_gemm_precision,
     get_symmetric_tensor_type,
@@ -22,6 +23,7 @@ __all__ = [
     "ASSERT_TOL",
     "AsymmetricTensor",
     "DiagonalTensor",
+    "SU2SymmetricTensor",
     "U1SymmetricTensor",
     "__version__",
     "get_gemm_precision",

@@ -140,6 +140,16 @@ _SIGNATURES = {
     "fs_permute_plan_execute_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "fs_permute_plan_execute_f32": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "fs_permute_plan_destroy": (c_int, [c_vp]),
+    "fs_lie_plan_create": (
+        c_int,
+        [c_i64, c_i64p, c_i64, c_i64p, c_i64, c_i64p, c_i64, c_dblp, c_i64, c_i64, ctypes.POINTER(c_vp)],
+    ),
+    "fs_lie_plan_check": (
+        c_int, [c_i64, c_i64p, c_i64, c_i64p, c_i64, c_i64p, c_i64, c_i64, c_i64],
+    ),
+    "fs_lie_plan_info": (c_int, [c_vp, c_i64p, c_dblp, c_dblp]),
+    "fs_lie_plan_execute_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
+    "fs_lie_plan_destroy": (c_int, [c_vp]),
     "fs_grouped_gemm_f64": (
         c_int,
         [c_int, ctypes.POINTER(GemmGroup), c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp],

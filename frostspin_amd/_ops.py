@@ -249,6 +249,55 @@ def permute(plan, src, dtype=DTYPE):
     return out
 
 
+class LiePlan:
+    """Owner of a native structural-transform plan (fs_lie_plan_*).
+
+    tasks (n, 8), rows (n, 3), digits (n, 5): int64 tables laid out as
+    fs_lie_task / fs_lie_row / fs_lie_digit; w: transposed structural matrices.
+    The host tables are kept for inspection."""
+
+    __slots__ = ("_handle", "coeffs", "digits", "dst_size", "flops", "rows", "src_size", "tasks", "w")
+
+    def __init__(self, tasks, rows, digits, w, src_size, dst_size):
+        self._handle = None
+        self.tasks = np.ascontiguousarray(tasks, dtype=np.int64).reshape(-1, 8)
+        self.rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1, 3)
+        self.digits = np.ascontiguousarray(digits, dtype=np.int64).reshape(-1, 5)
+        self.w = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        self.src_size = int(src_size)
+        self.dst_size = int(dst_size)
+        handle = ctypes.c_void_p()
+        check(
+            lib.fs_lie_plan_create(
+                self.tasks.shape[0], _lib.i64p(self.tasks),
+                self.rows.shape[0], _lib.i64p(self.rows),
+                self.digits.shape[0], _lib.i64p(self.digits),
+                self.w.size, _lib.dblp(self.w),
+                self.src_size, self.dst_size, ctypes.byref(handle),
+            )
+        )
+        self._handle = handle.value
+        nt = ctypes.c_int64()
+        fl = ctypes.c_double()
+        co = ctypes.c_double()
+        check(lib.fs_lie_plan_info(self._handle, ctypes.byref(nt), ctypes.byref(fl), ctypes.byref(co)))
+        self.flops = fl.value
+        self.coeffs = co.value
+
+    def execute(self, src, dst):
+        assert dst.dtype == torch.float64 and dst.numel() >= self.dst_size
+        assert src is None or src.numel() >= self.src_size
+        with _timed("lie", 8.0 * self.coeffs):
+            check(lib.fs_lie_plan_execute_f64(
+                self._handle, None if src is None else _ptr(src), _ptr(dst), _stream()))
+
+    def __del__(self):
+        h = self._handle
+        if h is not None and lib is not None:
+            lib.fs_lie_plan_destroy(h)
+            self._handle = None
+
+
 # ---------------------------------------------------------------------------
 # grouped GEMM
 # ---------------------------------------------------------------------------

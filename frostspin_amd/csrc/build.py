@@ -25,6 +25,7 @@ SOURCES = [
     "fs_svd.hip",
     "fs_qr.hip",
     "fs_chol.hip",
+    "fs_lie.hip",
     "fs_elementwise.hip",
 ]
 ARCH = os.environ.get("FS_OFFLOAD_ARCH", "gfx950")

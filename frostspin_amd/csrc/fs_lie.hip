@@ -1,0 +1,346 @@
+// Structural transforms of SU(2) tensors: permute, dense -> blocks and
+// blocks -> dense as one planned kernel.
+//
+// Each task is an elementary block (one irrep per leg) holding n_in input rows
+// and n_out output rows of E coefficients; the kernel computes
+// out[o, :] = sum_i W[o, i] in[i, :] with every row gathered from / scattered
+// to its own strided place in the block (or dense) buffers. One 64-lane wave
+// owns one (task, tile of 64 external indices): each lane decodes its external
+// multi-index once, stages its input column in LDS (own column only, so no
+// barrier is needed), then produces the output rows 8 at a time with the
+// transposed structural matrix read as wave-uniform scalar loads. The lane
+// order follows the output's fastest external index, so stores coalesce.
+// This replaces the per-simple-block numba loops of
+// lie_group_symmetric_tensor.py:96-265 (transpose-reshape + small GEMM +
+// slice) and the per-elementary-block projections of lie:856-1109.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "fs_hip.h"
+
+namespace {
+
+constexpr int kLanes = 64;
+constexpr int kOutChunk = 8;
+constexpr int kMaxLdsRows = 96;  // 96 * 64 * 8 B = 48 KiB of LDS per wave
+
+struct TaskDev {
+  int64_t w_off;
+  int32_t e_size, n_in, n_out, n_out_pad;
+  int32_t row0, ndig, dig0, pad;
+};
+
+struct RowDev {
+  int64_t base, rs, cs;
+};
+
+struct DigitDev {
+  int32_t radix, in_r, in_c, out_r, out_c;
+};
+
+struct TileDev {
+  int32_t task, e0;
+};
+
+__global__ void __launch_bounds__(kLanes)
+lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restrict__ tasks,
+                     const RowDev *__restrict__ rows, const DigitDev *__restrict__ digits,
+                     const double *__restrict__ w, const double *__restrict__ src,
+                     double *__restrict__ dst, int lds_rows) {
+  extern __shared__ double xs[];
+  const TileDev tile = tiles[blockIdx.x];
+  const TaskDev tk = tasks[tile.task];
+  const int lane = threadIdx.x;
+  const int32_t e = tile.e0 + lane;
+  const bool active = e < tk.e_size;
+
+  int64_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
+  if (active) {
+    int32_t rem = e;
+    for (int k = tk.ndig - 1; k >= 0; --k) {
+      const DigitDev d = digits[tk.dig0 + k];
+      const int32_t q = rem / d.radix;
+      const int32_t r = rem - q * d.radix;
+      rem = q;
+      er_in += static_cast<int64_t>(r) * d.in_r;
+      ec_in += static_cast<int64_t>(r) * d.in_c;
+      er_out += static_cast<int64_t>(r) * d.out_r;
+      ec_out += static_cast<int64_t>(r) * d.out_c;
+    }
+  }
+
+  const RowDev *in_rows = rows + tk.row0;
+  const RowDev *out_rows = in_rows + tk.n_in;
+  const bool staged = tk.n_in <= lds_rows;
+  if (staged) {
+    for (int i = 0; i < tk.n_in; ++i) {
+      const RowDev r = in_rows[i];
+      double v = 0.0;
+      if (active && r.base >= 0) v = src[r.base + er_in * r.rs + ec_in * r.cs];
+      xs[i * kLanes + lane] = v;
+    }
+  }
+
+  for (int o0 = 0; o0 < tk.n_out; o0 += kOutChunk) {
+    double acc[kOutChunk];
+#pragma unroll
+    for (int r = 0; r < kOutChunk; ++r) acc[r] = 0.0;
+    const double *wt = w + tk.w_off + o0;
+    for (int i = 0; i < tk.n_in; ++i) {
+      double x;
+      if (staged) {
+        x = xs[i * kLanes + lane];
+      } else {
+        const RowDev r = in_rows[i];
+        x = (active && r.base >= 0) ? src[r.base + er_in * r.rs + ec_in * r.cs] : 0.0;
+      }
+      const double *wi = wt + static_cast<int64_t>(i) * tk.n_out_pad;
+#pragma unroll
+      for (int r = 0; r < kOutChunk; ++r) acc[r] = fma(wi[r], x, acc[r]);
+    }
+    if (active) {
+      const int nr = min(kOutChunk, tk.n_out - o0);
+      for (int r = 0; r < nr; ++r) {
+        const RowDev ro = out_rows[o0 + r];
+        dst[ro.base + er_out * ro.rs + ec_out * ro.cs] = acc[r];
+      }
+    }
+  }
+}
+
+} // namespace
+
+struct fs_lie_plan {
+  int device = -1;
+  void *d_mem = nullptr;
+  const TileDev *tiles = nullptr;
+  const TaskDev *tasks = nullptr;
+  const RowDev *rows = nullptr;
+  const DigitDev *digits = nullptr;
+  const double *w = nullptr;
+  int64_t ntiles = 0;
+  int32_t lds_rows = 0;
+  int64_t src_size = 0, dst_size = 0;
+  double flops = 0.0, coeffs = 0.0;
+};
+
+namespace {
+
+int check_task(const fs_lie_task &t, int64_t nrows, const fs_lie_row *rows, int64_t ndigits,
+               const fs_lie_digit *digits, int64_t nw, int64_t src_size, int64_t dst_size) {
+  FS_CHECK_ARG(t.e_size >= 1 && t.e_size < (int64_t(1) << 31), "lie plan: bad external size");
+  FS_CHECK_ARG(t.n_in >= 0 && t.n_out >= 0 && t.n_in < (1 << 24) && t.n_out < (1 << 24),
+               "lie plan: bad row counts");
+  FS_CHECK_ARG(t.n_out_pad >= t.n_out && t.n_out_pad % kOutChunk == 0,
+               "lie plan: n_out_pad must be a multiple of 8 >= n_out");
+  FS_CHECK_ARG(t.ndig >= 1 && t.ndig <= FS_LIE_MAX_LEGS, "lie plan: bad digit count");
+  FS_CHECK_ARG(t.dig0 >= 0 && t.dig0 + t.ndig <= ndigits, "lie plan: digits out of range");
+  FS_CHECK_ARG(t.row0 >= 0 && t.row0 + t.n_in + t.n_out <= nrows, "lie plan: rows out of range");
+  FS_CHECK_ARG(t.w_off >= 0 && t.w_off + t.n_in * t.n_out_pad <= nw, "lie plan: W out of range");
+  int64_t prod = 1, mr_in = 0, mc_in = 0, mr_out = 0, mc_out = 0;
+  for (int64_t k = 0; k < t.ndig; ++k) {
+    const fs_lie_digit &d = digits[t.dig0 + k];
+    FS_CHECK_ARG(d.radix >= 1 && d.in_r >= 0 && d.in_c >= 0 && d.out_r >= 0 && d.out_c >= 0,
+                 "lie plan: bad digit");
+    FS_CHECK_ARG(d.in_r < (int64_t(1) << 31) && d.in_c < (int64_t(1) << 31) &&
+                     d.out_r < (int64_t(1) << 31) && d.out_c < (int64_t(1) << 31),
+                 "lie plan: digit stride too large");
+    prod *= d.radix;
+    FS_CHECK_ARG(prod <= t.e_size, "lie plan: digits exceed external size");
+    mr_in += (d.radix - 1) * d.in_r;
+    mc_in += (d.radix - 1) * d.in_c;
+    mr_out += (d.radix - 1) * d.out_r;
+    mc_out += (d.radix - 1) * d.out_c;
+    FS_CHECK_ARG(std::max(std::max(mr_in, mc_in), std::max(mr_out, mc_out)) < (int64_t(1) << 62),
+                 "lie plan: external index too large");
+  }
+  FS_CHECK_ARG(prod == t.e_size, "lie plan: digit radices do not multiply to the external size");
+  for (int64_t i = 0; i < t.n_in + t.n_out; ++i) {
+    const fs_lie_row &r = rows[t.row0 + i];
+    const bool input = i < t.n_in;
+    if (input && r.base < 0) continue;  // absent input block
+    FS_CHECK_ARG(r.base >= 0 && r.rs >= 0 && r.cs >= 0, "lie plan: bad row entry");
+    int64_t a = 0, b = 0, last = 0;
+    const bool ovf = __builtin_mul_overflow(input ? mr_in : mr_out, r.rs, &a) ||
+                     __builtin_mul_overflow(input ? mc_in : mc_out, r.cs, &b) ||
+                     __builtin_add_overflow(r.base, a, &last) ||
+                     __builtin_add_overflow(last, b, &last);
+    FS_CHECK_ARG(!ovf && last < (input ? src_size : dst_size),
+                 "lie plan: row entry %lld out of bounds", static_cast<long long>(t.row0 + i));
+  }
+  return FS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
+                       const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
+                       int64_t nw, const double *w, int64_t src_size, int64_t dst_size,
+                       fs_lie_plan **plan) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(plan != nullptr, "lie plan: null output");
+    *plan = nullptr;
+    FS_CHECK_ARG(ntasks >= 0 && nrows >= 0 && ndigits >= 0 && nw >= 0, "lie plan: negative size");
+    FS_CHECK_ARG(ntasks == 0 || (tasks && rows && digits), "lie plan: null table");
+    FS_CHECK_ARG(nw == 0 || w, "lie plan: null W");
+    FS_CHECK_ARG(src_size >= 0 && dst_size >= 0, "lie plan: negative buffer size");
+    FS_CHECK_ARG(nrows < (int64_t(1) << 31) && ndigits < (int64_t(1) << 31),
+                 "lie plan: tables too large");
+    auto p = std::make_unique<fs_lie_plan>();
+    std::vector<TaskDev> tdev;
+    std::vector<TileDev> tiles;
+    tdev.reserve(static_cast<size_t>(ntasks));
+    int64_t max_in = 0;
+    for (int64_t t = 0; t < ntasks; ++t) {
+      const fs_lie_task &s = tasks[t];
+      int st = check_task(s, nrows, rows, ndigits, digits, nw, src_size, dst_size);
+      if (st != FS_OK) return st;
+      TaskDev d;
+      d.w_off = s.w_off;
+      d.e_size = static_cast<int32_t>(s.e_size);
+      d.n_in = static_cast<int32_t>(s.n_in);
+      d.n_out = static_cast<int32_t>(s.n_out);
+      d.n_out_pad = static_cast<int32_t>(s.n_out_pad);
+      d.row0 = static_cast<int32_t>(s.row0);
+      d.ndig = static_cast<int32_t>(s.ndig);
+      d.dig0 = static_cast<int32_t>(s.dig0);
+      d.pad = 0;
+      const int32_t id = static_cast<int32_t>(tdev.size());
+      tdev.push_back(d);
+      if (s.n_out == 0) continue;
+      max_in = std::max(max_in, s.n_in);
+      for (int64_t e0 = 0; e0 < s.e_size; e0 += kLanes) tiles.push_back({id, static_cast<int32_t>(e0)});
+      p->flops += 2.0 * double(s.n_in) * double(s.n_out) * double(s.e_size);
+      p->coeffs += double(s.n_in + s.n_out) * double(s.e_size);
+    }
+    FS_CHECK_ARG(tiles.size() < (size_t(1) << 31), "lie plan: too many tiles");
+    std::vector<RowDev> rdev(static_cast<size_t>(nrows));
+    for (int64_t i = 0; i < nrows; ++i) rdev[i] = {rows[i].base, rows[i].rs, rows[i].cs};
+    std::vector<DigitDev> ddev(static_cast<size_t>(ndigits));
+    for (int64_t i = 0; i < ndigits; ++i) {
+      const fs_lie_digit &d = digits[i];
+      ddev[i] = {static_cast<int32_t>(d.radix), static_cast<int32_t>(d.in_r),
+                 static_cast<int32_t>(d.in_c), static_cast<int32_t>(d.out_r),
+                 static_cast<int32_t>(d.out_c)};
+    }
+    // one device allocation, 256-byte aligned parts
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t b_tiles = tiles.size() * sizeof(TileDev), b_tasks = tdev.size() * sizeof(TaskDev);
+    const size_t b_rows = rdev.size() * sizeof(RowDev), b_dig = ddev.size() * sizeof(DigitDev);
+    const size_t b_w = static_cast<size_t>(nw) * sizeof(double);
+    const size_t total = al(b_tiles) + al(b_tasks) + al(b_rows) + al(b_dig) + al(b_w) + 256;
+    FS_HIP_CHECK(hipGetDevice(&p->device));
+    char *mem = nullptr;
+    FS_HIP_CHECK(hipMalloc(&mem, total));
+    p->d_mem = mem;
+    size_t off = 0;
+    auto put = [&](size_t bytes) -> char * {
+      char *d = mem + off;
+      off += al(bytes);
+      return bytes ? d : nullptr;
+    };
+    char *d_tiles = put(b_tiles);
+    char *d_tasks = put(b_tasks);
+    char *d_rows = put(b_rows);
+    char *d_dig = put(b_dig);
+    char *d_w = put(b_w);
+    int st = FS_OK;
+    auto copy = [&](char *d, const void *h, size_t bytes) {
+      if (st == FS_OK && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        fs::set_error("lie plan: upload failed");
+        st = FS_ERR_HIP;
+      }
+    };
+    copy(d_tiles, tiles.data(), b_tiles);
+    copy(d_tasks, tdev.data(), b_tasks);
+    copy(d_rows, rdev.data(), b_rows);
+    copy(d_dig, ddev.data(), b_dig);
+    copy(d_w, w, b_w);
+    if (st != FS_OK) {
+      (void)hipFree(mem);
+      return st;
+    }
+    p->tiles = reinterpret_cast<const TileDev *>(d_tiles);
+    p->tasks = reinterpret_cast<const TaskDev *>(d_tasks);
+    p->rows = reinterpret_cast<const RowDev *>(d_rows);
+    p->digits = reinterpret_cast<const DigitDev *>(d_dig);
+    p->w = reinterpret_cast<const double *>(d_w);
+    p->ntiles = static_cast<int64_t>(tiles.size());
+    p->lds_rows = static_cast<int32_t>(std::min<int64_t>(max_in, kMaxLdsRows));
+    p->src_size = src_size;
+    p->dst_size = dst_size;
+    *plan = p.release();
+    return FS_OK;
+  });
+}
+
+int fs_lie_plan_check(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
+                      const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
+                      int64_t nw, int64_t src_size, int64_t dst_size) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(ntasks >= 0 && nrows >= 0 && ndigits >= 0 && nw >= 0, "lie plan: negative size");
+    FS_CHECK_ARG(ntasks == 0 || (tasks && rows && digits), "lie plan: null table");
+    for (int64_t t = 0; t < ntasks; ++t) {
+      int st = check_task(tasks[t], nrows, rows, ndigits, digits, nw, src_size, dst_size);
+      if (st != FS_OK) return st;
+    }
+    return FS_OK;
+  });
+}
+
+int fs_lie_plan_info(const fs_lie_plan *plan, int64_t *ntiles, double *flops, double *coeffs) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(plan != nullptr, "lie plan: null plan");
+    if (ntiles) *ntiles = plan->ntiles;
+    if (flops) *flops = plan->flops;
+    if (coeffs) *coeffs = plan->coeffs;
+    return FS_OK;
+  });
+}
+
+int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *dst,
+                            void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(plan != nullptr, "lie plan: null plan");
+    if (plan->ntiles == 0) return FS_OK;
+    FS_CHECK_ARG(dst != nullptr, "lie plan: null destination");
+    FS_CHECK_ARG(src != nullptr || plan->src_size == 0, "lie plan: null source");
+    int dev = 0;
+    FS_HIP_CHECK(hipGetDevice(&dev));
+    FS_CHECK_ARG(dev == plan->device, "lie plan was built on device %d, current device is %d",
+                 plan->device, dev);
+    const size_t lds = static_cast<size_t>(plan->lds_rows) * kLanes * sizeof(double);
+    hipLaunchKernelGGL(lie_transform_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
+                       dim3(kLanes), lds, fs::as_stream(stream), plan->tiles, plan->tasks,
+                       plan->rows, plan->digits, plan->w, src, dst, plan->lds_rows);
+    FS_LAUNCH_CHECK("lie_transform_kernel");
+    return FS_OK;
+  });
+}
+
+int fs_lie_plan_destroy(fs_lie_plan *plan) {
+  return fs::guarded([&]() -> int {
+    if (!plan) return FS_OK;
+    if (plan->d_mem) {
+      int cur = 0;
+      FS_HIP_CHECK(hipGetDevice(&cur));
+      if (cur != plan->device) FS_HIP_CHECK(hipSetDevice(plan->device));
+      hipError_t e = hipFree(plan->d_mem);
+      if (cur != plan->device) (void)hipSetDevice(cur);
+      plan->d_mem = nullptr;
+      if (e != hipSuccess) {
+        delete plan;
+        fs::set_error("hipFree failed: %s", hipGetErrorString(e));
+        return FS_ERR_HIP;
+      }
+    }
+    delete plan;
+    return FS_OK;
+  });
+}
+
+} // extern "C"

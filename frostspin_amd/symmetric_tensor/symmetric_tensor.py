@@ -444,12 +444,16 @@ class SymmetricTensor:
             return self
         if row_axes == tuple(range(self._nrr, self._ndim)) and col_axes == tuple(range(self._nrr)):
             return self.transpose()
+        return self._permuted(axes, len(row_axes))
+
+    def _permuted(self, axes, nrr):
+        """New tensor with legs `axes` (first nrr are rows); data moved by the
+        symmetry's _permute_data."""
         signature = np.empty((self._ndim,), dtype=bool)
         reps = [None] * self._ndim
         for i, ax in enumerate(axes):
             signature[i] = self._signature[ax]
             reps[i] = self._row_reps[ax] if ax < self._nrr else self._col_reps[ax - self._nrr]
-        nrr = len(row_axes)
         if self._nblocks == 0:
             data, shapes, block_irreps = _ops.empty(0), np.zeros((0, 2), np.int64), self._block_irreps
         else:

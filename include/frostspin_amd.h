@@ -135,6 +135,86 @@ int fs_permute_plan_execute_f32(fs_permute_plan *plan, const float *src,
 int fs_permute_plan_destroy(fs_permute_plan *plan);
 
 /* ------------------------------------------------------------------------- */
+/* Structural transforms of non-abelian (SU(2)) tensors                      */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Data movement of Lie-group tensors (SU(2)) in one planned kernel. A plan is
+ * a list of tasks; a task is one elementary ("simple") block: one irrep per
+ * leg, E = product of the legs' external degeneracies. Its coefficients form
+ * n_in input rows and n_out output rows of E entries each (structural x
+ * external), and the task computes
+ *
+ *     out[o, e] = sum_i W[o, i] * in[i, e],      e = 0 .. E-1,
+ *
+ * where e is a row-major multi-index over the task's digits (one per leg,
+ * radix = that leg's external degeneracy). Digit k contributes
+ * digit*in_r / digit*in_c to the input row/column index and digit*out_r /
+ * digit*out_c to the output ones; the element of row entry r at (er, ec) is
+ * at r.base + er*r.rs + ec*r.cs. An input row with base < 0 is an absent
+ * block and reads as zero. W is stored transposed per task: W^T[i, o] at
+ * w[w_off + i*n_out_pad + o], n_out_pad a multiple of 8 >= n_out.
+ *
+ * Permute (W = CG-tree overlap unitaries, lie_group_symmetric_tensor.py:
+ * 96-265, 432-835, 1116-1134), dense -> blocks (W = CG projector / irrep
+ * dimension, lie:856-967) and blocks -> dense (W = CG projector, lie:969-1109)
+ * are all plans of this form; every output coefficient is written exactly
+ * once by the permute plans, so their destination needs no initialisation.
+ *
+ * fs_lie_plan_create validates every access against src_size / dst_size
+ * (in doubles), copies the tables to the current device and returns an
+ * opaque plan.
+ */
+#define FS_LIE_MAX_LEGS 16
+
+typedef struct fs_lie_task {
+  int64_t e_size;
+  int64_t w_off;
+  int64_t n_in;
+  int64_t n_out;
+  int64_t n_out_pad;
+  int64_t row0;   /* n_in input row entries, then n_out output row entries */
+  int64_t ndig;
+  int64_t dig0;
+} fs_lie_task;
+
+typedef struct fs_lie_row {
+  int64_t base;
+  int64_t rs;
+  int64_t cs;
+} fs_lie_row;
+
+typedef struct fs_lie_digit {
+  int64_t radix;
+  int64_t in_r;
+  int64_t in_c;
+  int64_t out_r;
+  int64_t out_c;
+} fs_lie_digit;
+
+typedef struct fs_lie_plan fs_lie_plan;
+
+int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
+                       const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
+                       int64_t nw, const double *w, int64_t src_size, int64_t dst_size,
+                       fs_lie_plan **plan);
+
+/* Validation of create's tables alone (no device work): FS_OK or
+ * FS_ERR_INVALID with the reason in fs_last_error(). */
+int fs_lie_plan_check(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
+                      const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
+                      int64_t nw, int64_t src_size, int64_t dst_size);
+
+/* Work of one execution: flops (2 * sum n_in*n_out*E) and coefficients read
+ * and written (sum (n_in + n_out) * E). */
+int fs_lie_plan_info(const fs_lie_plan *plan, int64_t *ntiles, double *flops, double *coeffs);
+
+int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *dst,
+                            void *stream);
+
+int fs_lie_plan_destroy(fs_lie_plan *plan);
+
+/* ------------------------------------------------------------------------- */
 /* Grouped GEMM over uneven blocks                                           */
 /* ------------------------------------------------------------------------- */
 

@@ -177,6 +177,60 @@ def fill_normal_(x, seed):
     x.copy_(torch.from_numpy(rng.standard_normal(x.numel())))
 
 
+class EmulatedLiePlan:
+    """fs_lie_plan semantics in numpy; tables are validated by the native
+    host-only checker (fs_lie_plan_check) exactly as fs_lie_plan_create does."""
+
+    def __init__(self, tasks, rows, digits, w, src_size, dst_size):
+        from frostspin_amd import _lib
+
+        self.tasks = np.ascontiguousarray(tasks, dtype=np.int64).reshape(-1, 8)
+        self.rows = np.ascontiguousarray(rows, dtype=np.int64).reshape(-1, 3)
+        self.digits = np.ascontiguousarray(digits, dtype=np.int64).reshape(-1, 5)
+        self.w = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        self.src_size, self.dst_size = int(src_size), int(dst_size)
+        _lib.check(_lib.lib.fs_lie_plan_check(
+            self.tasks.shape[0], _lib.i64p(self.tasks), self.rows.shape[0], _lib.i64p(self.rows),
+            self.digits.shape[0], _lib.i64p(self.digits), self.w.size, self.src_size, self.dst_size))
+        t = self.tasks
+        self.flops = float((2 * t[:, 2] * t[:, 3] * t[:, 0]).sum())
+        self.coeffs = float(((t[:, 2] + t[:, 3]) * t[:, 0]).sum())
+
+    def execute(self, src, dst):
+        s = _np(src) if src is not None else np.zeros((0,))
+        d = _np(dst)
+        for E, w_off, n_in, n_out, pad, row0, ndig, dig0 in self.tasks:
+            if n_out == 0:
+                continue
+            e = np.arange(E)
+            er_i = np.zeros_like(e)
+            ec_i = np.zeros_like(e)
+            er_o = np.zeros_like(e)
+            ec_o = np.zeros_like(e)
+            rem = e.copy()
+            for radix, in_r, in_c, out_r, out_c in self.digits[dig0 + ndig - 1 : dig0 - 1 if dig0 else None : -1]:
+                r = rem % radix
+                rem //= radix
+                er_i += r * in_r
+                ec_i += r * in_c
+                er_o += r * out_r
+                ec_o += r * out_c
+            X = np.zeros((n_in, E))
+            for i in range(n_in):
+                base, rs, cs = self.rows[row0 + i]
+                if base >= 0:
+                    idx = base + er_i * rs + ec_i * cs
+                    assert idx.min() >= 0 and idx.max() < self.src_size
+                    X[i] = s[idx]
+            Wt = self.w[w_off : w_off + n_in * pad].reshape(n_in, pad)[:, :n_out]
+            Y = Wt.T @ X
+            for o in range(n_out):
+                base, rs, cs = self.rows[row0 + n_in + o]
+                idx = base + er_o * rs + ec_o * cs
+                assert idx.min() >= 0 and idx.max() < self.dst_size
+                d[idx] = Y[o]
+
+
 def _execute(self, src, dst):
     _permute_execute(self, src, dst)
 
@@ -185,6 +239,7 @@ def install(monkeypatch):
     """Route frostspin_amd._ops device calls to the numpy emulation (CPU tensors)."""
     monkeypatch.setattr(_ops, "device", lambda: torch.device("cpu"))
     monkeypatch.setattr(_ops.PermutePlan, "execute", _execute)
+    monkeypatch.setattr(_ops, "LiePlan", EmulatedLiePlan)
     for name in (
         "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_trsm_right_upper",
         "weighted_norm2", "scale_", "scale_rsqrt_dev_",
