@@ -195,14 +195,20 @@ def kernel_roofline(summary):
 
 
 def run_ctmrg(args):
-    """U(1) XXZ (Jz=0.5, Jperp=1) finite-temperature CTMRG on the AB/BA cell,
-    seeded by simple update at (D, beta); one step = one SequentialCTMRG.iterate()."""
+    """Finite-temperature CTMRG on the AB/BA cell, seeded by simple update at
+    (D, beta); one step = one SequentialCTMRG.iterate(). SU(2): spin-1/2
+    Heisenberg model (S.S); U1: XXZ model (Jz=0.5, Jperp=1)."""
     import torch
 
     from frostspin_amd import _ops
     from frostspin_amd.ctmrg import models
 
-    h = models.xxz_bond(0.5, 1.0)
+    if args.symmetry == "SU2":
+        h = models.heisenberg_bond("SU2")
+        model = "SU(2) spin-1/2 Heisenberg"
+    else:
+        h = models.xxz_bond(0.5, 1.0)
+        model = "U(1) XXZ (Jz=0.5, Jperp=1)"
     t0 = time.perf_counter()
     su, ctm = models.thermal_ctmrg(h, args.D, args.beta, args.chi)
     torch.cuda.synchronize()
@@ -230,7 +236,7 @@ def run_ctmrg(args):
     step_ms = np.array(step_ms)
     ms = wall * 1000.0 / args.steps
     gemm = kr.get("gemm", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
-    perm = kr.get("permute", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
+    perm = kr.get("lie" if args.symmetry == "SU2" else "permute", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
     res = {
         "metric": METRIC,
         "value": 1000.0 / ms,
@@ -245,11 +251,10 @@ def run_ctmrg(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (simple-update thermal state of the XXZ model, no checkpoint)",
+        "data": "synthetic (simple-update thermal state, no checkpoint)",
         "config": {
-            "workload": f"U(1) XXZ (Jz=0.5, Jperp=1) CTMRG, AB/BA cell, D={args.D}, "
-            f"chi={args.chi}, beta={su.beta:.3g} (simple update tau=0.01); "
-            "SU(2) symmetry not built: U(1) stands in",
+            "workload": f"{model} finite-temperature CTMRG, AB/BA cell, D={args.D}, "
+            f"chi={args.chi}, beta={su.beta:.3g} (simple update tau=0.01), FP64",
             "chi_values": [int(c) for c in ctm.chi_values],
             "energy_per_bond": energy,
             "simple_update_s": t_su,
@@ -287,6 +292,7 @@ def main():
     ap.add_argument("--chi", type=int, default=64)
     ap.add_argument("--D", type=int, default=6)
     ap.add_argument("--beta", type=float, default=1.0)
+    ap.add_argument("--symmetry", default="SU2", choices=["SU2", "U1"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--debug-checks", action="store_true")
     args = ap.parse_args()

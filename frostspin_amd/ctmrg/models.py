@@ -6,10 +6,32 @@ examples/heisenberg_simple_update.py:22-111 of frostspin.
 import numpy as np
 
 from ..simple_update.simple_update import SimpleUpdate
+from ..symmetric_tensor.su2_symmetric_tensor import SU2SymmetricTensor
 from ..symmetric_tensor.u1_symmetric_tensor import U1SymmetricTensor
 from .sequential_ctmrg import SequentialCTMRG
 
 SPIN_HALF_U1 = np.array([1, -1], dtype=np.int8)  # 2 S_z of |up>, |down>
+SPIN_HALF_SU2 = np.array([[1], [2]])  # one spin-1/2 multiplet
+
+# S.S on two spins 1/2 (examples/heisenberg_simple_update.py:25-32)
+SDS_22 = np.array(
+    [
+        [0.25, 0.0, 0.0, 0.0],
+        [0.0, -0.25, 0.5, 0.0],
+        [0.0, 0.5, -0.25, 0.0],
+        [0.0, 0.0, 0.0, 0.25],
+    ]
+).reshape(2, 2, 2, 2)
+
+
+def heisenberg_bond(symmetry="SU2"):
+    """h = S.S as a (2, 2 | 2, 2) SU(2) or U(1) tensor."""
+    if symmetry == "SU2":
+        reps = (SPIN_HALF_SU2, SPIN_HALF_SU2)
+        return SU2SymmetricTensor.from_array(SDS_22, reps, reps)
+    if symmetry == "U1":
+        return xxz_bond(1.0, 1.0)
+    raise ValueError(f"unsupported symmetry {symmetry}")
 
 
 def xxz_bond(jz=1.0, jperp=1.0):
