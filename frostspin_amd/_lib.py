@@ -72,6 +72,18 @@ class CholDesc(ctypes.Structure):
     _fields_ = [("l", c_i64), ("g_off", c_i64), ("r_off", c_i64), ("shift_rel", c_dbl)]
 
 
+class CholInvDesc(ctypes.Structure):
+    _fields_ = [
+        ("l", c_i64),
+        ("g_off", c_i64),
+        ("g_stride", c_i64),
+        ("n_part", c_i64),
+        ("r_off", c_i64),
+        ("rinv_off", c_i64),
+        ("shift_rel", c_dbl),
+    ]
+
+
 class TrsmDesc(ctypes.Structure):
     _fields_ = [
         ("m", c_i64),
@@ -169,6 +181,10 @@ _SIGNATURES = {
     "fs_batched_cholesky_f64": (
         c_int,
         [c_int, ctypes.POINTER(CholDesc), c_vp, c_vp, c_vp, c_vp],
+    ),
+    "fs_batched_cholinv_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(CholInvDesc), c_vp, c_vp, c_vp, c_vp, c_vp],
     ),
     "fs_batched_trsm_right_upper_f64": (
         c_int,

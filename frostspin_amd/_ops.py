@@ -358,6 +358,25 @@ def batched_cholesky(descs, G, R, status):
         check(lib.fs_batched_cholesky_f64(n, arr, _ptr(G), _ptr(R), _ptr(status), _stream()))
 
 
+CHOLINV_MAX_ORDER = 128
+
+
+def batched_cholinv(descs, G, R, Rinv, status):
+    """descs: (l, g_off, g_stride, n_part, r_off, rinv_off, shift_rel): G is the sum
+    of n_part l x l partials at g_off + p*g_stride; R may be None. Writes the upper
+    Cholesky factor (if R) and its inverse; status: device int32 (len(descs))."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.CholInvDesc * n)(
+        *[_lib.CholInvDesc(*(int(x) for x in d[:6]), float(d[6])) for d in descs]
+    )
+    flop = sum(d[0] ** 3 / 3.0 + d[0] ** 3 / 3.0 for d in descs)
+    with _timed("cholinv", flop):
+        check(lib.fs_batched_cholinv_f64(
+            n, arr, _ptr(G), None if R is None else _ptr(R), _ptr(Rinv), _ptr(status), _stream()))
+
+
 def batched_trsm_right_upper(descs, Y, R, O):
     """descs: sequence of (m, l, y_off, y_rs, y_cs, r_off, o_off): O = Y R^-1."""
     n = len(descs)

@@ -135,9 +135,233 @@ __global__ void __launch_bounds__(256) trsm_rows_kernel(SolveBatch b) {
   }
 }
 
+// Cholesky and triangular inverse of one small SPD Gram matrix per workgroup,
+// entirely in LDS: A = sym(G) + shift I = R^T R, then R^-1 in place, so that
+// the solve Y R^-1 of CholeskyQR becomes one grouped MFMA GEMM instead of a
+// sequential substitution per row. Order l <= kCholInvMaxOrder.
+constexpr int kCholInvMaxOrder = 128;
+constexpr int kCholInvThreads = 256;
+
+struct CholInvDev {
+  int32_t l, n_part;
+  int64_t g_off, g_stride, r_off, rinv_off;
+  double shift_rel;
+};
+
+struct CholInvBatch {
+  const double *G;
+  double *R;  // may be null: R not written
+  double *Rinv;
+  int32_t *status;
+  int32_t nmat, pad;
+  CholInvDev d[kMaxMat];
+};
+
+// Register-tiled: a 16 x 16 grid of threads, thread (ty, tx) owns the 8 x 8
+// elements (ty + 16p, tx + 16q) of the trailing matrix (later of X = R^-1).
+// One barrier per step: the owners of the pivot row publish it unscaled in a
+// double-buffered LDS row, then every thread applies the branch-free rank-1
+// update to its registers (rows that must not change get a zero factor).
+// Finished rows of R go to an LDS copy (row stride l_max + 1: the inverse reads
+// its columns conflict-free); finished rows of X go straight to global memory.
+//   Cholesky (right-looking): r_j = a_j / sqrt(a_jj);  a_ic -= a_ji a_jc / a_jj
+//   inverse (R X = I, rows bottom-up): X_i = B_i / R_ii;  B_k -= R_ki X_i, k < i
+constexpr int kCIGrid = 16;
+constexpr int kCITile = kCholInvMaxOrder / kCIGrid;  // 8
+constexpr int kCIRowStride = kCholInvMaxOrder + 1;
+constexpr size_t kCholInvLds = static_cast<size_t>(kCholInvMaxOrder) * kCIRowStride * sizeof(double);
+
+__global__ void __launch_bounds__(kCholInvThreads) cholinv_kernel(CholInvBatch bt) {
+  extern __shared__ double Rl[];  // l x kCIRowStride: rows of R
+  const CholInvDev D = bt.d[blockIdx.x];
+  const int l = D.l;
+  const double *G = bt.G + D.g_off;
+  const int tx = threadIdx.x & (kCIGrid - 1);
+  const int ty = threadIdx.x / kCIGrid;
+  __shared__ double rowbuf[2][kCholInvMaxOrder];
+  __shared__ double red[kCholInvThreads / 64];
+  auto g_at = [&](int i, int k) {
+    double v = 0.0;
+    for (int p = 0; p < D.n_part; ++p) v += G[p * D.g_stride + static_cast<int64_t>(i) * l + k];
+    return v;
+  };
+  double tr = 0.0;
+  if (tx == ty) {
+#pragma unroll
+    for (int p = 0; p < kCITile; ++p) {
+      const int i = ty + kCIGrid * p;
+      if (i < l) tr += g_at(i, i);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tr += __shfl_xor(tr, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = tr;
+  __syncthreads();
+  double trace = 0.0;
+#pragma unroll
+  for (int w = 0; w < kCholInvThreads / 64; ++w) trace += red[w];
+  const double shift = D.shift_rel * trace;
+
+  double a[kCITile][kCITile];
+#pragma unroll
+  for (int p = 0; p < kCITile; ++p) {
+#pragma unroll
+    for (int q = 0; q < kCITile; ++q) {
+      const int i = ty + kCIGrid * p, c = tx + kCIGrid * q;
+      // upper triangle of the symmetrised Gram matrix (+ shift); the lower
+      // triangle of the registers is never read
+      a[p][q] = (i < l && c < l && i <= c) ? g_at(i, c) + (i == c ? shift : 0.0) : 0.0;
+    }
+  }
+  int fail = 0;
+  for (int j = 0; j < l; ++j) {
+    const int jp = j / kCIGrid, jt = j % kCIGrid, buf = j & 1;
+    if (ty == jt) {
+#pragma unroll
+      for (int q = 0; q < kCITile; ++q) {
+        double v = a[0][q];
+#pragma unroll
+        for (int p = 1; p < kCITile; ++p) v = p == jp ? a[p][q] : v;
+        const int c = tx + kCIGrid * q;
+        if (c < l) rowbuf[buf][c] = v;
+      }
+    }
+    __syncthreads();
+    const double piv = rowbuf[buf][j];
+    if (!(piv > 0.0)) {  // uniform: every thread read the same value after the barrier
+      fail = j + 1;
+      break;
+    }
+    const double rinv = 1.0 / sqrt(piv);
+    const double pinv = rinv * rinv;
+    if (ty == 0) {
+#pragma unroll
+      for (int q = 0; q < kCITile; ++q) {
+        const int c = tx + kCIGrid * q;
+        if (c < l) Rl[j * kCIRowStride + c] = c < j ? 0.0 : (c == j ? piv * rinv : rowbuf[buf][c] * rinv);
+      }
+    }
+    double rc[kCITile];
+#pragma unroll
+    for (int q = 0; q < kCITile; ++q) {
+      const int c = tx + kCIGrid * q;
+      rc[q] = (c > j && c < l) ? rowbuf[buf][c] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < kCITile; ++p) {
+      const int i = ty + kCIGrid * p;
+      const double ri = (i > j && i < l) ? rowbuf[buf][i] * pinv : 0.0;
+#pragma unroll
+      for (int q = 0; q < kCITile; ++q) a[p][q] = fma(-ri, rc[q], a[p][q]);
+    }
+  }
+  if (fail) {
+    if (threadIdx.x == 0) bt.status[blockIdx.x] = fail;
+    return;
+  }
+  __syncthreads();  // all rows of R are in LDS
+  if (bt.R) {
+    double *R = bt.R + D.r_off;
+    for (int e = threadIdx.x; e < l * l; e += kCholInvThreads) {
+      const int i = e / l, c = e - i * l;
+      R[e] = Rl[i * kCIRowStride + c];
+    }
+  }
+  double *X = bt.Rinv + D.rinv_off;
+  double x[kCITile][kCITile];
+#pragma unroll
+  for (int p = 0; p < kCITile; ++p) {
+#pragma unroll
+    for (int q = 0; q < kCITile; ++q) x[p][q] = (ty + kCIGrid * p == tx + kCIGrid * q) ? 1.0 : 0.0;
+  }
+  for (int i = l - 1; i >= 0; --i) {
+    const int ip = i / kCIGrid, it = i % kCIGrid, buf = i & 1;
+    if (ty == it) {  // row i of X is final: scale, publish, store
+      const double dinv = 1.0 / Rl[i * kCIRowStride + i];
+#pragma unroll
+      for (int q = 0; q < kCITile; ++q) {
+        double v = x[0][q];
+#pragma unroll
+        for (int p = 1; p < kCITile; ++p) v = p == ip ? x[p][q] : v;
+        const int c = tx + kCIGrid * q;
+        if (c < l) {
+          v = c >= i ? v * dinv : 0.0;
+          rowbuf[buf][c] = v;
+          X[i * l + c] = v;
+        }
+      }
+    }
+    __syncthreads();
+    double rx[kCITile];
+#pragma unroll
+    for (int q = 0; q < kCITile; ++q) {
+      const int c = tx + kCIGrid * q;
+      rx[q] = (c >= i && c < l) ? rowbuf[buf][c] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < kCITile; ++p) {
+      const int k = ty + kCIGrid * p;
+      const double ck = k < i ? Rl[k * kCIRowStride + i] : 0.0;
+#pragma unroll
+      for (int q = 0; q < kCITile; ++q) x[p][q] = fma(-ck, rx[q], x[p][q]);
+    }
+  }
+  if (threadIdx.x == 0) bt.status[blockIdx.x] = 0;
+}
+
 } // namespace
 
 extern "C" {
+
+int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double *G, double *R,
+                           double *Rinv, int32_t *status, void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(nmat >= 0, "batched cholinv: negative count");
+    FS_CHECK_ARG(nmat == 0 || (descs && G && Rinv && status), "batched cholinv: null argument");
+    for (int i = 0; i < nmat; ++i) {
+      FS_CHECK_ARG(descs[i].l >= 1 && descs[i].l <= kCholInvMaxOrder,
+                   "batched cholinv: order must be in [1, %d]", kCholInvMaxOrder);
+      FS_CHECK_ARG(descs[i].shift_rel >= 0.0, "batched cholinv: negative shift");
+      FS_CHECK_ARG(descs[i].n_part >= 1 && descs[i].n_part <= (1 << 16) && descs[i].g_stride >= 0,
+                   "batched cholinv: bad partial count / stride");
+    }
+    static thread_local int attr_device = -1;
+    int dev = 0;
+    FS_HIP_CHECK(hipGetDevice(&dev));
+    if (attr_device != dev) {
+      FS_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(cholinv_kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(kCholInvLds)));
+      attr_device = dev;
+    }
+    int i0 = 0;
+    while (i0 < nmat) {
+      CholInvBatch bt;
+      std::memset(&bt, 0, sizeof(bt));
+      bt.G = G;
+      bt.R = R;
+      bt.Rinv = Rinv;
+      bt.status = status + i0;
+      int nb = 0;
+      for (; i0 < nmat && nb < kMaxMat; ++i0) {
+        const fs_cholinv_desc &s = descs[i0];
+        CholInvDev &d = bt.d[nb++];
+        d.l = static_cast<int32_t>(s.l);
+        d.n_part = static_cast<int32_t>(s.n_part);
+        d.g_off = s.g_off;
+        d.g_stride = s.g_stride;
+        d.r_off = s.r_off;
+        d.rinv_off = s.rinv_off;
+        d.shift_rel = s.shift_rel;
+      }
+      bt.nmat = nb;
+      hipLaunchKernelGGL(cholinv_kernel, dim3(nb), dim3(kCholInvThreads), kCholInvLds,
+                         fs::as_stream(stream), bt);
+      FS_LAUNCH_CHECK("cholinv_kernel");
+    }
+    return FS_OK;
+  });
+}
 
 int fs_batched_cholesky_f64(int nmat, const fs_chol_desc *descs, const double *G, double *R,
                             int32_t *status, void *stream) {

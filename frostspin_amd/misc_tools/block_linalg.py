@@ -138,50 +138,102 @@ def qr(A, *, want_r=True):
     return Q, R
 
 
-def cholqr(Y, *, shifted=True, passes=3):
+GRAM_ROWS_PER_PART = 192
+GRAM_MAX_PARTS = 16
+
+
+def gram(Y):
+    """Y_i^T Y_i for every i as a split-K grouped GEMM: rows of Y_i are cut into
+    n_i chunks, each chunk's l x l product is a separate group (so even l x l
+    outputs fill the device); partials are stored back to back (l*l apart) and
+    summed by the consumer. Returns (Mats of the first partials, n_i)."""
+    nparts = np.clip((Y.rows + GRAM_ROWS_PER_PART - 1) // GRAM_ROWS_PER_PART, 1, GRAM_MAX_PARTS)
+    l2 = Y.cols * Y.cols
+    sizes = nparts * l2
+    off = np.zeros_like(sizes)
+    if sizes.size > 1:
+        off[1:] = np.cumsum(sizes[:-1])
+    buf = _ops.empty(int(sizes.sum()))
+    groups = []
+    for i in range(len(Y)):
+        m, l = int(Y.rows[i]), int(Y.cols[i])
+        bounds = np.linspace(0, m, int(nparts[i]) + 1).astype(np.int64)
+        for p in range(int(nparts[i])):
+            r0, r1 = int(bounds[p]), int(bounds[p + 1])
+            a_off = Y.off[i] + r0 * Y.rs[i]
+            groups.append((l, l, r1 - r0, a_off, Y.cs[i], Y.rs[i], a_off, Y.rs[i], Y.cs[i],
+                           off[i] + p * l2[i], l))
+    A = Y.buf
+    _ops.grouped_gemm(groups, A, A, buf)
+    return Mats(buf, off, Y.cols.copy(), Y.cols.copy(), Y.cols.copy(), np.ones_like(Y.cols)), nparts
+
+
+def cholqr(Y, *, shifted=True, passes=3, want_r=False):
     """Orthonormal basis of the column space of each Y_i (m x l, m >= l) by
-    (shifted) CholeskyQR: per pass G = Y^T Y (grouped GEMM), R = chol(G + s I),
-    Y <- Y R^-1. Shifted CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 42
-    (2020) A477) stays defined up to cond(Y) ~ 1/eps. Matrices whose Cholesky
-    breaks down anyway are finished by Householder QR."""
+    (shifted) CholeskyQR: per pass G = Y^T Y (grouped GEMM), R = chol(G + s I)
+    and R^-1 (one fused LDS kernel), Y <- Y R^-1 (grouped GEMM). Shifted
+    CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 42 (2020) A477) stays
+    defined up to cond(Y) ~ 1/eps. Matrices whose Cholesky breaks down anyway
+    are finished by Householder QR. With want_r, also returns the triangular
+    factor R (l x l, Y = Q R) as the product of the passes' factors."""
     eps = np.finfo(np.float64).eps
     cur = Y
+    Rtot = None
     for p in range(passes):
-        G = gemm(cur.T(), cur)
-        R = Mats.packed(cur.cols, cur.cols)
+        G, nparts = gram(cur)
         status = _ops.zeros(len(cur), dtype=torch.int32)
         shift = [
             11.0 * (float(cur.rows[i]) * cur.cols[i] + cur.cols[i] * (cur.cols[i] + 1)) * eps
             if (shifted and p == 0) else 0.0
             for i in range(len(cur))
         ]
-        _ops.batched_cholesky(
-            [(cur.cols[i], G.off[i], R.off[i], shift[i]) for i in range(len(cur))], G.buf, R.buf, status
-        )
-        out = Mats.packed(cur.rows, cur.cols)
-        _ops.batched_trsm_right_upper(
-            [(cur.rows[i], cur.cols[i], cur.off[i], cur.rs[i], cur.cs[i], R.off[i], out.off[i])
-             for i in range(len(cur))],
-            cur.buf, R.buf, out.buf,
-        )
+        R = Mats.packed(cur.cols, cur.cols) if want_r else None
+        if len(cur) and cur.cols.max() <= _ops.CHOLINV_MAX_ORDER:
+            Rinv = Mats.packed(cur.cols, cur.cols)
+            _ops.batched_cholinv(
+                [(cur.cols[i], G.off[i], cur.cols[i] ** 2, nparts[i], 0 if R is None else R.off[i],
+                  Rinv.off[i], shift[i])
+                 for i in range(len(cur))],
+                G.buf, None if R is None else R.buf, Rinv.buf, status,
+            )
+            out = gemm(cur, Rinv)
+        else:
+            if R is None:
+                R = Mats.packed(cur.cols, cur.cols)
+            if np.any(nparts > 1):
+                G = gemm(cur.T(), cur)
+            _ops.batched_cholesky(
+                [(cur.cols[i], G.off[i], R.off[i], shift[i]) for i in range(len(cur))], G.buf, R.buf, status
+            )
+            out = Mats.packed(cur.rows, cur.cols)
+            _ops.batched_trsm_right_upper(
+                [(cur.rows[i], cur.cols[i], cur.off[i], cur.rs[i], cur.cs[i], R.off[i], out.off[i])
+                 for i in range(len(cur))],
+                cur.buf, R.buf, out.buf,
+            )
+        if want_r:
+            Rtot = R if Rtot is None else gemm(R, Rtot)
         bad = np.nonzero(_ops.download(status))[0]
         if bad.size:
             # breakdown (numerically rank-deficient beyond the shift): Householder
             # on the original inputs of those matrices
-            Qh, _ = qr(Y.select(bad), want_r=False)
-            descs = [
-                (Qh.rows[j], Qh.cols[j], Qh.off[j], Qh.rs[j], Qh.cs[j], out.off[i], out.cols[i], 1,
-                 0, 0, 0, 1.0, 0.0)
-                for j, i in enumerate(bad)
-            ]
-            _ops.copy2d(descs, Qh.buf, out.buf)
+            Qh, Rh = qr(Y.select(bad), want_r=want_r)
+            for src, dst in ((Qh, out), (Rh, Rtot)):
+                if src is None:
+                    continue
+                descs = [
+                    (src.rows[j], src.cols[j], src.off[j], src.rs[j], src.cs[j], dst.off[i], dst.cols[i], 1,
+                     0, 0, 0, 1.0, 0.0)
+                    for j, i in enumerate(bad)
+                ]
+                _ops.copy2d(descs, src.buf, dst.buf)
         cur = out
-    return cur
+    return (cur, Rtot) if want_r else cur
 
 
-def _orthonormal_columns(Y):
+def _orthonormal_columns(Y, passes=3):
     """Orthonormal basis of the column space of each Y_i."""
-    return cholqr(Y)
+    return cholqr(Y, passes=passes)
 
 
 def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):
@@ -195,16 +247,18 @@ def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):
     omega = Mats.packed(A.cols, ls)
     _ops.fill_normal_(omega.buf, seed)
     Y = gemm(A, omega)
+    # intermediate bases only need to be well conditioned (Halko et al. sec. 4.5):
+    # two CholeskyQR passes; the final basis gets the full three
     for _ in range(power_iters):
-        Q = _orthonormal_columns(Y)
+        Q = _orthonormal_columns(Y, passes=2)
         Z = gemm(A.T(), Q)
-        Qz = _orthonormal_columns(Z)
+        Qz = _orthonormal_columns(Z, passes=2)
         Y = gemm(A, Qz)
     Q = _orthonormal_columns(Y)
     B = gemm(Q.T(), A)  # l x n
-    # B^T = Q2 R2 (Householder), then the small l x l SVD of R2^T = Ur S Vr^T:
-    # B = Ur S (Q2 Vr)^T
-    Q2, R2 = qr(B.T())
+    # B^T = Q2 R2 (shifted CholeskyQR3), then the small l x l SVD of
+    # R2^T = Ur S Vr^T: B = Ur S (Q2 Vr)^T
+    Q2, R2 = cholqr(B.T(), want_r=True)
     Ur, S, s_off, s_list, Vrt = svd(R2.T())
     U = gemm(Q, Ur)  # m x l
     VT = gemm(Vrt, Q2.T())  # l x n

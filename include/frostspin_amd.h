@@ -317,6 +317,27 @@ int fs_batched_cholesky_f64(int nmat, const fs_chol_desc *descs,
  * O = Y R^-1 with R upper triangular (l x l row-major at R + r_off), Y (m x l)
  * strided (y_rs, y_cs) at Y + y_off, O row-major (m x l) at O + o_off.
  */
+/*
+ * Cholesky and triangular inverse in one launch: R = chol(sym(G) + shift I)
+ * (upper, G = R^T R) and Rinv = R^-1, both row-major l x l, one workgroup per
+ * matrix in LDS (l <= 128). R may be NULL (not written). status[i] = 0 on
+ * success, j+1 when pivot j is not positive (R, Rinv unspecified). The solve
+ * Y R^-1 of CholeskyQR then runs as a grouped GEMM with Rinv. Replaces the
+ * QR step of the reference's randomized/ARPACK sector SVDs like
+ * fs_batched_cholesky_f64 (misc_tools/svd_tools.py:154-197).
+ */
+typedef struct fs_cholinv_desc {
+  int64_t l;
+  int64_t g_off;     /* G = sum_{p < n_part} of the l x l matrices at g_off + p*g_stride */
+  int64_t g_stride;  /* (partial Gram products of a split-K GEMM) */
+  int64_t n_part;
+  int64_t r_off, rinv_off;
+  double shift_rel;
+} fs_cholinv_desc;
+
+int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double *G,
+                           double *R, double *Rinv, int32_t *status, void *stream);
+
 typedef struct fs_trsm_desc {
   int64_t m, l, y_off, y_rs, y_cs, r_off, o_off;
 } fs_trsm_desc;

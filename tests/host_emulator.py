@@ -106,6 +106,25 @@ def batched_cholesky(descs, G, R, status):
         r[ro : ro + l * l] = rr.reshape(-1)
 
 
+def batched_cholinv(descs, G, R, Rinv, status):
+    g, ri, st = _np(G), _np(Rinv), _np(status)
+    r = None if R is None else _np(R)
+    for b, (l, go, gs, npart, ro, rio, shift_rel) in enumerate(descs):
+        assert 1 <= l <= _ops.CHOLINV_MAX_ORDER
+        gm = sum(g[go + p * gs : go + p * gs + l * l].reshape(l, l) for p in range(npart))
+        gm = np.triu(gm) + np.triu(gm, 1).T
+        gm = gm + shift_rel * np.trace(gm) * np.eye(l)
+        try:
+            rr = np.linalg.cholesky(gm).T
+            st[b] = 0
+        except np.linalg.LinAlgError:
+            st[b] = 1
+            continue
+        if r is not None:
+            r[ro : ro + l * l] = rr.reshape(-1)
+        ri[rio : rio + l * l] = np.linalg.inv(rr).reshape(-1)
+
+
 def batched_trsm_right_upper(descs, Y, R, O):
     y, r, o = _np(Y), _np(R), _np(O)
     for m, l, yo, yrs, ycs, ro, oo in descs:
@@ -241,7 +260,8 @@ def install(monkeypatch):
     monkeypatch.setattr(_ops.PermutePlan, "execute", _execute)
     monkeypatch.setattr(_ops, "LiePlan", EmulatedLiePlan)
     for name in (
-        "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_trsm_right_upper",
+        "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_cholinv",
+        "batched_trsm_right_upper",
         "weighted_norm2", "scale_", "scale_rsqrt_dev_",
         "axpby_", "copy2d", "weighted_trace", "fill_normal_",
     ):

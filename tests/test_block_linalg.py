@@ -92,3 +92,19 @@ def test_dense_svd_lds_and_global_paths(backend):
         ref = np.linalg.svd(a, compute_uv=False)
         assert np.abs(s - ref).max() < 1e-13 * ref[0]
         assert np.abs((u * s) @ vt - a).max() < 1e-12 * ref[0]
+
+
+def test_cholqr_q_and_r(backend):
+    """Shifted CholeskyQR3: fused Cholesky+inverse path (l <= 128), the
+    Cholesky+substitution path (l > 128), ill-conditioned inputs (cond 1e12)
+    and the Householder fallback on exact rank deficiency."""
+    rng = np.random.default_rng(4)
+    mats = [rng.standard_normal((300, 40)), _decaying(rng, 500, 69, 0.67)[0], rng.standard_normal((400, 150))]
+    deficient = rng.standard_normal((200, 30))
+    deficient[:, 5:10] = 0.0
+    mats.append(deficient)
+    Q, R = bl.cholqr(_pack(mats), want_r=True)
+    for a, q, r in zip(mats, _unpack(Q), _unpack(R), strict=True):
+        assert np.abs(q.T @ q - np.eye(q.shape[1])).max() < 1e-12
+        assert np.abs(np.tril(r, -1)).max() < 1e-12 * np.abs(r).max()
+        assert np.abs(q @ r - a).max() < 1e-12 * np.abs(a).max()
