@@ -37,8 +37,12 @@ class KernelTimer:
 
     active = None
 
-    def __init__(self):
+    def __init__(self, *, pad_cycles=0):
         self.records = []  # (kind, work, start_event, end_event)
+        # pad_cycles > 0: a short spin kernel ahead of each timed launch keeps the
+        # GPU busy while the host enqueues it, so the event pair brackets the
+        # kernel itself rather than the host's launch latency
+        self.pad_cycles = int(pad_cycles)
 
     def __enter__(self):
         KernelTimer.active = self
@@ -68,7 +72,10 @@ class _timed:
         self.e0 = None
 
     def __enter__(self):
-        if KernelTimer.active is not None:
+        kt = KernelTimer.active
+        if kt is not None:
+            if kt.pad_cycles:
+                torch.cuda._sleep(kt.pad_cycles)
             self.e0 = torch.cuda.Event(enable_timing=True)
             self.e0.record()
         return self

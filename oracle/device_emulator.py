@@ -1,10 +1,15 @@
-"""Test-only emulation of the device entry points wrapped by frostspin_amd._ops.
+"""Oracle: numpy restatement of every device entry point (TEST INFRASTRUCTURE
+ONLY).
 
-CPU tests install it (fixture `emulated_device`) to run the product's host logic
-(block bookkeeping, offsets, strides, plan tables, GEMM/SVD descriptors) without
-a GPU: every `_ops` device function is replaced by a numpy implementation of the
-kernel's documented semantics, operating on torch CPU tensors. The product never
-imports this module; GPU tests run the real kernels.
+Each function reproduces the documented semantics of one kernel behind
+frostspin_amd._ops (grouped GEMM, planned gathers, structural transforms,
+batched SVD/QR/Cholesky, reductions) on torch CPU tensors with numpy/LAPACK.
+Installed in place of the device functions it runs the product's host logic
+(block bookkeeping, plan tables, descriptors) on the CPU:
+* CPU tests (fixture `emulated_device`) check that host logic without a GPU;
+* bench.py's cpu_baseline leg times one step of the same algorithm on the
+  host cores.
+The product never imports this module.
 """
 
 import numpy as np
@@ -252,6 +257,30 @@ class EmulatedLiePlan:
 
 def _execute(self, src, dst):
     _permute_execute(self, src, dst)
+
+
+class _Patcher:
+    """Minimal monkeypatch stand-in that can undo its changes."""
+
+    def __init__(self):
+        self._saved = []
+
+    def setattr(self, obj, name, value):
+        self._saved.append((obj, name, getattr(obj, name)))
+        setattr(obj, name, value)
+
+    def undo(self):
+        for obj, name, value in reversed(self._saved):
+            setattr(obj, name, value)
+        self._saved.clear()
+
+
+def install_global():
+    """Install the emulation process-wide; returns a callable that restores the
+    device functions."""
+    p = _Patcher()
+    install(p)
+    return p.undo
 
 
 def install(monkeypatch):

@@ -33,10 +33,10 @@ def pytest_collection_modifyitems(config, items):
 @pytest.fixture
 def emulated_device(monkeypatch):
     """Run product host code with the numpy emulation of the device entry points."""
-    from tests import host_emulator
+    from oracle import device_emulator
 
-    host_emulator.install(monkeypatch)
-    return host_emulator
+    device_emulator.install(monkeypatch)
+    return device_emulator
 
 
 @pytest.fixture
