@@ -34,6 +34,7 @@ if ROOT not in sys.path:
 METRIC = "CTMRG iters/s SU(2) χ=256 mixed vs plain FP64 GPU at 1e-8 obs err; p50/p99"
 PEAK_FP64_MFMA = 78.6e12
 PEAK_HBM = 8.0e12
+PROFILE_MARKER_CYCLES = 20_000_000
 
 
 def leg_rep():
@@ -195,7 +196,10 @@ def kernel_roofline(summary):
     out = {}
     for kind, d in summary.items():
         rate = d["work"] / (d["ms"] * 1e-3) if d["ms"] > 0 else 0.0
-        out[kind] = {"launches": d["launches"], "ms": d["ms"], "work": d["work"], "rate": rate}
+        out[kind] = {
+            "launches": d["launches"], "dispatches": d["dispatches"], "ms": d["ms"], "work": d["work"],
+            "rate": rate,
+        }
     return out
 
 
@@ -254,6 +258,10 @@ def run_ctmrg(args):
         ctm.iterate()
     torch.cuda.synchronize()
     step_ms = []
+    if args.profile_markers:
+        # a long spin kernel on each side of the timed region: lets a kernel
+        # trace be cut to exactly these iterations
+        torch.cuda._sleep(PROFILE_MARKER_CYCLES)
     t_all = time.perf_counter()
     for _ in range(args.steps):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -265,6 +273,9 @@ def run_ctmrg(args):
         step_ms.append(e0.elapsed_time(e1))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_all
+    if args.profile_markers:
+        torch.cuda._sleep(PROFILE_MARKER_CYCLES)
+        torch.cuda.synchronize()
     # one extra iteration under the kernel timer for the roofline of each kernel class
     with _ops.KernelTimer(pad_cycles=200_000) as kt:
         ctm.iterate()
@@ -306,7 +317,8 @@ def run_ctmrg(args):
             "frac": gemm["rate"] / PEAK_FP64_MFMA,
             "traffic": None,
             "launches": gemm["launches"],
-            "avg_ms": gemm["ms"] / max(gemm["launches"], 1),
+            "dispatches": gemm["dispatches"],
+            "avg_ms": gemm["ms"] / max(gemm["dispatches"], 1),
         },
         "kernels": kr,
         "permute": {
@@ -352,6 +364,8 @@ def main():
     ap.add_argument("--symmetry", default="SU2", choices=["SU2", "U1"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--debug-checks", action="store_true")
+    ap.add_argument("--profile-markers", action="store_true",
+                    help="bracket the timed iterations with two long spin kernels (kernel-trace cuts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(

@@ -54,21 +54,23 @@ class KernelTimer:
     def summary(self):
         torch.cuda.synchronize()
         out = {}
-        for kind, work, e0, e1 in self.records:
+        for kind, work, e0, e1, disp in self.records:
             ms = e0.elapsed_time(e1)
-            d = out.setdefault(kind, {"launches": 0, "ms": 0.0, "work": 0.0})
+            d = out.setdefault(kind, {"launches": 0, "dispatches": 0, "ms": 0.0, "work": 0.0})
             d["launches"] += 1
+            d["dispatches"] += disp
             d["ms"] += ms
             d["work"] += work
         return out
 
 
 class _timed:
-    __slots__ = ("kind", "work", "e0")
+    __slots__ = ("dispatches", "e0", "kind", "work")
 
-    def __init__(self, kind, work):
+    def __init__(self, kind, work, dispatches=1):
         self.kind = kind
         self.work = work
+        self.dispatches = dispatches
         self.e0 = None
 
     def __enter__(self):
@@ -84,7 +86,7 @@ class _timed:
         if self.e0 is not None and KernelTimer.active is not None:
             e1 = torch.cuda.Event(enable_timing=True)
             e1.record()
-            KernelTimer.active.records.append((self.kind, self.work, self.e0, e1))
+            KernelTimer.active.records.append((self.kind, self.work, self.e0, e1, self.dispatches))
 
 
 def _ptr(t):
@@ -308,6 +310,9 @@ class LiePlan:
 # ---------------------------------------------------------------------------
 # grouped GEMM
 # ---------------------------------------------------------------------------
+GEMM_GROUPS_PER_DISPATCH = 32
+
+
 def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
     """groups: sequence of 11-tuples (m, n, k, a_off, a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld)."""
     n = len(groups)
@@ -316,7 +321,12 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
     arr = (_lib.GemmGroup * n)(*[_lib.GemmGroup(*map(int, g)) for g in groups])
     fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
     flop = 2.0 * sum(float(g[0]) * float(g[1]) * float(g[2]) for g in groups)
-    with _timed("gemm_split" if split else "gemm", flop):
+    # the library launches one kernel per 32 groups that hold a non-empty group
+    disp = sum(
+        1 for c in range(0, n, GEMM_GROUPS_PER_DISPATCH)
+        if any(g[0] > 0 and g[1] > 0 for g in groups[c : c + GEMM_GROUPS_PER_DISPATCH])
+    )
+    with _timed("gemm_split" if split else "gemm", flop, disp):
         check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
 
 

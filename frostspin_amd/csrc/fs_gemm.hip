@@ -356,12 +356,22 @@ constexpr int kBK = 16;
 // reaches 42-47 TFLOP/s from 2048 up for every operand layout, 128x128 (one
 // workgroup per CU at 398 registers) 30-39 TFLOP/s and far less on thin or
 // small shapes, so 64x64 is the default; FS_GEMM_TILE=128 forces the other.
-inline int pick_tile(int, const fs_gemm_group *) {
+// 64 x 64 tiles by default; 32 x 32 when a launch would not cover the chip's
+// 256 CUs with 64 x 64 tiles (narrow or small products: range-finder panels,
+// Gram partials), which quadruples the workgroups at the same total traffic
+// through L2. FS_GEMM_TILE=32/64/128 forces a size (tuning only).
+constexpr int kCUs = 256;
+
+inline int pick_tile(int ng, const fs_gemm_group *g) {
   static const int forced = [] {
     const char *e = std::getenv("FS_GEMM_TILE");
     return e ? std::atoi(e) : 0;
   }();
-  return forced == 128 ? 128 : 64;
+  if (forced == 32 || forced == 64 || forced == 128) return forced;
+  int64_t tiles64 = 0;
+  for (int i = 0; i < ng; ++i)
+    if (g[i].m > 0 && g[i].n > 0) tiles64 += fs::ceil_div(g[i].m, 64) * fs::ceil_div(g[i].n, 64);
+  return tiles64 < kCUs ? 32 : 64;
 }
 
 template <bool SPLIT>
@@ -417,6 +427,9 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
                          fs::as_stream(stream), bt);
     else if (tsz == 128)
       hipLaunchKernelGGL((gemm_f64_kernel<128, 128, kBK>), grid, dim3(kThreads), 0,
+                         fs::as_stream(stream), bt);
+    else if (tsz == 32)
+      hipLaunchKernelGGL((gemm_f64_kernel<32, 32, kBK>), grid, dim3(kThreads), 0,
                          fs::as_stream(stream), bt);
     else
       hipLaunchKernelGGL((gemm_f64_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,

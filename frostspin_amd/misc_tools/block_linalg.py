@@ -168,14 +168,18 @@ def gram(Y):
     return Mats(buf, off, Y.cols.copy(), Y.cols.copy(), Y.cols.copy(), np.ones_like(Y.cols)), nparts
 
 
-def cholqr(Y, *, shifted=True, passes=3, want_r=False):
+def cholqr(Y, *, shifted=True, passes=3, want_r=False, statuses=None):
     """Orthonormal basis of the column space of each Y_i (m x l, m >= l) by
     (shifted) CholeskyQR: per pass G = Y^T Y (grouped GEMM), R = chol(G + s I)
     and R^-1 (one fused LDS kernel), Y <- Y R^-1 (grouped GEMM). Shifted
     CholeskyQR3 (Fukaya et al., SIAM J. Sci. Comput. 42 (2020) A477) stays
     defined up to cond(Y) ~ 1/eps. Matrices whose Cholesky breaks down anyway
     are finished by Householder QR. With want_r, also returns the triangular
-    factor R (l x l, Y = Q R) as the product of the passes' factors."""
+    factor R (l x l, Y = Q R) as the product of the passes' factors.
+
+    statuses: if a list, the per-pass device status vectors are appended to it
+    instead of being read back (no host synchronisation); matrices whose
+    factorisation broke down are then left to the caller to redo."""
     eps = np.finfo(np.float64).eps
     cur = Y
     Rtot = None
@@ -213,6 +217,10 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False):
             )
         if want_r:
             Rtot = R if Rtot is None else gemm(R, Rtot)
+        if statuses is not None:
+            statuses.append(status)
+            cur = out
+            continue
         bad = np.nonzero(_ops.download(status))[0]
         if bad.size:
             # breakdown (numerically rank-deficient beyond the shift): Householder
@@ -231,35 +239,69 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False):
     return (cur, Rtot) if want_r else cur
 
 
-def _orthonormal_columns(Y, passes=3):
-    """Orthonormal basis of the column space of each Y_i."""
-    return cholqr(Y, passes=passes)
+def _orthonormal_columns(Y, passes=3, statuses=None, robust=False):
+    """Orthonormal basis of the column space of each Y_i (Householder if robust)."""
+    if robust:
+        return qr(Y, want_r=False)[0]
+    return cholqr(Y, passes=passes, statuses=statuses)
 
 
-def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0):
+def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
     """Randomized truncated SVD (range finder + power iterations, Halko et al.,
     SIAM Rev. 53 (2011) Alg. 4.4/5.1) of every A_i, keeping l_i = k_i + p_i
-    vectors. Returns U (m x l), S, s_off, s_list, VT (l x n) like svd()."""
+    vectors. Returns U (m x l), S, s_off, s_list, VT (l x n) like svd().
+
+    Orthonormalisations use shifted CholeskyQR without reading its status back
+    (no host synchronisation inside the chain); the statuses are read once with
+    the singular values, and any matrix whose factorisation broke down is redone
+    with Householder QR (robust=True) and spliced into the results."""
     ks = np.asarray(ks, dtype=np.int64)
     mins = np.minimum(A.rows, A.cols)
     p = np.maximum(10, ks // 2) if oversampling is None else np.full_like(ks, oversampling)
     ls = np.minimum(ks + p, mins)
     omega = Mats.packed(A.cols, ls)
     _ops.fill_normal_(omega.buf, seed)
+    statuses = None if robust else []
     Y = gemm(A, omega)
     # intermediate bases only need to be well conditioned (Halko et al. sec. 4.5):
     # two CholeskyQR passes; the final basis gets the full three
     for _ in range(power_iters):
-        Q = _orthonormal_columns(Y, passes=2)
+        Q = _orthonormal_columns(Y, passes=2, statuses=statuses, robust=robust)
         Z = gemm(A.T(), Q)
-        Qz = _orthonormal_columns(Z, passes=2)
+        Qz = _orthonormal_columns(Z, passes=2, statuses=statuses, robust=robust)
         Y = gemm(A, Qz)
-    Q = _orthonormal_columns(Y)
+    Q = _orthonormal_columns(Y, statuses=statuses, robust=robust)
     B = gemm(Q.T(), A)  # l x n
-    # B^T = Q2 R2 (shifted CholeskyQR3), then the small l x l SVD of
-    # R2^T = Ur S Vr^T: B = Ur S (Q2 Vr)^T
-    Q2, R2 = cholqr(B.T(), want_r=True)
+    # B^T = Q2 R2, then the small l x l SVD of R2^T = Ur S Vr^T: B = Ur S (Q2 Vr)^T
+    if robust:
+        Q2, R2 = qr(B.T())
+    else:
+        Q2, R2 = cholqr(B.T(), want_r=True, statuses=statuses)
     Ur, S, s_off, s_list, Vrt = svd(R2.T())
     U = gemm(Q, Ur)  # m x l
     VT = gemm(Vrt, Q2.T())  # l x n
+    if statuses:
+        failed = _ops.download(torch.stack(statuses)).any(axis=0).nonzero()[0]
+        if failed.size:
+            _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed)
     return U, S, s_off, s_list, VT
+
+
+def _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed):
+    """Redo the failed matrices with Householder orthonormalisation and copy the
+    results over theirs (same shapes)."""
+    U2, S2, s_off2, s_list2, VT2 = rsvd(
+        A.select(failed), ks[failed], oversampling=oversampling, power_iters=power_iters,
+        seed=seed, robust=True,
+    )
+    for src, dst in ((U2, U), (VT2, VT)):
+        descs = [
+            (src.rows[j], src.cols[j], src.off[j], src.rs[j], src.cs[j], dst.off[i], dst.cols[i], 1,
+             0, 0, 0, 1.0, 0.0)
+            for j, i in enumerate(failed)
+        ]
+        _ops.copy2d(descs, src.buf, dst.buf)
+    descs = [(1, len(s_list2[j]), s_off2[j], 1, 1, s_off[i], 1, 1, 0, 0, 0, 1.0, 0.0) for j, i in enumerate(failed)]
+    _ops.copy2d(descs, S2, S)
+    for j, i in enumerate(failed):
+        s_list[i] = s_list2[j]

@@ -108,3 +108,29 @@ def test_cholqr_q_and_r(backend):
         assert np.abs(q.T @ q - np.eye(q.shape[1])).max() < 1e-12
         assert np.abs(np.tril(r, -1)).max() < 1e-12 * np.abs(r).max()
         assert np.abs(q @ r - a).max() < 1e-12 * np.abs(a).max()
+
+
+def test_rsvd_recovers_from_cholesky_breakdown(backend, monkeypatch):
+    """An all-zero block breaks CholeskyQR down (zero pivot even with the shift);
+    the deferred status check redoes it with Householder QR and splices the
+    result in. The rank-deficient and healthy blocks next to it keep their
+    CholeskyQR results."""
+    rng = np.random.default_rng(6)
+    low = rng.standard_normal((200, 5)) @ rng.standard_normal((5, 150))
+    good, spec = _decaying(rng, 180, 160, 0.7)
+    zero = np.zeros((120, 90))
+    spliced = []
+    orig = bl._splice_rsvd
+    monkeypatch.setattr(bl, "_splice_rsvd", lambda A, ks, failed, *a: (spliced.append(list(failed)), orig(A, ks, failed, *a)))
+    U, S, s_off, s_list, VT = bl.rsvd(_pack([low, good, zero]), [20, 20, 20], seed=2)
+    assert spliced == [[2]]
+    ref_low = np.linalg.svd(low, compute_uv=False)
+    assert np.abs(s_list[0][:5] - ref_low[:5]).max() < 1e-12 * ref_low[0]
+    assert np.abs(s_list[1][:20] - spec[:20]).max() < 1e-10
+    assert np.abs(s_list[2]).max() == 0.0
+    us, vts = _unpack(U), _unpack(VT)
+    assert np.isfinite(us[2]).all() and np.isfinite(vts[2]).all()
+    for i, a in enumerate([low, good]):
+        rec = (us[i][:, :5] * s_list[i][:5]) @ vts[i][:5]
+        ref = np.linalg.svd(a)
+        assert np.abs(rec - (ref[0][:, :5] * ref[1][:5]) @ ref[2][:5]).max() < 1e-10 * ref[1][0]

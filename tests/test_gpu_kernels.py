@@ -14,9 +14,14 @@ def _dev(a):
     return _ops.upload(np.asarray(a, dtype=np.float64))
 
 
-def test_mfma_gemm_layouts_and_uneven_groups(gpu):
+GEMM_SIZES_SMALL = [(1, 1, 1), (3, 70, 5), (64, 64, 16), (65, 63, 17), (130, 7, 200), (17, 129, 33)]
+# one group with >= 256 64x64 tiles switches the launch to the 64x64 kernel
+GEMM_SIZES_LARGE = [(1100, 1050, 70), (33, 47, 65), (70, 1, 130)]
+
+
+@pytest.mark.parametrize("sizes", [GEMM_SIZES_SMALL, GEMM_SIZES_LARGE], ids=["tile32", "tile64"])
+def test_mfma_gemm_layouts_and_uneven_groups(gpu, sizes):
     rng = np.random.default_rng(0)
-    sizes = [(1, 1, 1), (3, 70, 5), (64, 64, 16), (65, 63, 17), (130, 7, 200), (17, 129, 33)]
     for a_f in (False, True):
         for b_f in (False, True):
             A_parts, B_parts, groups, refs = [], [], [], []
