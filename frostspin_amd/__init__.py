@@ -9,6 +9,7 @@ torch provides device buffers and streams only.
 from .symmetric_tensor import (
     AsymmetricTensor,
     DiagonalTensor,
+    O2SymmetricTensor,
     SU2SymmetricTensor,
     U1SymmetricTensor,
     get_gemm_precision,
@@ -23,6 +24,7 @@ __all__ = [
     "ASSERT_TOL",
     "AsymmetricTensor",
     "DiagonalTensor",
+    "O2SymmetricTensor",
     "SU2SymmetricTensor",
     "U1SymmetricTensor",
     "__version__",

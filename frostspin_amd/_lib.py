@@ -96,6 +96,14 @@ class TrsmDesc(ctypes.Structure):
     ]
 
 
+class Gather2dDesc(ctypes.Structure):
+    _fields_ = [
+        (n, c_i64)
+        for n in ("rows", "cols", "src_rows", "src_cols", "s_off", "s_rs", "s_cs", "d_off", "d_rs", "d_cs",
+                  "ri_off", "rw_off", "ci_off", "cw_off")
+    ] + [("beta", c_dbl)]
+
+
 class Copy2dDesc(ctypes.Structure):
     _fields_ = [
         ("rows", c_i64),
@@ -197,6 +205,12 @@ _SIGNATURES = {
     "fs_batched_copy2d_f64": (
         c_int,
         [c_int, ctypes.POINTER(Copy2dDesc), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    ),
+    "fs_gather2d_check": (
+        c_int, [c_int, ctypes.POINTER(Gather2dDesc), c_i64p, c_i64, c_i64, c_i64, c_i64],
+    ),
+    "fs_batched_gather2d_f64": (
+        c_int, [c_int, ctypes.POINTER(Gather2dDesc), c_vp, c_vp, c_vp, c_vp, c_vp],
     ),
     "fs_weighted_trace_f64": (
         c_int,

@@ -456,6 +456,25 @@ def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
         )
 
 
+def gather2d(descs, idx, w, src, dst):
+    """Two-term weighted gather (fs_batched_gather2d_f64). descs: sequences of
+    (rows, cols, src_rows, src_cols, s_off, s_rs, s_cs, d_off, d_rs, d_cs, ri_off,
+    rw_off, ci_off, cw_off, beta); idx (int64) and w (float64) are host arrays,
+    validated against the buffer sizes before upload."""
+    n = len(descs)
+    if n == 0:
+        return
+    arr = (_lib.Gather2dDesc * n)(*[_lib.Gather2dDesc(*(int(x) for x in d[:14]), float(d[14])) for d in descs])
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    check(lib.fs_gather2d_check(n, arr, _lib.i64p(idx), idx.size, w.size, src.numel(), dst.numel()))
+    idx_dev = upload(idx, dtype=torch.int64)
+    w_dev = upload(w)
+    moved = sum(float(d[0]) * float(d[1]) for d in descs)
+    with _timed("gather2d", 16.0 * moved):
+        check(lib.fs_batched_gather2d_f64(n, arr, _ptr(idx_dev), _ptr(w_dev), _ptr(src), _ptr(dst), _stream()))
+
+
 def weighted_trace(x, offs, ns, lds, weights):
     """Device scalar sum_b w_b trace(block_b)."""
     offs = np.ascontiguousarray(offs, dtype=np.int64)

@@ -197,6 +197,153 @@ inline unsigned grid_for(int64_t n) {
 
 } // namespace
 
+namespace {
+
+// Two-term weighted gather, the data movement of O(2) tensors: every output
+// element combines at most 2 source rows and 2 source columns,
+//   out[i][j] = beta*out[i][j] + sum_{a,b<2} rw[i][a] cw[j][b] src[ri[i][a]][ci[j][b]],
+// which covers the Sz = 0 block split into 0odd / 0even (fixed points with
+// weight 1, doublets with +-1/sqrt(2); o2_symmetric_tensor.py:118-310), its
+// inverse, and the reflected Sz < 0 blocks (one term, sign weight;
+// :325-352, :798-851). Terms with zero weight are skipped.
+constexpr int kMaxGather = 32;
+
+struct GatherDev {
+  int64_t rows, cols;
+  int64_t s_off, s_rs, s_cs;
+  int64_t d_off, d_rs, d_cs;
+  int64_t ri_off, rw_off, ci_off, cw_off;
+  double beta;
+};
+
+struct GatherBatch {
+  const double *src;
+  double *dst;
+  const int64_t *idx;
+  const double *w;
+  int32_t n, pad;
+  int64_t tile_prefix[kMaxGather + 1];
+  GatherDev d[kMaxGather];
+};
+
+__global__ void __launch_bounds__(kThreads) gather2d_kernel(GatherBatch b) {
+  const int64_t tile = blockIdx.x;
+  int g = 0;
+  while (g + 1 < b.n && b.tile_prefix[g + 1] <= tile) ++g;
+  const GatherDev D = b.d[g];
+  const int64_t base = (tile - b.tile_prefix[g]) * kCopyTile;
+  const int64_t n = D.rows * D.cols;
+  for (int k = 0; k < kCopyTile / kThreads; ++k) {
+    const int64_t e = base + k * kThreads + threadIdx.x;
+    if (e >= n) break;
+    const int64_t i = e / D.cols;
+    const int64_t j = e - i * D.cols;
+    double v = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const double wr = b.w[D.rw_off + 2 * i + a];
+      if (wr == 0.0) continue;
+      const int64_t r = b.idx[D.ri_off + 2 * i + a];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const double wc = b.w[D.cw_off + 2 * j + c];
+        if (wc == 0.0) continue;
+        v = fma(wr * wc, b.src[D.s_off + r * D.s_rs + b.idx[D.ci_off + 2 * j + c] * D.s_cs], v);
+      }
+    }
+    double *d = b.dst + D.d_off + i * D.d_rs + j * D.d_cs;
+    *d = (D.beta == 0.0) ? v : v + D.beta * (*d);
+  }
+}
+
+int check_gather(int ndesc, const fs_gather2d_desc *descs, const int64_t *idx, int64_t nidx,
+                 int64_t nw, int64_t src_size, int64_t dst_size) {
+  FS_CHECK_ARG(ndesc >= 0 && nidx >= 0 && nw >= 0, "gather2d: negative size");
+  FS_CHECK_ARG(ndesc == 0 || descs, "gather2d: null descriptors");
+  for (int t = 0; t < ndesc; ++t) {
+    const fs_gather2d_desc &q = descs[t];
+    FS_CHECK_ARG(q.rows >= 0 && q.cols >= 0 && q.src_rows >= 0 && q.src_cols >= 0,
+                 "gather2d: negative extent");
+    if (q.rows == 0 || q.cols == 0) continue;
+    FS_CHECK_ARG(idx != nullptr, "gather2d: null index table");
+    FS_CHECK_ARG(q.src_rows > 0 && q.src_cols > 0, "gather2d: empty source for a non-empty output");
+    FS_CHECK_ARG(q.s_off >= 0 && q.s_rs >= 0 && q.s_cs >= 0 && q.d_off >= 0 && q.d_rs >= 0 && q.d_cs >= 0,
+                 "gather2d: negative offset or stride");
+    FS_CHECK_ARG(q.s_off + (q.src_rows - 1) * q.s_rs + (q.src_cols - 1) * q.s_cs < src_size,
+                 "gather2d: source out of bounds");
+    FS_CHECK_ARG(q.d_off + (q.rows - 1) * q.d_rs + (q.cols - 1) * q.d_cs < dst_size,
+                 "gather2d: destination out of bounds");
+    FS_CHECK_ARG(q.ri_off >= 0 && q.ri_off + 2 * q.rows <= nidx && q.ci_off >= 0 &&
+                     q.ci_off + 2 * q.cols <= nidx,
+                 "gather2d: index table out of range");
+    FS_CHECK_ARG(q.rw_off >= 0 && q.rw_off + 2 * q.rows <= nw && q.cw_off >= 0 &&
+                     q.cw_off + 2 * q.cols <= nw,
+                 "gather2d: weight table out of range");
+    for (int64_t i = 0; i < 2 * q.rows; ++i)
+      FS_CHECK_ARG(idx[q.ri_off + i] >= 0 && idx[q.ri_off + i] < q.src_rows, "gather2d: row index out of range");
+    for (int64_t j = 0; j < 2 * q.cols; ++j)
+      FS_CHECK_ARG(idx[q.ci_off + j] >= 0 && idx[q.ci_off + j] < q.src_cols, "gather2d: column index out of range");
+  }
+  return FS_OK;
+}
+
+} // namespace
+
+extern "C" int fs_gather2d_check(int ndesc, const fs_gather2d_desc *descs, const int64_t *idx,
+                                 int64_t nidx, int64_t nw, int64_t src_size, int64_t dst_size) {
+  return fs::guarded([&]() -> int { return check_gather(ndesc, descs, idx, nidx, nw, src_size, dst_size); });
+}
+
+extern "C" int fs_batched_gather2d_f64(int ndesc, const fs_gather2d_desc *descs, const int64_t *idx,
+                                       const double *w, const double *src, double *dst,
+                                       void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(ndesc >= 0 && (ndesc == 0 || descs), "gather2d: bad descriptors");
+    int i0 = 0;
+    while (i0 < ndesc) {
+      GatherBatch b;
+      std::memset(&b, 0, sizeof(b));
+      b.src = src;
+      b.dst = dst;
+      b.idx = idx;
+      b.w = w;
+      int nb = 0;
+      int64_t tiles = 0;
+      for (; i0 < ndesc && nb < kMaxGather; ++i0) {
+        const fs_gather2d_desc &q = descs[i0];
+        FS_CHECK_ARG(q.rows >= 0 && q.cols >= 0, "gather2d: negative extent");
+        if (q.rows == 0 || q.cols == 0) continue;
+        FS_CHECK_ARG(src && dst && idx && w, "gather2d: null buffer");
+        GatherDev &d = b.d[nb];
+        d.rows = q.rows;
+        d.cols = q.cols;
+        d.s_off = q.s_off;
+        d.s_rs = q.s_rs;
+        d.s_cs = q.s_cs;
+        d.d_off = q.d_off;
+        d.d_rs = q.d_rs;
+        d.d_cs = q.d_cs;
+        d.ri_off = q.ri_off;
+        d.rw_off = q.rw_off;
+        d.ci_off = q.ci_off;
+        d.cw_off = q.cw_off;
+        d.beta = q.beta;
+        b.tile_prefix[nb] = tiles;
+        tiles += fs::ceil_div(q.rows * q.cols, kCopyTile);
+        ++nb;
+      }
+      if (nb == 0) continue;
+      b.tile_prefix[nb] = tiles;
+      b.n = nb;
+      FS_CHECK_ARG(tiles < (1ll << 31), "gather2d: too many tiles");
+      hipLaunchKernelGGL(gather2d_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kThreads), 0,
+                         fs::as_stream(stream), b);
+      FS_LAUNCH_CHECK("gather2d_kernel");
+    }
+    return FS_OK;
+  });
+}
+
 extern "C" {
 
 int fs_weighted_norm2_f64(int nseg, const int64_t *seg_off, const double *w, const double *x,

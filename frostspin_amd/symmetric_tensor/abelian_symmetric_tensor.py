@@ -8,9 +8,17 @@ driven by planner tables (fs_abelian_permute_plan_create).
 """
 
 import numpy as np
+import torch
 
 from .. import _ops
 from .symmetric_tensor import SymmetricTensor
+
+
+def _dense_source(arr):
+    """Flat float64 device copy of a dense array (device arrays are used as is)."""
+    if isinstance(arr, torch.Tensor):
+        return arr.reshape(-1).to(dtype=_ops.DTYPE).contiguous()
+    return _ops.upload(np.asarray(arr, dtype=np.float64))
 
 
 class ChargeGatherEngine:
@@ -52,7 +60,7 @@ class ChargeGatherEngine:
             nrr,
             old_dense=True,
         )
-        src = _ops.upload(np.asarray(arr, dtype=np.float64))
+        src = _dense_source(arr)
         data = _ops.permute(plan, src)
         return data, plan.shapes, cls._irreps_dtype(plan.irreps)
 

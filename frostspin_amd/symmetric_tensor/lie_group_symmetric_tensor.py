@@ -25,6 +25,7 @@ from functools import reduce
 import numpy as np
 
 from .. import _ops
+from .abelian_symmetric_tensor import _dense_source
 from .non_abelian_symmetric_tensor import NonAbelianSymmetricTensor
 
 # Overlap coefficients below this magnitude are exact zeros polluted by
@@ -508,13 +509,17 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
             reps, signature, nrr, block_irreps, offs, shapes[:, 1].copy(), np.ones_like(offs), to_dense=False
         )
         if tables is not None:
-            src = _ops.upload(np.ascontiguousarray(arr, dtype=np.float64).reshape(-1))
+            src = _dense_source(arr)
             plan = _ops.LiePlan(*tables, src.numel(), out.numel())
             plan.execute(src, out)
         return out, shapes, block_irreps
 
     def _tomatrix(self):
         """Blocks -> dense (lie:969-1109), zero outside the invariant subspace."""
+        return _ops.download(self._tomatrix_device()).reshape(self.matrix_shape)
+
+    def _tomatrix_device(self):
+        """Dense row-major matrix as a flat device buffer."""
         reps = self._row_reps + self._col_reps
         n = int(np.prod(self._shape))
         rs_b, cs_b = self._block_strides()
@@ -525,7 +530,7 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         if tables is not None:
             plan = _ops.LiePlan(*tables, self._data.numel(), n)
             plan.execute(self._data, dense)
-        return _ops.download(dense).reshape(self.matrix_shape)
+        return dense
 
     def update_signature(self, sign_update):
         raise NotImplementedError("signature update of Lie-group tensors (lie:1136-1176)")

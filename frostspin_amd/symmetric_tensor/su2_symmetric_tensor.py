@@ -8,6 +8,7 @@ the spin-reversal map (su2:87-91).
 
 import numpy as np
 
+from .. import _ops
 from ..groups import su2_clebsch_gordan as cg
 from .lie_group_symmetric_tensor import LieGroupSymmetricTensor
 from .tools import check_norm, symmetric_tensor_types
@@ -135,6 +136,73 @@ class SU2SymmetricTensor(LieGroupSymmetricTensor):
         )
         assert check_norm(self, out)
         return out
+
+
+    def toO2(self):
+        """Cast to O(2) (su2:280-318): every leg's basis is reordered from SU(2)
+        multiplets to O(2) sectors (0odd, 0even, then +n / -n blocks with n =
+        2|Sz|) and gets the O(2) reflection signs; one signed row/column
+        permutation of the dense matrix on the device, then O(2) from_array."""
+        from .o2_symmetric_tensor import O2SymmetricTensor
+
+        legs = [_o2_leg_cast(tuple(r[0].tolist()), tuple(r[1].tolist())) for r in self._row_reps + self._col_reps]
+        nrr = self._nrr
+        rows, cols = self.matrix_shape
+
+        def composed(parts):
+            shape = [len(p[0]) for p in parts]
+            grid = np.indices(shape).reshape(len(shape), -1) if shape else np.zeros((0, 1), np.int64)
+            src = np.zeros(grid.shape[1], dtype=np.int64)
+            sign = np.ones(grid.shape[1])
+            for k, (perm, sg, _) in enumerate(parts):
+                src = src * shape[k] + perm[grid[k]]
+                sign *= sg[perm[grid[k]]]
+            return src, sign
+
+        rsrc, rsign = composed(legs[:nrr])
+        csrc, csign = composed(legs[nrr:])
+        idx = np.zeros((2 * rows + 2 * cols,), dtype=np.int64)
+        w = np.zeros((2 * rows + 2 * cols,))
+        idx[0 : 2 * rows : 2] = rsrc
+        w[0 : 2 * rows : 2] = rsign
+        idx[2 * rows :: 2] = csrc
+        w[2 * rows :: 2] = csign
+        dense = self._tomatrix_device()
+        out = _ops.empty(rows * cols)
+        _ops.gather2d(
+            [(rows, cols, rows, cols, 0, cols, 1, 0, cols, 1, 0, 0, 2 * rows, 2 * rows, 0.0)], idx, w, dense, out
+        )
+        o2_reps = [leg[2] for leg in legs]
+        res = O2SymmetricTensor.from_array(out.reshape(self._shape), o2_reps[:nrr], o2_reps[nrr:], self._signature)
+        assert check_norm(self, res)
+        return res
+
+
+def _o2_leg_cast(degen, irreps):
+    """(source index, sign, O(2) representation) of one leg (su2:295-318)."""
+    sz, sz_o2, flip = [], [], []
+    for d, irr in zip(degen, irreps, strict=True):
+        s = np.arange(irr - 1, -irr - 1, -2, dtype=np.int64)
+        so2 = np.abs(s)
+        f = np.zeros(s.shape, dtype=bool)
+        if irr % 2:
+            f[(s % 4 == (irr + 1) % 4) & (s < 0)] = True
+            if irr % 4 == 3:  # odd integer spin: its Sz = 0 state is 0odd
+                so2[irr // 2] = -1
+        else:
+            f[(s % 4 == (irr - 1) % 4) & (s < 0)] = True
+        for _ in range(d):
+            sz.append(s)
+            sz_o2.append(so2)
+            flip.append(f)
+    sz = np.concatenate(sz)
+    sz_o2 = np.concatenate(sz_o2)
+    sign = 1.0 - 2.0 * np.concatenate(flip)
+    swap = np.argsort(-sz, kind="stable")
+    swap = swap[np.argsort(sz_o2[swap], kind="stable")]
+    o2_irreps, o2_degen = np.unique(sz_o2, return_counts=True)
+    o2_degen[o2_irreps > 0] //= 2
+    return swap, sign, np.array([o2_degen, o2_irreps], dtype=np.int64)
 
 
 class _CGView:

@@ -179,8 +179,14 @@ class SymmetricTensor:
         row_reps = tuple(row_reps)
         col_reps = tuple(col_reps)
         in_reps = row_reps + col_reps
-        arr = arr.detach().cpu().numpy() if isinstance(arr, torch.Tensor) else np.asarray(arr)
-        if arr.shape != tuple(cls.representation_dimension(rep) for rep in in_reps):
+        if isinstance(arr, torch.Tensor):
+            # device arrays stay on the device; host torch tensors become numpy
+            arr = arr.detach()
+            if arr.device != _ops.device():
+                arr = arr.cpu().numpy()
+        else:
+            arr = np.asarray(arr)
+        if tuple(arr.shape) != tuple(cls.representation_dimension(rep) for rep in in_reps):
             raise ValueError("Representations do not match array shape")
         signature = validated_signature(signature, len(row_reps), len(col_reps))
         data, shapes, block_irreps = cls._blocks_from_dense(arr, row_reps, col_reps, signature)

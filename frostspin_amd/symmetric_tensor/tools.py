@@ -22,12 +22,21 @@ def get_symmetric_tensor_type(symmetry):
     return st_type
 
 
+def _norm_of(t):
+    import torch
+
+    if isinstance(t, torch.Tensor):
+        return float(t.double().norm())
+    if hasattr(t, "norm"):
+        return float(t.norm())
+    return float(np.linalg.norm(t))
+
+
 def check_norm(t1, t2, *, tol=ASSERT_TOL):
     """Debug check (runs under assert): both norms agree to relative tol."""
     if not config["debug_checks"]:
         return True
-    n1 = t1.norm() if hasattr(t1, "norm") else float(np.linalg.norm(t1))
-    n2 = t2.norm() if hasattr(t2, "norm") else float(np.linalg.norm(t2))
+    n1, n2 = (_norm_of(t) for t in (t1, t2))
     dn = abs(n1 - n2)
     b = dn > tol * max(n1, n2)
     if b:

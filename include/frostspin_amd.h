@@ -403,6 +403,32 @@ int fs_batched_copy2d_f64(int ndesc, const fs_copy2d_desc *descs,
                           void *stream);
 
 /*
+ * Two-term weighted gather (O(2) data movement): for each descriptor,
+ *   out[i][j] = beta*out[i][j] + sum_{a,b<2} w[rw_off+2i+a] w[cw_off+2j+b]
+ *               * src[s_off + idx[ri_off+2i+a]*s_rs + idx[ci_off+2j+b]*s_cs]
+ * with out[i][j] at dst + d_off + i*d_rs + j*d_cs; source row/column indices
+ * lie in [0, src_rows) x [0, src_cols); zero-weight terms are not read.
+ * idx and w are device arrays; fs_gather2d_check validates the same
+ * descriptors and (host copies of the) index table against the buffer sizes.
+ * Replaces split_b0 / _numba_merge_b0oe / _numba_generate_refl_block
+ * (o2_symmetric_tensor.py:268-352) and the reflected writes of
+ * _numba_O2_transpose (:357-581).
+ */
+typedef struct fs_gather2d_desc {
+  int64_t rows, cols, src_rows, src_cols;
+  int64_t s_off, s_rs, s_cs;
+  int64_t d_off, d_rs, d_cs;
+  int64_t ri_off, rw_off, ci_off, cw_off;
+  double beta;
+} fs_gather2d_desc;
+
+int fs_gather2d_check(int ndesc, const fs_gather2d_desc *descs, const int64_t *idx, int64_t nidx,
+                      int64_t nw, int64_t src_size, int64_t dst_size);
+
+int fs_batched_gather2d_f64(int ndesc, const fs_gather2d_desc *descs, const int64_t *idx,
+                            const double *w, const double *src, double *dst, void *stream);
+
+/*
  * Weighted sum of the diagonals: out[0] = sum_b w[b] * trace(block_b) where
  * block b is n_b x n_b at off_b with row stride ld_b (trace, :633-637).
  */

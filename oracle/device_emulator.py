@@ -188,6 +188,31 @@ def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
         d[di] = v if beta == 0.0 else v + beta * d[di]
 
 
+def gather2d(descs, idx, w, src, dst):
+    from frostspin_amd import _lib
+
+    idx = np.ascontiguousarray(idx, dtype=np.int64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    arr = (_lib.Gather2dDesc * max(len(descs), 1))(
+        *[_lib.Gather2dDesc(*(int(x) for x in d[:14]), float(d[14])) for d in descs])
+    _lib.check(_lib.lib.fs_gather2d_check(len(descs), arr, _lib.i64p(idx), idx.size, w.size,
+                                          src.numel(), dst.numel()))
+    s, d = _np(src), _np(dst)
+    for rows, cols, _sr, _sc, so, srs, scs, do, drs, dcs, rio, rwo, cio, cwo, beta in descs:
+        if rows == 0 or cols == 0:
+            continue
+        ri = idx[rio : rio + 2 * rows].reshape(rows, 2)
+        rw = w[rwo : rwo + 2 * rows].reshape(rows, 2)
+        ci = idx[cio : cio + 2 * cols].reshape(cols, 2)
+        cw = w[cwo : cwo + 2 * cols].reshape(cols, 2)
+        v = np.zeros((rows, cols))
+        for a in range(2):
+            for b in range(2):
+                v += rw[:, a, None] * cw[None, :, b] * s[so + ri[:, a, None] * srs + ci[None, :, b] * scs]
+        di = do + np.arange(rows)[:, None] * drs + np.arange(cols)[None, :] * dcs
+        d[di] = v if beta == 0.0 else v + beta * d[di]
+
+
 def weighted_trace(x, offs, ns, lds, weights):
     xs = _np(x)
     tot = 0.0
@@ -292,6 +317,6 @@ def install(monkeypatch):
         "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_cholinv",
         "batched_trsm_right_upper",
         "weighted_norm2", "scale_", "scale_rsqrt_dev_",
-        "axpby_", "copy2d", "weighted_trace", "fill_normal_",
+        "axpby_", "copy2d", "gather2d", "weighted_trace", "fill_normal_",
     ):
         monkeypatch.setattr(_ops, name, globals()[name])
