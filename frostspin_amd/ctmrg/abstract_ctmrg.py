@@ -8,7 +8,7 @@ grouped MFMA GEMM, every projector a batched device SVD.
 
 import numpy as np
 
-from . import rdm
+from . import observables, rdm
 from .ctm_contract import (
     contract_C1234,
     contract_dl,
@@ -238,6 +238,24 @@ class AbstractCTMRG:
             e.get_T2(x + 2, y + 2), e.get_C4(x, y + 3), e.get_T3(x + 1, y + 3), e.get_C3(x + 2, y + 3),
         )
 
+    def compute_rdm_diag_dr(self, x, y, *, free_memory=False):
+        """Sites (x+1, y+1) and (x+2, y+2) (abstract_ctmrg.py:438-455)."""
+        e = self._env
+        return rdm.rdm_diag_dr(
+            e.get_C1(x, y), e.get_T1(x + 1, y), self.construct_enlarged_ur(x, y, free_memory=free_memory),
+            e.get_T4(x, y + 1), e.get_A(x + 1, y + 1), self.construct_enlarged_dl(x, y, free_memory=free_memory),
+            e.get_A(x + 2, y + 2), e.get_T2(x + 3, y + 2), e.get_T3(x + 2, y + 3), e.get_C3(x + 3, y + 3),
+        )
+
+    def compute_rdm_diag_ur(self, x, y, *, free_memory=False):
+        """Sites (x+1, y+2) and (x+2, y+1) (abstract_ctmrg.py:457-474)."""
+        e = self._env
+        return rdm.rdm_diag_ur(
+            self.construct_enlarged_ul(x, y, free_memory=free_memory), e.get_T1(x + 2, y), e.get_C2(x + 3, y),
+            e.get_A(x + 2, y + 1), e.get_T2(x + 3, y + 1), e.get_T4(x, y + 2), e.get_A(x + 1, y + 2),
+            self.construct_enlarged_dr(x, y, free_memory=free_memory), e.get_C4(x, y + 3), e.get_T3(x + 1, y + 3),
+        )
+
     def compute_rdm1x1_cell(self):
         return [self.compute_rdm1x1(x, y) for x, y in self._site_coords]
 
@@ -248,6 +266,61 @@ class AbstractCTMRG:
             rdm1x2_cell.append(self.compute_rdm1x2(x, y))
             rdm2x1_cell.append(self.compute_rdm2x1(x, y))
         return rdm1x2_cell, rdm2x1_cell
+
+    def compute_rdm_2nd_neighbor_cell(self, *, free_memory=True):
+        """Both diagonal rdms for every cell site (abstract_ctmrg.py:500-517).
+        With free_memory the cached enlarged corners are dropped first and the
+        ones built here are not kept."""
+        if free_memory:
+            self._env.reset_constructed_corners()
+        rdm_dr_cell = []
+        rdm_ur_cell = []
+        for x, y in self._site_coords:
+            rdm_dr_cell.append(self.compute_rdm_diag_dr(x, y, free_memory=free_memory))
+            rdm_ur_cell.append(self.compute_rdm_diag_ur(x, y, free_memory=free_memory))
+        return rdm_dr_cell, rdm_ur_cell
+
+    def compute_transfer_spectrum_h(self, y, nvals, *, dmax_full=200, maxiter=1000, arpack_tol=0):
+        """nvals leading eigenvalues of the horizontal transfer matrix at cell row y
+        (abstract_ctmrg.py:519-539)."""
+        e = self._env
+        T1s = [e.get_T1(-x, y) for x in range(self.Lx)]
+        T3s = [e.get_T3(-x, y + 1) for x in range(self.Lx)]
+        return observables.compute_mps_transfer_spectrum(
+            T1s, T3s, nvals, dmax_full=dmax_full, maxiter=maxiter, arpack_tol=arpack_tol, verbosity=self.verbosity
+        )
+
+    def compute_transfer_spectrum_v(self, x, nvals, *, dmax_full=200, maxiter=1000, arpack_tol=0):
+        """nvals leading eigenvalues of the vertical transfer matrix at cell column x
+        (abstract_ctmrg.py:541-561)."""
+        e = self._env
+        T1s = [e.get_T2(x + 1, -y).permute((1,), (2, 3, 0)) for y in range(self.Ly)]
+        T3s = [e.get_T4(x, -y).permute((1, 2), (3, 0)) for y in range(self.Ly)]
+        return observables.compute_mps_transfer_spectrum(
+            T1s, T3s, nvals, dmax_full=dmax_full, maxiter=maxiter, arpack_tol=arpack_tol, verbosity=self.verbosity
+        )
+
+    def compute_corr_length_h(self, y, *, maxiter=1000, arpack_tol=0):
+        """xi = -Lx / ln|lambda_2 / lambda_1| (abstract_ctmrg.py:563-570)."""
+        _, v2 = self.compute_transfer_spectrum_h(y, 2, maxiter=maxiter, arpack_tol=arpack_tol)
+        return -self.Lx / np.log(np.abs(v2))
+
+    def compute_corr_length_v(self, x, *, maxiter=1000, arpack_tol=0):
+        """xi = -Ly / ln|lambda_2 / lambda_1| (abstract_ctmrg.py:572-579)."""
+        _, v2 = self.compute_transfer_spectrum_v(x, 2, maxiter=maxiter, arpack_tol=arpack_tol)
+        return -self.Ly / np.log(np.abs(v2))
+
+    def print_tensor_shapes(self):
+        """(abstract_ctmrg.py:321-335)"""
+        e = self._env
+        print("tensor shapes for C1 T1 C2 // T4 A T2 // C4 T3 C4:")
+        for x, y in self._site_coords:
+            print(
+                f"({x},{y}):",
+                e.get_C1(x, y).shape, e.get_T1(x + 1, y).shape, e.get_C2(x + 2, y).shape,
+                e.get_T4(x, y + 1).shape, e.get_A(x + 1, y + 1).shape, e.get_T2(x + 2, y + 1).shape,
+                e.get_C4(x, y + 3).shape, e.get_T3(x + 1, y + 2).shape, e.get_C3(x + 2, y + 2).shape,
+            )
 
     # enlarged corners (cached in the environment) -------------------------
     def _enlarged(self, getter, setter, build, x, y, free_memory):

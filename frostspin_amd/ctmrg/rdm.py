@@ -45,10 +45,7 @@ def contract_open_corner_mirror(T1, C2, A, T2):
 
 def rdm_1x2(C1, T1l, T1r, C2, T4, Al, Ar, T2, C4, T3l, T3r, C3):
     """Two horizontally adjacent sites (rdm.py:103-136)."""
-    left = T4.permute((0, 1, 2), (3,)) @ C4
-    left = contract_open_corner(C1, T1l, left, Al)
-    left = left @ T3l.permute((0, 1, 3), (2,))
-    left = left.permute((0, 1), (2, 3, 4, 5))
+    left = _half_1x2(C1, T1l, T4, Al, C4, T3l)
     right = T2.permute((0, 2, 3), (1,)) @ C3
     right = right.permute((0, 3), (1, 2))
     right = contract_open_corner_mirror(T1r, C2, Ar, right)
@@ -76,3 +73,71 @@ def rdm_2x1(C1, T1, C2, T4u, Au, T2u, T4d, Ad, T2d, C4, T3, C3):
         T4d.permute((1, 2, 3), (0,)),
         C4.transpose(),
     )
+
+
+def _half_1x2(C1, T1l, T4, Al, C4, T3l):
+    """Left half of a horizontal pair with both physical legs open: legs
+    (bra, ket) x (T1 chi, C? chi, ...) ready to meet the right half."""
+    left = contract_open_corner(C1, T1l, T4.permute((0, 1, 2), (3,)) @ C4, Al)
+    left = left @ T3l.permute((0, 1, 3), (2,))
+    return left.permute((0, 1), (2, 3, 4, 5))
+
+
+def rdm_symmetrized_1x2(C1, T1l, T4, Al, C4, T3l):
+    """Horizontal pair in an environment symmetric under the reflection that
+    maps the left site onto the right one (rdm.py:139-153): the right half is
+    the mirrored left half."""
+    left = _half_1x2(C1, T1l, T4, Al, C4, T3l)
+    rdm = (left @ left.permute((3, 2, 5, 4), (1, 0))).permute((1, 3), (0, 2))
+    rdm /= rdm.trace()
+    return rdm
+
+
+def rdm_symmetrized_2x1(C1, T1, C2, T4u, Au, T2u):
+    """Vertical pair through rdm_symmetrized_1x2 on rotated legs (rdm.py:156-168)."""
+    return rdm_symmetrized_1x2(
+        C2,
+        T2u.permute((1, 2, 3), (0,)),
+        T1,
+        Au.permute((0, 1), (3, 4, 5, 2)),
+        C1,
+        T4u.permute((1, 2, 3), (0,)),
+    )
+
+
+def rdm_diag_dr(C1, T1l, ur, T4u, Aul, dl, Adr, T2d, T3r, C3):
+    """Next-nearest neighbours on the down-right diagonal (rdm.py:192-245):
+    open upper-left corner, closed enlarged ur and dl corners, open down-right
+    corner built as the mirror of the upper-right one."""
+    rdm = contract_open_corner(C1, T1l, T4u, Aul) @ dl
+    rdm = ur @ rdm.permute((2, 3, 4), (0, 1, 5, 6, 7))
+    rdm = rdm.permute((3, 4), (0, 1, 2, 5, 6, 7))
+    dr = contract_open_corner_mirror(
+        T2d.permute((1, 2, 3), (0,)),
+        C3.transpose(),
+        Adr.permute((0, 1, 3, 4), (5, 2)),
+        T3r.permute((2, 3), (0, 1)),
+    )
+    rdm = rdm @ dr.permute((5, 2, 6, 4, 1, 7), (3, 0))
+    rdm = rdm.permute((1, 3), (0, 2))
+    rdm /= rdm.trace()
+    return rdm
+
+
+def rdm_diag_ur(ul, T1r, C2, Aur, T2u, T4d, Adl, dr_T, C4, T3l):
+    """Next-nearest neighbours on the up-right diagonal through rdm_diag_dr on
+    the lattice rotated by a quarter turn (rdm.py:248-272); dr_T is the
+    enlarged down-right corner in its transposed storage order."""
+    rdm = rdm_diag_dr(
+        C4,
+        T4d,
+        ul,
+        T3l.permute((3,), (0, 1, 2)),
+        Adl.permute((0, 1), (5, 2, 3, 4)),
+        dr_T,
+        Aur.permute((0, 1), (5, 2, 3, 4)),
+        T1r.permute((3,), (0, 1, 2)),
+        T2u.permute((2, 3, 0), (1,)),
+        C2.transpose(),
+    )
+    return rdm.permute((1, 0), (3, 2))

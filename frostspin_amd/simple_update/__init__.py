@@ -1,3 +1,4 @@
 from .simple_update import SimpleUpdate
+from .symmetrized_simple_update import SymmetrizedSimpleUpdate
 
-__all__ = ["SimpleUpdate"]
+__all__ = ["SimpleUpdate", "SymmetrizedSimpleUpdate"]

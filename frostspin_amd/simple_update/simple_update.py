@@ -11,6 +11,9 @@ match it update by update; every tensor operation runs on the device.
 import numpy as np
 
 from ..symmetric_tensor.diagonal_tensor import DiagonalTensor
+from ..symmetric_tensor.tools import get_symmetric_tensor_type
+from .abstract_simple_update import AbstractSimpleUpdate
+from .simple_update_tools import check_hamiltonians, check_tensor_bond_indices
 
 
 def _leg_swap(init, end):
@@ -71,16 +74,19 @@ def update_schedule(tensor_bond_indices, raw_update_data, raw_hamilts, *, second
     return bonds, gate_idx, left_idx, right_idx, lperm, rperm, hamiltonians, initial_swap, final_swap
 
 
-class SimpleUpdate:
+class SimpleUpdate(AbstractSimpleUpdate):
     """Simple update of a thermal ensemble (purification with an ancilla leg)."""
 
     _classname = "SimpleUpdate"
-    _is_second_order = True
 
     def __init__(self, D, beta, tau, rcutoff, degen_ratio, tensor_bond_indices, tensors,
                  raw_update_data, raw_hamilts, weights, logZ, *, verbosity=0):
         self.verbosity = int(verbosity)
         self._tensor_bond_indices = [np.array(t, dtype=int) for t in tensor_bond_indices]
+        check_tensor_bond_indices(self._tensor_bond_indices)
+        check_hamiltonians(raw_hamilts)
+        if len(tensors) != len(self._tensor_bond_indices):
+            raise ValueError("Invalid tensor number")
         (
             self._bonds, self._gate_indices, self._left_indices, self._right_indices,
             self._lperm, self._rperm, self._hamiltonians, self._initial_swap, self._final_swap,
@@ -150,91 +156,29 @@ class SimpleUpdate:
         return cls(D, 0.0, tau, rcutoff, degen_ratio, tensor_bond_indices, tensors, raw_update_data,
                    raw_hamilts, weights, 0.0, verbosity=verbosity)
 
-    # properties ------------------------------------------------------------
-    @property
-    def Dmax(self):
-        return max(max(t.shape[2:]) for t in self._tensors)
-
-    @property
-    def beta(self):
-        return self._beta
-
-    @property
-    def n_bonds(self):
-        return self._n_bonds
-
-    @property
-    def n_tensors(self):
-        return self._n_tensors
-
-    @property
-    def logZ(self):
-        return self._logZ / self._n_tensors
-
-    @property
-    def tau(self):
-        return self._tau
-
-    @tau.setter
-    def tau(self, tau):
-        self._tau = tau
-        self._gates = tuple((-tau * h).expm() for h in self._hamiltonians)
-        # 2nd order Trotter-Suzuki and rho quadratic in psi (abstract_simple_update.py:29-36)
-        self._dbeta = 2 * (1 + self._is_second_order) * tau
-
-    def symmetry(self):
-        return self._symmetry
-
-    def get_bond_representation(self, i):
-        for j in range(self._n_tensors):
-            ind = (self._tensor_bond_indices[j] == i).nonzero()[0]
-            if ind.size:
-                return self._tensors[j].col_reps[ind[0]]
-        raise ValueError(f"Unknown bond {i}")
-
-    def get_bond_representations(self):
-        return [self.get_bond_representation(i) for i in range(self._n_bonds)]
-
-    def get_weights(self, *, sort=True):
-        return [w.toarray(sort=sort) for w in self._weights]
-
-    def bond_entanglement_entropy(self):
-        return np.array([-w @ np.log(w) for w in self.get_weights()])
-
-    def get_tensors(self):
-        """Tensors with sqrt(weights) absorbed on every virtual leg
-        (abstract_simple_update.py:73-92)."""
-        sqw = [w**-0.5 for w in self._weights]
-        out = []
-        for i, t0 in enumerate(self._tensors):
-            rswap = (0, 1, *range(3, t0.ndim))
-            t = t0
-            for leg in self._tensor_bond_indices[i]:
-                t = t.permute(rswap, (2,)) * sqw[leg]
-            out.append(t.permute((0, 1), tuple(range(2, t0.ndim))))
-        return out
-
-    # evolution ---------------------------------------------------------------
-    def evolve(self, beta_evolve):
-        """Imaginary-time evolution by beta_evolve with the 2nd-order Trotter string
-        (abstract_simple_update.py:94-147)."""
-        if beta_evolve < -1e-12:
-            raise ValueError("Cannot evolve for negative imaginary time")
-        if beta_evolve < 0.9 * self._dbeta:
-            return
-        niter = round(beta_evolve / self._dbeta)
-        for i in range(self._n_tensors):
-            self._tensors[i] = self._tensors[i].permute(*self._initial_swap[i])
-        self._elementary_update(self._n_updates)
-        for j in range(1, self._n_updates):
-            self._elementary_update(j)
-        for _ in range(niter - 1):
-            for j in range(self._n_updates):
-                self._elementary_update(j)
-        self._elementary_update(self._n_updates)
-        for i in range(self._n_tensors):
-            self._tensors[i] = self._tensors[i].permute(*self._final_swap[i])
-        self._beta += niter * self._dbeta
+    @classmethod
+    def load_from_file(cls, savefile, *, verbosity=0):
+        """Restore from a save_to_file npz (simple_update.py:538-597)."""
+        with np.load(savefile) as fin:
+            cls._check_savefile(fin, savefile)
+            ST = get_symmetric_tensor_type(str(fin["_SimpleUpdate_symmetry"][()]))
+            raw_hamilts = [
+                ST.load_from_dic(fin, prefix=f"_SimpleUpdate_hamiltonian_{i}")
+                for i in range(int(fin["_SimpleUpdate_n_hamiltonians"]))
+            ]
+            n_tensors = int(fin["_SimpleUpdate_n_tensors"])
+            tensors = [ST.load_from_dic(fin, prefix=f"_SimpleUpdate_tensor_{i}") for i in range(n_tensors)]
+            tbi = [fin[f"_SimpleUpdate_tensor_bond_indices_{i}"] for i in range(n_tensors)]
+            weights = [
+                DiagonalTensor.load_from_dic(fin, prefix=f"_SimpleUpdate_weights_{i}")
+                for i in range(int(fin["_SimpleUpdate_n_bonds"]))
+            ]
+            return cls(
+                fin["_SimpleUpdate_D"][()], fin["_SimpleUpdate_beta"][()], fin["_SimpleUpdate_tau"][()],
+                fin["_SimpleUpdate_rcutoff"][()], fin["_SimpleUpdate_degen_ratio"][()], tbi, tensors,
+                fin["_SimpleUpdate_raw_update_data"], raw_hamilts, weights, fin["_SimpleUpdate_logZ"][()],
+                verbosity=verbosity,
+            )
 
     def _elementary_update(self, i):
         left = self._tensors[self._left_indices[i]].permute(*self._lperm[i])

@@ -783,9 +783,12 @@ class SymmetricTensor:
         truncation to about nvals values (symmetric_tensor.py:1087-1132, 1250-1434).
         Every present block is diagonalised exactly on the device; missing blocks
         contribute zero eigenvalues that the cut removes."""
-        del dmax_full, rng, maxiter, arpack_tol
         if matmat is not None:
-            raise NotImplementedError("implicit (matmat) eigensolver is not available on device")
+            return self._sparse_eig(
+                matmat, nvals, symmetric=True, atol=atol, rtol=rtol, degen_ratio=degen_ratio,
+                dmax_full=dmax_full, compute_vectors=compute_vectors, rng=rng, maxiter=maxiter,
+                arpack_tol=arpack_tol,
+            )
         if not self.is_square_matrix():
             raise ValueError("Tensor legs do not define a square matrix")
         all_irreps, all_shapes = self.get_block_sizes(self._row_reps, self._col_reps, self._signature)
@@ -891,7 +894,198 @@ class SymmetricTensor:
         raise NotImplementedError("solve is not provided by the device engine")
 
     def eig(self, *, compute_vectors=True):
-        raise NotImplementedError("non-symmetric eigensolver is not provided by the device engine")
+        """All eigenvalues of every block (symmetric_tensor.py:779-815); exact
+        zeros of missing blocks are truncated as in the reference."""
+        if not self.is_square_matrix():
+            raise ValueError("Tensor legs do not define a square matrix")
+        return self._sparse_eig(None, 2**31, symmetric=False, compute_vectors=compute_vectors)
+
+    @classmethod
+    def eigs(cls, matmat, nvals, reps, signature, *, atol=0.0, rtol=None, degen_ratio=1.0, dmax_full=100,
+             compute_vectors=True, rng=None, maxiter=4000, arpack_tol=0):
+        """nvals largest-magnitude eigenvalues of the real square matrix defined
+        by matmat on tensors with row representations reps and signature
+        (symmetric_tensor.py:880-956)."""
+        m = cls.init_square_matrix(reps, signature)
+        return m.truncated_eig(
+            nvals, atol=atol, rtol=rtol, degen_ratio=degen_ratio, dmax_full=dmax_full,
+            compute_vectors=compute_vectors, rng=rng, maxiter=maxiter, arpack_tol=arpack_tol, matmat=matmat,
+        )
+
+    @classmethod
+    def eigsh(cls, matmat, nvals, reps, signature, *, atol=0.0, rtol=None, degen_ratio=1.0, dmax_full=100,
+              compute_vectors=True, rng=None, maxiter=4000, arpack_tol=0):
+        """Real symmetric counterpart of eigs (symmetric_tensor.py:1006-1085)."""
+        m = cls.init_square_matrix(reps, signature)
+        return m.truncated_eigh(
+            nvals, atol=atol, rtol=rtol, degen_ratio=degen_ratio, dmax_full=dmax_full,
+            compute_vectors=compute_vectors, rng=rng, maxiter=maxiter, arpack_tol=arpack_tol, matmat=matmat,
+        )
+
+    def truncated_eig(self, nvals, *, atol=0.0, rtol=None, degen_ratio=1.0, dmax_full=100,
+                      compute_vectors=True, rng=None, maxiter=4000, arpack_tol=0, matmat=None):
+        """nvals largest-magnitude eigenvalues of a real square block matrix, or of
+        the matrix implicitly defined by matmat (symmetric_tensor.py:958-1004)."""
+        return self._sparse_eig(
+            matmat, nvals, symmetric=False, atol=atol, rtol=rtol, degen_ratio=degen_ratio,
+            dmax_full=dmax_full, compute_vectors=compute_vectors, rng=rng, maxiter=maxiter,
+            arpack_tol=arpack_tol,
+        )
+
+    def _sparse_eig(self, matmat, nvals, *, symmetric, atol=0.0, rtol=None, degen_ratio=1.0, dmax_full=100,
+                    compute_vectors=False, rng=None, maxiter=4000, arpack_tol=0):
+        """Blockwise largest-magnitude eigenpairs (symmetric_tensor.py:1250-1434).
+
+        Per block of the square matrix, k = nvals // irrep_dim + 1 values are
+        wanted. Blocks smaller than max(dmax_full, 3k) are formed explicitly (from
+        self, or with one matmat on identity blocks, on the device) and
+        diagonalized in full with LAPACK on the host: they are at most a few
+        hundred wide. Larger blocks go to the device Krylov-Schur solver, which
+        applies matmat to one-block tensors whose data are the Krylov vectors
+        themselves. Then the multiplet-aware global cut of find_block_cuts."""
+        from ..misc_tools import krylov
+
+        if rng is None:
+            rng = np.random.default_rng()
+        if not self.is_square_matrix():
+            raise ValueError("Tensor legs do not define a square matrix")
+        use_blocks = matmat is None
+        irreps, shapes = self.get_block_sizes(self._row_reps, self._col_reps, self._signature)
+        irreps = np.asarray(irreps)
+        nb = len(irreps)
+        dims = np.array([self.irrep_dimension(r) for r in irreps], dtype=np.int64)
+        sigv = np.hstack((self._signature[: self._nrr], [True]))
+        vals = [np.zeros((0,))] * nb
+        absv = [np.zeros((1,))] * nb
+        vecs = [None] * nb  # host (d x k) array or device (buffer, d, k) row-major
+        ks = [int(nvals // dims[b] + 1) for b in range(nb)]
+        full = [b for b in range(nb) if shapes[b, 0] < max(dmax_full, 3 * ks[b])]
+        sparse = [b for b in range(nb) if b not in set(full)]
+
+        def keep(bi, lam, vec):
+            order = np.argsort(-np.abs(lam), kind="stable")[: ks[bi]]
+            vals[bi] = lam[order]
+            absv[bi] = np.abs(lam[order])
+            if vec is not None:
+                vecs[bi] = vec[:, order]
+
+        # dense blocks
+        if full:
+            if use_blocks:
+                st = self
+            else:
+                irr_full = np.ascontiguousarray(irreps[full])
+                rfull = self.init_representation(shapes[full, 0], irr_full)
+                eyes = [np.eye(int(shapes[b, 0])) for b in full]
+                st = matmat(type(self)(self._row_reps, (rfull,), eyes, irr_full, sigv))
+            present = {int(r): j for j, r in enumerate(st.block_irreps)}
+            for bi in full:
+                j = present.get(int(irreps[bi]))
+                if j is None:
+                    continue  # missing block: zero eigenvalues, truncated
+                b = _ops.download(st.blocks[j].contiguous())
+                if symmetric:
+                    lam, vec = np.linalg.eigh(b) if compute_vectors else (np.linalg.eigvalsh(b), None)
+                else:
+                    lam, vec = np.linalg.eig(b) if compute_vectors else (np.linalg.eigvals(b), None)
+                keep(bi, lam, vec)
+
+        # large blocks: device Krylov-Schur
+        for bi in sparse:
+            d = int(shapes[bi, 0])
+            v0 = _ops.upload(rng.standard_normal(size=(d,)))
+            irr1 = np.ascontiguousarray(irreps[bi : bi + 1])
+            if use_blocks:
+                js = [j for j, r in enumerate(self._block_irreps) if int(r) == int(irreps[bi])]
+                if not js:
+                    continue
+                j = js[0]
+                rs, cs = self._strides(j)
+                off = int(self._boffs[j])
+
+                def apply(x, y, off=off, rs=rs, cs=cs, d=d):
+                    _ops.grouped_gemm([(d, 1, d, off, rs, cs, 0, 1, 1, 0, 1)], self._data, x, y)
+            else:
+                brep = self.init_representation(np.ones((1,), dtype=np.int64), irr1)
+
+                def probe(x, brep=brep, irr1=irr1, d=d):
+                    st0 = type(self)._build(self._row_reps, (brep,), x, np.array([[d, 1]]), np.zeros(1, np.int64),
+                                            irr1, sigv)
+                    st1 = matmat(st0)
+                    js = [j for j, r in enumerate(st1.block_irreps) if int(r) == int(irr1[0])]
+                    return st1, (js[0] if js else None)
+
+                _, j0 = probe(v0)
+                if j0 is None:
+                    continue  # the block does not appear in the output: zero block
+
+                def apply(x, y, probe=probe, d=d):
+                    st1, j = probe(x)
+                    if j is None or tuple(st1._bshapes[j]) != (d, 1):
+                        raise ValueError("matmat changed the block structure of its input")
+                    _ops.copy2d([(1, d, int(st1._boffs[j]), 1, 1, 0, 1, 1, 0, 0, 0, 1.0, 0.0)], st1._data, y)
+            k = ks[bi]
+            try:
+                lam, X = krylov.largest_eigenpairs(
+                    apply, d, k, v0, maxiter=maxiter, tol=arpack_tol, compute_vectors=compute_vectors,
+                    symmetric=symmetric,
+                )
+            except krylov.KrylovNoConvergence as err:
+                print("Warning: Krylov-Schur did not converge", err)
+                lam, X = err.eigenvalues, err.eigenvectors
+                print(f"Keep {lam.size}/{k} converged values")
+            vals[bi] = lam
+            absv[bi] = np.abs(lam) if lam.size else np.zeros((1,))
+            vecs[bi] = None if X is None else (X, d, lam.size)
+
+        cuts = find_block_cuts(absv, nvals, dims=dims, atol=atol, rtol=rtol, degen_ratio=degen_ratio)
+        for bi in range(nb):
+            cuts[bi] = min(int(cuts[bi]), vals[bi].size)
+        if symmetric:
+            vals = [np.real(v) for v in vals]
+        if not compute_vectors:
+            _, s, _ = self._truncate(cuts, vals, irreps)
+            return s
+        return self._eig_vectors(cuts, vals, irreps, vecs)
+
+    def _eig_vectors(self, cuts, vals, irreps, vecs):
+        """Pack the kept eigenvectors of every block into one device buffer and
+        build (s, u) with _truncate."""
+        rows, qs, offs = [], [], []
+        total = 0
+        host_parts = []
+        dev_parts = []
+        for bi, v in enumerate(vecs):
+            c = int(cuts[bi])
+            if c == 0 or v is None:
+                rows.append(0)
+                qs.append(0)
+                offs.append(total)
+                continue
+            if isinstance(v, tuple):
+                X, d, q = v
+                dev_parts.append((X, d, q, total))
+            else:
+                if np.iscomplexobj(v) and np.abs(v[:, :c].imag).max(initial=0.0) > 0:
+                    raise NotImplementedError("complex eigenvectors are not supported by the float64 device tensors")
+                d, q = v.shape
+                host_parts.append((np.ascontiguousarray(np.real(v)), total))
+            rows.append(d)
+            qs.append(q)
+            offs.append(total)
+            total += d * q
+        if any(np.iscomplexobj(v[: int(c)]) and np.abs(np.imag(v[: int(c)])).max(initial=0.0) > 0
+               for v, c in zip(vals, cuts, strict=True)):
+            raise NotImplementedError("complex eigenvectors are not supported by the float64 device tensors")
+        host = np.zeros((total,))
+        for arr, o in host_parts:
+            host[o : o + arr.size] = arr.reshape(-1)
+        buf = _ops.upload(host)
+        for X, d, q, o in dev_parts:
+            _ops.copy2d([(1, d * q, 0, 1, 1, o, 1, 1, 0, 0, 0, 1.0, 0.0)], X, buf)
+        vals = [np.real(v) for v in vals]
+        u, s, _ = self._truncate(cuts, vals, irreps, left=(buf, offs, rows, qs, None))
+        return s, u
 
     @classmethod
     def init_square_matrix(cls, reps, signature):

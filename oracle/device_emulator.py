@@ -79,6 +79,13 @@ def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
     for m, n, ao, ars, acs, uo, so, vo, _wo in descs:
         q = min(m, n)
         am = _view(a, ao, m, n, ars, acs).copy()
+        if not np.isfinite(am).all():
+            # the Jacobi kernel runs its bounded sweeps on non-finite input and
+            # returns non-finite factors; callers detect and redo such matrices
+            u[uo : uo + m * q] = np.nan
+            s[so : so + q] = np.nan
+            vt[vo : vo + q * n] = np.nan
+            continue
         uu, ss, vv = np.linalg.svd(am, full_matrices=False)
         u[uo : uo + m * q] = uu.reshape(-1)
         s[so : so + q] = ss
@@ -123,7 +130,12 @@ def batched_cholinv(descs, G, R, Rinv, status):
             rr = np.linalg.cholesky(gm).T
             st[b] = 0
         except np.linalg.LinAlgError:
+            # the kernel stops at the failed pivot and leaves R / Rinv unwritten;
+            # NaN stands for that unspecified content
             st[b] = 1
+            ri[rio : rio + l * l] = np.nan
+            if r is not None:
+                r[ro : ro + l * l] = np.nan
             continue
         if r is not None:
             r[ro : ro + l * l] = rr.reshape(-1)
