@@ -174,6 +174,11 @@ _SIGNATURES = {
         c_int,
         [c_int, ctypes.POINTER(GemmGroup), c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp],
     ),
+    "fs_grouped_gemm_f64_multi": (
+        c_int,
+        [c_int, ctypes.POINTER(GemmGroup), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),
+         c_dbl, c_dbl, c_vp],
+    ),
     "fs_grouped_gemm_f64_split": (
         c_int,
         [c_int, ctypes.POINTER(GemmGroup), c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp],

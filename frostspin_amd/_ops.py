@@ -314,18 +314,37 @@ GEMM_GROUPS_PER_DISPATCH = 32
 
 
 def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
-    """groups: sequence of 11-tuples (m, n, k, a_off, a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld)."""
+    """groups: sequence of 11-tuples (m, n, k, a_off, a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld).
+    A, B, C: one device buffer each, or a list with one buffer per group (the
+    groups then read and write blocks of several tensors in one launch)."""
     n = len(groups)
     if n == 0:
         return
+    multi = any(isinstance(x, (list, tuple)) for x in (A, B, C))
+    if multi:
+        As, Bs, Cs = (list(x) if isinstance(x, (list, tuple)) else [x] * n for x in (A, B, C))
+        if not (len(As) == len(Bs) == len(Cs) == n):
+            raise ValueError("one operand buffer per group is required")
+        if split:  # the compensated form has a single-base entry point only
+            by = {}
+            for g, a, b, c in zip(groups, As, Bs, Cs, strict=True):
+                by.setdefault((a.data_ptr(), b.data_ptr(), c.data_ptr()), (a, b, c, []))[3].append(g)
+            for a, b, c, gs in by.values():
+                grouped_gemm(gs, a, b, c, alpha=alpha, beta=beta, split=True)
+            return
     arr = (_lib.GemmGroup * n)(*[_lib.GemmGroup(*map(int, g)) for g in groups])
-    fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
     flop = 2.0 * sum(float(g[0]) * float(g[1]) * float(g[2]) for g in groups)
     # the library launches one kernel per 32 groups that hold a non-empty group
     disp = sum(
         1 for c in range(0, n, GEMM_GROUPS_PER_DISPATCH)
         if any(g[0] > 0 and g[1] > 0 for g in groups[c : c + GEMM_GROUPS_PER_DISPATCH])
     )
+    if multi:
+        bases = [(_lib.c_vp * n)(*[_ptr(t) for t in lst]) for lst in (As, Bs, Cs)]
+        with _timed("gemm", flop, disp):
+            check(lib.fs_grouped_gemm_f64_multi(n, arr, *bases, float(alpha), float(beta), _stream()))
+        return
+    fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
     with _timed("gemm_split" if split else "gemm", flop, disp):
         check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
 

@@ -26,17 +26,18 @@ constexpr int kThreads = 256;
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// Each group carries its own operand base addresses, so one launch can take
+// its blocks from several tensors (the multi-operand entry point).
 struct GroupDev {
   int32_t m, n, k, tiles_n;
   int32_t a_rs, a_cs, b_rs, b_cs;
   int32_t c_ld, pad;
-  int64_t a_off, b_off, c_off;
+  const double *a;
+  const double *b;
+  double *c;
 };
 
 struct Batch {
-  const double *A;
-  const double *B;
-  double *C;
   double alpha, beta;
   int32_t ngroups, xcd_remap;
   int32_t tile_prefix[kMaxGroups + 1];
@@ -123,8 +124,8 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
   const int tn = t - tm * G.tiles_n;
   const int m0 = tm * BM;
   const int n0 = tn * BN;
-  const double *__restrict__ A = bt.A + G.a_off;
-  const double *__restrict__ B = bt.B + G.b_off;
+  const double *__restrict__ A = G.a;
+  const double *__restrict__ B = G.b;
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
 
   // epilogue: for v_mfma_f64_16x16x4f64 lane l holds D[l/16 + 4*r][l%16] of
   // each 16x16 tile (rows interleaved by 4, measured on gfx950)
-  double *__restrict__ C = bt.C + G.c_off;
+  double *__restrict__ C = G.c;
   const double alpha = bt.alpha, beta = bt.beta;
 #pragma unroll
   for (int a = 0; a < TI; ++a)
@@ -234,8 +235,8 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(Batch bt) {
   const int tn = t - tm * G.tiles_n;
   const int m0 = tm * BM;
   const int n0 = tn * BN;
-  const double *__restrict__ A = bt.A + G.a_off;
-  const double *__restrict__ B = bt.B + G.b_off;
+  const double *__restrict__ A = G.a;
+  const double *__restrict__ B = G.b;
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -332,7 +333,7 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(Batch bt) {
         for (int r = 0; r < 4; ++r) acc[a][b][r] += static_cast<double>(part[a][b][r]);
   }
 
-  double *__restrict__ C = bt.C + G.c_off;
+  double *__restrict__ C = G.c;
   const double alpha = bt.alpha, beta = bt.beta;
 #pragma unroll
   for (int a = 0; a < TI; ++a)
@@ -374,9 +375,12 @@ inline int pick_tile(int ng, const fs_gemm_group *g) {
   return tiles64 < kCUs ? 32 : 64;
 }
 
+// A, B, C: per-group operand bases (stride 1 for one shared base, 0 entries
+// otherwise unused); group offsets are relative to them.
 template <bool SPLIT>
-int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, const double *B,
-                   double *C, double alpha, double beta, void *stream) {
+int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const *A,
+                   const double *const *B, double *const *C, int base_stride, double alpha,
+                   double beta, void *stream) {
   FS_CHECK_ARG(ngroups >= 0, "grouped gemm: negative group count");
   FS_CHECK_ARG(ngroups == 0 || groups, "grouped gemm: null groups");
   int g0 = 0;
@@ -385,9 +389,6 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
     const int tsz = SPLIT ? 64 : pick_tile(g_end - g0, groups + g0);
     Batch bt;
     std::memset(&bt, 0, sizeof(bt));
-    bt.A = A;
-    bt.B = B;
-    bt.C = C;
     bt.alpha = alpha;
     bt.beta = beta;
     int ng = 0;
@@ -399,7 +400,11 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
       if (q.m == 0 || q.n == 0) continue;
       FS_CHECK_ARG(q.m < (1ll << 31) && q.n < (1ll << 31) && q.k < (1ll << 31),
                    "grouped gemm: dimension too large");
-      FS_CHECK_ARG(A && B && C, "grouped gemm: null operand");
+      const double *a_base = A[g0 * base_stride];
+      const double *b_base = B[g0 * base_stride];
+      double *c_base = C[g0 * base_stride];
+      FS_CHECK_ARG(a_base && b_base && c_base, "grouped gemm: null operand");
+      FS_CHECK_ARG(q.a_off >= 0 && q.b_off >= 0 && q.c_off >= 0, "grouped gemm: negative offset");
       GroupDev &d = bt.g[ng];
       d.m = static_cast<int32_t>(q.m);
       d.n = static_cast<int32_t>(q.n);
@@ -410,9 +415,9 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *A, co
       d.b_rs = static_cast<int32_t>(q.b_rs);
       d.b_cs = static_cast<int32_t>(q.b_cs);
       d.c_ld = static_cast<int32_t>(q.c_ld);
-      d.a_off = q.a_off;
-      d.b_off = q.b_off;
-      d.c_off = q.c_off;
+      d.a = a_base + q.a_off;
+      d.b = b_base + q.b_off;
+      d.c = c_base + q.c_off;
       tiles += fs::ceil_div(q.m, tsz) * d.tiles_n;
       FS_CHECK_ARG(tiles < (1ll << 31), "grouped gemm: too many tiles");
       bt.tile_prefix[ng + 1] = static_cast<int32_t>(tiles);
@@ -445,15 +450,26 @@ extern "C" {
 
 int fs_grouped_gemm_f64(int ngroups, const fs_gemm_group *groups, const double *A,
                         const double *B, double *C, double alpha, double beta, void *stream) {
-  return fs::guarded(
-      [&]() -> int { return launch_grouped<false>(ngroups, groups, A, B, C, alpha, beta, stream); });
+  return fs::guarded([&]() -> int {
+    return launch_grouped<false>(ngroups, groups, &A, &B, &C, 0, alpha, beta, stream);
+  });
+}
+
+int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups, const double *const *A,
+                              const double *const *B, double *const *C, double alpha, double beta,
+                              void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(ngroups == 0 || (A && B && C), "grouped gemm: null base array");
+    return launch_grouped<false>(ngroups, groups, A, B, C, 1, alpha, beta, stream);
+  });
 }
 
 int fs_grouped_gemm_f64_split(int ngroups, const fs_gemm_group *groups, const double *A,
                               const double *B, double *C, double alpha, double beta,
                               void *stream) {
-  return fs::guarded(
-      [&]() -> int { return launch_grouped<true>(ngroups, groups, A, B, C, alpha, beta, stream); });
+  return fs::guarded([&]() -> int {
+    return launch_grouped<true>(ngroups, groups, &A, &B, &C, 0, alpha, beta, stream);
+  });
 }
 
 } // extern "C"

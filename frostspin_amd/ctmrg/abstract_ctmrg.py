@@ -20,6 +20,7 @@ from .ctm_contract import (
     contract_ur,
 )
 from .ctm_environment import CTMEnvironment
+from .ctm_renormalize import construct_projectors_batch
 
 
 class AbstractCTMRG:
@@ -321,6 +322,14 @@ class AbstractCTMRG:
                 e.get_T4(x, y + 1).shape, e.get_A(x + 1, y + 1).shape, e.get_T2(x + 2, y + 1).shape,
                 e.get_C4(x, y + 3).shape, e.get_T3(x + 1, y + 2).shape, e.get_C3(x + 2, y + 2).shape,
             )
+
+    def _projectors_batch(self, problems):
+        """Projector pairs of several independent (corner1, corner2, corner3,
+        corner4, last_renormalized) problems, computed in shared launches."""
+        return construct_projectors_batch(
+            problems, self.chi_target, block_chi_ratio=self.block_chi_ratio, ncv_ratio=self.ncv_ratio,
+            rtol=self.cutoff, degen_ratio=self.degen_ratio,
+        )
 
     # enlarged corners (cached in the environment) -------------------------
     def _enlarged(self, getter, setter, build, x, y, free_memory):

@@ -39,34 +39,67 @@ def construct_projectors(
 ):
     """Isometries P (mid -> corner2 columns) and Pt keeping about chi_target
     singular values of R @ Rt; interface of ctm_renormalize.py:6-298."""
-    shared = sorted(
-        set(corner1.block_irreps.tolist())
-        & set(corner2.block_irreps.tolist())
-        & set(corner3.block_irreps.tolist())
-        & set(corner4.block_irreps.tolist())
-    )
-    shared = np.array(shared, dtype=corner2.block_irreps.dtype)
-    ind = [c.block_irreps.searchsorted(shared) for c in (corner1, corner2, corner3, corner4)]
+    return construct_projectors_batch(
+        [(corner1, corner2, corner3, corner4, last_renormalized)], chi_target,
+        block_chi_ratio=block_chi_ratio, ncv_ratio=ncv_ratio, atol=atol, rtol=rtol,
+        degen_ratio=degen_ratio, rsvd_seed=rsvd_seed,
+    )[0]
 
-    # number of values per sector, from the last renormalized corner (:244-250)
-    block_chi = np.zeros(shared.shape, dtype=int)
-    last_irreps = last_renormalized.block_irreps
-    for bi, i in enumerate(last_irreps.searchsorted(shared)):
-        if i < last_irreps.size and shared[bi] == last_irreps[i]:
-            block_chi[bi] = last_renormalized._bshapes[i, 1]
-    block_chi = np.maximum(block_chi + 10, (block_chi_ratio * block_chi).astype(int))
-    block_chi = np.minimum(chi_target + 10, block_chi)
+
+def construct_projectors_batch(
+    problems,
+    chi_target,
+    *,
+    block_chi_ratio=1.1,
+    ncv_ratio=2.0,
+    atol=0.0,
+    rtol=None,
+    degen_ratio=1.0,
+    rsvd_seed=0,
+):
+    """construct_projectors for several independent (corner1, corner2, corner3,
+    corner4, last_renormalized) problems at once: the sectors of all problems
+    share every launch (operands gathered from the different corners through the
+    per-group operand bases of the grouped GEMM), while the truncation is decided
+    per problem exactly as for a single call. Returns a list of (P, Pt)."""
+    sectors = []  # (problem, shared irrep, block_chi, corner block indices)
+    for pi, (corner1, corner2, corner3, corner4, last) in enumerate(problems):
+        shared = sorted(
+            set(corner1.block_irreps.tolist())
+            & set(corner2.block_irreps.tolist())
+            & set(corner3.block_irreps.tolist())
+            & set(corner4.block_irreps.tolist())
+        )
+        shared = np.array(shared, dtype=corner2.block_irreps.dtype)
+        ind = [c.block_irreps.searchsorted(shared) for c in (corner1, corner2, corner3, corner4)]
+        # number of values per sector, from the last renormalized corner (:244-250)
+        block_chi = np.zeros(shared.shape, dtype=int)
+        last_irreps = last.block_irreps
+        for bi, i in enumerate(last_irreps.searchsorted(shared)):
+            if i < last_irreps.size and shared[bi] == last_irreps[i]:
+                block_chi[bi] = last._bshapes[i, 1]
+        block_chi = np.maximum(block_chi + 10, (block_chi_ratio * block_chi).astype(int))
+        block_chi = np.minimum(chi_target + 10, block_chi)
+        for bi in range(len(shared)):
+            sectors.append((pi, shared[bi], int(block_chi[bi]), [int(x[bi]) for x in ind]))
 
     split = _precision_split()
-    c1 = bl.Mats.from_tensor_blocks(corner1, ind[0])
-    c2 = bl.Mats.from_tensor_blocks(corner2, ind[1])
-    c3 = bl.Mats.from_tensor_blocks(corner3, ind[2])
-    c4 = bl.Mats.from_tensor_blocks(corner4, ind[3])
-    R = bl.gemm(c1, c2, split=split)
-    Rt = bl.gemm(c3, c4, split=split)
+    # sectors are ordered by problem: gather each corner's blocks problem by problem
+    cmats = [
+        bl.Mats.concat(
+            [
+                bl.Mats.from_tensor_blocks(prob[c], [sec[3][c] for sec in sectors if sec[0] == pi])
+                for pi, prob in enumerate(problems)
+            ]
+        )
+        for c in range(4)
+    ]
+    R = bl.gemm(cmats[0], cmats[1], split=split)
+    Rt = bl.gemm(cmats[2], cmats[3], split=split)
     M = bl.gemm(R, Rt, split=split)
 
-    n = len(shared)
+    n = len(sectors)
+    block_chi = np.array([sec[2] for sec in sectors], dtype=int)
     mins = np.minimum(M.rows, M.cols)
     ncv = (ncv_ratio * block_chi).astype(int) + 10
     # the reference uses ARPACK (block_chi values) where min(shape) >= max(100, 4 ncv)
@@ -98,8 +131,15 @@ def construct_projectors(
         if sparse_ref[i] or not dense[i]:
             s_of[i] = s_of[i][: block_chi[i]]
 
-    dims = [corner2.irrep_dimension(r) for r in shared]
-    cuts = find_block_cuts(s_of, chi_target, dims=dims, atol=atol, rtol=rtol, degen_ratio=degen_ratio)
+    # multiplet-aware cut, per problem
+    cuts = np.zeros((n,), dtype=np.int64)
+    for pi, prob in enumerate(problems):
+        ids = [i for i in range(n) if sectors[i][0] == pi]
+        if not ids:
+            continue
+        dims = [prob[1].irrep_dimension(sectors[i][1]) for i in ids]
+        cuts[ids] = find_block_cuts([s_of[i] for i in ids], chi_target, dims=dims, atol=atol, rtol=rtol,
+                                    degen_ratio=degen_ratio)
     non_empty = cuts.nonzero()[0]
 
     # s^-1/2 of the kept values, one device vector for all sectors
@@ -110,40 +150,42 @@ def construct_projectors(
     w_dev = _ops.upload(w_host)
 
     # Us = U[:, :cut] * s^-1/2 and Vs = s^-1/2 * VT[:cut, :] per sector (grouped by source buffer)
-    def gather(src_of, take_rows):
-        mats = []
-        for i in non_empty:
-            buf_mats, j = src_of[i]
-            mats.append((buf_mats, j))
-        return mats
+    kept = [int(cuts[i]) for i in non_empty]
+    Us = _scaled([U_of[i] for i in non_empty], kept, w_dev, w_off, cols=True)
+    Vs = _scaled([VT_of[i] for i in non_empty], kept, w_dev, w_off, cols=False)
 
-    us_parts = gather(U_of, False)
-    vs_parts = gather(VT_of, True)
-    Us = _scaled(us_parts, [int(cuts[i]) for i in non_empty], w_dev, w_off, cols=True)
-    Vs = _scaled(vs_parts, [int(cuts[i]) for i in non_empty], w_dev, w_off, cols=False)
-
+    # P = Us^T R (cut x n2) and Pt = Rt Vs^T (m3 x cut), one buffer per problem
+    ne_of = [[k for k, i in enumerate(non_empty) if sectors[i][0] == pi] for pi in range(len(problems))]
     Rk = R.select(non_empty)
     Rtk = Rt.select(non_empty)
-    P_mats = bl.gemm(Us.T(), Rk, split=split)  # cut x n2
-    Pt_mats = bl.gemm(Rtk, Vs.T(), split=split)  # m3 x cut
+    P_out = [bl.Mats.packed(np.array([kept[k] for k in ks], np.int64), Rk.cols[ks]) for ks in ne_of]
+    Pt_out = [bl.Mats.packed(Rtk.rows[ks], np.array([kept[k] for k in ks], np.int64)) for ks in ne_of]
+    order = [k for ks in ne_of for k in ks]
+    if order:
+        bl.gemm_into(Us.T().select(order), Rk.select(order), bl.Mats.concat(P_out), split=split)
+        bl.gemm_into(Rtk.select(order), Vs.T().select(order), bl.Mats.concat(Pt_out), split=split)
 
-    block_irreps = corner2.block_irreps[ind[1][non_empty]]
-    mid_rep = corner2.init_representation(cuts[non_empty], block_irreps)
-    nrr2 = corner2.n_row_reps
-    sp = np.zeros((nrr2 + 1,), dtype=bool)
-    sp[1:] = corner2.signature[nrr2:]
-    spt = np.ones((nrr2 + 1,), dtype=bool)
-    spt[:-1] = ~corner2.signature[nrr2:]
-    ST = type(corner2)
-    P = ST._build(
-        (mid_rep,), corner2.col_reps, P_mats.buf,
-        np.stack([P_mats.rows, P_mats.cols], axis=1), P_mats.off, block_irreps, sp,
-    ).transpose()
-    Pt = ST._build(
-        corner2.col_reps, (mid_rep,), Pt_mats.buf,
-        np.stack([Pt_mats.rows, Pt_mats.cols], axis=1), Pt_mats.off, block_irreps, spt,
-    )
-    return P, Pt
+    out = []
+    for pi, (_, corner2, _, _, _) in enumerate(problems):
+        ks = ne_of[pi]
+        sec_ids = non_empty[ks] if ks else np.zeros((0,), np.int64)
+        block_irreps = np.array([sectors[i][1] for i in sec_ids], dtype=corner2.block_irreps.dtype)
+        mid_rep = corner2.init_representation(cuts[sec_ids], block_irreps)
+        nrr2 = corner2.n_row_reps
+        sp = np.zeros((nrr2 + 1,), dtype=bool)
+        sp[1:] = corner2.signature[nrr2:]
+        spt = np.ones((nrr2 + 1,), dtype=bool)
+        spt[:-1] = ~corner2.signature[nrr2:]
+        ST = type(corner2)
+        Pm, Ptm = P_out[pi], Pt_out[pi]
+        P = ST._build(
+            (mid_rep,), corner2.col_reps, Pm.buf, np.stack([Pm.rows, Pm.cols], axis=1), Pm.off, block_irreps, sp,
+        ).transpose()
+        Pt = ST._build(
+            corner2.col_reps, (mid_rep,), Ptm.buf, np.stack([Ptm.rows, Ptm.cols], axis=1), Ptm.off, block_irreps, spt,
+        )
+        out.append((P, Pt))
+    return out
 
 
 def _scaled(parts, cuts, w_dev, w_off, *, cols):

@@ -2,8 +2,9 @@
 
 One iterate() = up, right, down, left moves. Each move builds the projectors
 of every inequivalent site from the four enlarged corners (freeing the two
-corners it renormalizes), then renormalizes C, T, C of every site, then swaps
-the new tensors in. The move table below fixes, for each direction, which
+corners it renormalizes) -- the sites' projector computations are independent
+and share their device launches -- then renormalizes C, T, C of every site,
+then swaps the new tensors in. The move table below fixes, for each direction, which
 corners feed construct_projectors, which tensors are renormalized and the
 coordinate offsets, exactly as in the reference so environments agree leg by
 leg.
@@ -11,7 +12,6 @@ leg.
 
 from .abstract_ctmrg import AbstractCTMRG
 from .ctm_renormalize import (
-    construct_projectors,
     renormalize_C1_left,
     renormalize_C1_up,
     renormalize_C2_right,
@@ -40,24 +40,26 @@ class SequentialCTMRG(AbstractCTMRG):
         self.down_move()
         self.left_move()
 
-    def _projectors(self, c1, c2, c3, c4, last):
-        return construct_projectors(
-            c1, c2, c3, c4, self.chi_target, last,
-            block_chi_ratio=self.block_chi_ratio, ncv_ratio=self.ncv_ratio,
-            rtol=self.cutoff, degen_ratio=self.degen_ratio,
-        )
+    def _move_projectors(self, store, problems):
+        """problems: ((x, y) of the stored projectors, corner1..4, last corner)
+        per site; all sites share the projector launches."""
+        pairs = self._projectors_batch([p[1:] for p in problems])
+        for (xy, *_), (P, Pt) in zip(problems, pairs, strict=True):
+            store(*xy, P, Pt)
 
     def up_move(self):
         e = self._env
-        for x, y in self._site_coords:
-            P, Pt = self._projectors(
+        self._move_projectors(e.store_up_projectors, [
+            (
+                (x + 2, y),
                 self.construct_enlarged_dr(x, y),
                 self.construct_enlarged_ur(x, y, free_memory=True),
                 self.construct_enlarged_ul(x, y, free_memory=True),
                 self.construct_enlarged_dl(x, y),
                 e.get_C2(x + 2, y + 1),
             )
-            e.store_up_projectors(x + 2, y, P, Pt)
+            for x, y in self._site_coords
+        ])
         for x, y in self._site_coords:
             P = e.get_up_P(x + 1, y)
             Pt = e.get_up_Pt(x, y)
@@ -69,15 +71,17 @@ class SequentialCTMRG(AbstractCTMRG):
 
     def right_move(self):
         e = self._env
-        for x, y in self._site_coords:
-            P, Pt = self._projectors(
+        self._move_projectors(e.store_right_projectors, [
+            (
+                (x + 3, y + 2),
                 self.construct_enlarged_dl(x, y),
                 self.construct_enlarged_dr(x, y, free_memory=True),
                 self.construct_enlarged_ur(x, y, free_memory=True),
                 self.construct_enlarged_ul(x, y),
                 e.get_C3(x + 2, y + 2).transpose(),
             )
-            e.store_right_projectors(x + 3, y + 2, P, Pt)
+            for x, y in self._site_coords
+        ])
         for x, y in self._site_coords:
             P = e.get_right_P(x, y + 1)
             Pt = e.get_right_Pt(x, y)
@@ -89,15 +93,17 @@ class SequentialCTMRG(AbstractCTMRG):
 
     def down_move(self):
         e = self._env
-        for x, y in self._site_coords:
-            P, Pt = self._projectors(
+        self._move_projectors(e.store_down_projectors, [
+            (
+                (x + 1, y + 3),
                 self.construct_enlarged_ul(x, y),
                 self.construct_enlarged_dl(x, y, free_memory=True),
                 self.construct_enlarged_dr(x, y, free_memory=True),
                 self.construct_enlarged_ur(x, y),
                 e.get_C4(x + 1, y + 2),
             )
-            e.store_down_projectors(x + 1, y + 3, P, Pt)
+            for x, y in self._site_coords
+        ])
         for x, y in self._site_coords:
             P = e.get_down_P(x - 1, y)
             Pt = e.get_down_Pt(x, y)
@@ -109,15 +115,17 @@ class SequentialCTMRG(AbstractCTMRG):
 
     def left_move(self):
         e = self._env
-        for x, y in self._site_coords:
-            P, Pt = self._projectors(
+        self._move_projectors(e.store_left_projectors, [
+            (
+                (x, y + 1),
                 self.construct_enlarged_ur(x, y),
                 self.construct_enlarged_ul(x, y, free_memory=True),
                 self.construct_enlarged_dl(x, y, free_memory=True),
                 self.construct_enlarged_dr(x, y),
                 e.get_C1(x + 1, y + 1),
             )
-            e.store_left_projectors(x, y + 1, P, Pt)
+            for x, y in self._site_coords
+        ])
         for x, y in self._site_coords:
             P = e.get_left_P(x, y - 1)
             Pt = e.get_left_Pt(x, y)

@@ -10,7 +10,6 @@ that environments agree leg by leg.
 
 from .abstract_ctmrg import AbstractCTMRG
 from .ctm_renormalize import (
-    construct_projectors,
     renormalize_quadrant,
     renormalize_T1,
     renormalize_T2,
@@ -40,31 +39,32 @@ class SimultaneousCTMRG(AbstractCTMRG):
             self.construct_enlarged_ul(x, y)
             self.construct_enlarged_dl(x, y)
 
-    def _projectors(self, c1, c2, c3, c4, last):
-        return construct_projectors(
-            c1, c2, c3, c4, self.chi_target, last,
-            block_chi_ratio=self.block_chi_ratio, ncv_ratio=self.ncv_ratio,
-            rtol=self.cutoff, degen_ratio=self.degen_ratio,
-        )
-
     def compute_projectors(self):
         """Projectors of the four directions from the same corners
         (simultaneous_ctmrg.py:82-150): each direction cuts the 2x2 block of
-        enlarged corners along a different bond."""
+        enlarged corners along a different bond. All sites and directions are
+        independent and share their device launches."""
         e = self._env
+        problems, stores = [], []
         for x, y in self._site_coords:
             dl = self.construct_enlarged_dl(x, y)
             dr = self.construct_enlarged_dr(x, y)
             ur = self.construct_enlarged_ur(x, y)
             ul = self.construct_enlarged_ul(x, y)
-            P, Pt = self._projectors(dr, ur, ul, dl, e.get_C2(x + 2, y + 1))
-            e.store_up_projectors(x + 2, y, P, Pt)
-            P, Pt = self._projectors(dl, dr, ur, ul, e.get_C3(x + 2, y + 2).transpose())
-            e.store_right_projectors(x + 3, y + 2, P, Pt)
-            P, Pt = self._projectors(ul, dl, dr, ur, e.get_C4(x + 1, y + 2))
-            e.store_down_projectors(x + 1, y + 3, P, Pt)
-            P, Pt = self._projectors(ur, ul, dl, dr, e.get_C1(x + 1, y + 1))
-            e.store_left_projectors(x, y + 1, P, Pt)
+            problems += [
+                (dr, ur, ul, dl, e.get_C2(x + 2, y + 1)),
+                (dl, dr, ur, ul, e.get_C3(x + 2, y + 2).transpose()),
+                (ul, dl, dr, ur, e.get_C4(x + 1, y + 2)),
+                (ur, ul, dl, dr, e.get_C1(x + 1, y + 1)),
+            ]
+            stores += [
+                (e.store_up_projectors, x + 2, y),
+                (e.store_right_projectors, x + 3, y + 2),
+                (e.store_down_projectors, x + 1, y + 3),
+                (e.store_left_projectors, x, y + 1),
+            ]
+        for (store, px, py), (P, Pt) in zip(stores, self._projectors_batch(problems), strict=True):
+            store(px, py, P, Pt)
 
     def renormalize_tensors(self):
         """New C1..C4 (projected enlarged corners) and T1..T4 of every site

@@ -13,7 +13,6 @@ from . import rdm
 from .abstract_ctmrg import AbstractCTMRG
 from .ctm_contract import contract_dl, contract_dr, contract_ul, contract_ur
 from .ctm_renormalize import (
-    construct_projectors,
     renormalize_quadrant,
     renormalize_T1,
     renormalize_T2,
@@ -88,37 +87,28 @@ class SymmetrizedCTMRG(AbstractCTMRG):
         self.compute_projectors()
         self.renormalize_tensors()
 
-    def _projectors(self, c1, c2, c3, c4, last):
-        return construct_projectors(
-            c1, c2, c3, c4, self.chi_target, last,
-            block_chi_ratio=self.block_chi_ratio, ncv_ratio=self.ncv_ratio,
-            rtol=self.cutoff, degen_ratio=self.degen_ratio,
-        )
-
     def compute_projectors(self):
-        """Four projector pairs from the corners of site (0, 0), each stored
-        with its mirror (symmetrized_ctmrg.py:183-224)."""
+        """Four projector pairs from the corners of site (0, 0), computed in
+        shared launches, each stored with its mirror (symmetrized_ctmrg.py:183-224)."""
         e = self._env
         dl = self.construct_enlarged_dl(0, 0)
         dr = self.construct_enlarged_dr(0, 0)
         ur = self.construct_enlarged_ur(0, 0)
         ul = self.construct_enlarged_ul(0, 0)
-
-        P, Pt = self._projectors(dr, ur, ul, dl, e.get_C2(0, 1))
-        e.store_up_projectors(0, 0, P, Pt)
-        e.store_down_projectors(0, 1, *_mirror_projectors(P, Pt))
-
-        P, Pt = self._projectors(dl, dr, ur, ul, e.get_C3(0, 0).transpose())
-        e.store_right_projectors(1, 0, P, Pt)
-        e.store_left_projectors(0, 0, *_mirror_projectors(P, Pt))
-
-        P, Pt = self._projectors(ul, dl, dr, ur, e.get_C4(1, 0))
-        e.store_down_projectors(1, 1, P, Pt)
-        e.store_up_projectors(0, 1, *_mirror_projectors(P, Pt))
-
-        P, Pt = self._projectors(ur, ul, dl, dr, e.get_C1(1, 1))
-        e.store_left_projectors(0, 1, P, Pt)
-        e.store_right_projectors(0, 0, *_mirror_projectors(P, Pt))
+        (Pu, Ptu), (Pr, Ptr), (Pd, Ptd), (Pl, Ptl) = self._projectors_batch([
+            (dr, ur, ul, dl, e.get_C2(0, 1)),
+            (dl, dr, ur, ul, e.get_C3(0, 0).transpose()),
+            (ul, dl, dr, ur, e.get_C4(1, 0)),
+            (ur, ul, dl, dr, e.get_C1(1, 1)),
+        ])
+        e.store_up_projectors(0, 0, Pu, Ptu)
+        e.store_down_projectors(0, 1, *_mirror_projectors(Pu, Ptu))
+        e.store_right_projectors(1, 0, Pr, Ptr)
+        e.store_left_projectors(0, 0, *_mirror_projectors(Pr, Ptr))
+        e.store_down_projectors(1, 1, Pd, Ptd)
+        e.store_up_projectors(0, 1, *_mirror_projectors(Pd, Ptd))
+        e.store_left_projectors(0, 1, Pl, Ptl)
+        e.store_right_projectors(0, 0, *_mirror_projectors(Pl, Ptl))
 
     def renormalize_tensors(self):
         """Renormalized tensors of site (0, 0); those of site (0, 1) are their

@@ -17,7 +17,12 @@ from .. import _ops
 
 @dataclass
 class Mats:
-    buf: torch.Tensor
+    """Matrices in device memory: matrix i is element (r, c) at
+    buf[off[i] + r * rs[i] + c * cs[i]]. buf is one buffer, or a list with one
+    buffer per matrix (blocks gathered from several tensors; only GEMM operands
+    and outputs may be of that kind)."""
+
+    buf: object
     off: np.ndarray
     rows: np.ndarray
     cols: np.ndarray
@@ -26,6 +31,10 @@ class Mats:
 
     def __len__(self):
         return len(self.off)
+
+    def bufs(self):
+        """One buffer per matrix."""
+        return list(self.buf) if isinstance(self.buf, list) else [self.buf] * len(self)
 
     @classmethod
     def packed(cls, rows, cols, buf=None):
@@ -49,30 +58,52 @@ class Mats:
         strides = np.array([st._strides(b) for b in ids], dtype=np.int64).reshape(-1, 2)
         return cls(st._data, off, rows, cols, strides[:, 0].copy(), strides[:, 1].copy())
 
+    @classmethod
+    def concat(cls, parts):
+        """All matrices of several Mats, in order (per-matrix buffers if they differ)."""
+        parts = list(parts)
+        cat = [np.concatenate([getattr(p, f) for p in parts]).astype(np.int64) if parts else np.zeros((0,), np.int64)
+               for f in ("off", "rows", "cols", "rs", "cs")]
+        single = [p.buf for p in parts if not isinstance(p.buf, list)]
+        if len(single) == len(parts) and all(b is single[0] for b in single) and single:
+            return cls(single[0], *cat)
+        return cls([b for p in parts for b in p.bufs()], *cat)
+
     def T(self):
         """Transposed views (no copy)."""
         return Mats(self.buf, self.off, self.cols, self.rows, self.cs, self.rs)
 
     def select(self, idx):
         idx = list(idx)
-        return Mats(self.buf, self.off[idx], self.rows[idx], self.cols[idx], self.rs[idx], self.cs[idx])
+        buf = [self.buf[i] for i in idx] if isinstance(self.buf, list) else self.buf
+        return Mats(buf, self.off[idx], self.rows[idx], self.cols[idx], self.rs[idx], self.cs[idx])
 
 
 def gemm(A, B, *, alpha=1.0, split=False):
     """C_i = alpha * A_i @ B_i for all i in one grouped launch; C packed row-major."""
-    assert len(A) == len(B)
-    assert np.array_equal(A.cols, B.rows)
     C = Mats.packed(A.rows, B.cols)
+    gemm_into(A, B, C, alpha=alpha, split=split)
+    return C
+
+
+def gemm_into(A, B, C, *, alpha=1.0, split=False):
+    """C_i = alpha * A_i @ B_i into given row-major outputs (C_i row stride c.rs,
+    column stride 1); operands and outputs may span several buffers."""
+    assert len(A) == len(B) == len(C)
+    assert np.array_equal(A.cols, B.rows)
+    assert np.array_equal(C.rows, A.rows) and np.array_equal(C.cols, B.cols)
+    assert (C.cs[C.rows * C.cols > 0] == 1).all()
     groups = [
         (A.rows[i], B.cols[i], A.cols[i], A.off[i], A.rs[i], A.cs[i], B.off[i], B.rs[i], B.cs[i],
-         C.off[i], C.cols[i])
+         C.off[i], C.rs[i])
         for i in range(len(A))
     ]
-    if A.buf is B.buf or A.buf.data_ptr() == B.buf.data_ptr():
+    if any(isinstance(x.buf, list) for x in (A, B, C)):
+        _ops.grouped_gemm(groups, A.bufs(), B.bufs(), C.bufs(), alpha=alpha, split=split)
+    elif A.buf is B.buf or A.buf.data_ptr() == B.buf.data_ptr():
         _ops.grouped_gemm(groups, A.buf, A.buf, C.buf, alpha=alpha, split=split)
     else:
         _ops.grouped_gemm(groups, A.buf, B.buf, C.buf, alpha=alpha, split=split)
-    return C
 
 
 def scale_slice(A, nrows, ncols, *, row_w=None, col_w=None, w_off=None):

@@ -239,6 +239,19 @@ int fs_grouped_gemm_f64(int ngroups, const fs_gemm_group *groups,
                         double alpha, double beta, void *stream);
 
 /*
+ * Same product with per-group operand bases: group i reads A[i] + a_off,
+ * B[i] + b_off and writes C[i] + c_off (host arrays of ngroups device
+ * pointers). One launch then covers blocks that live in different tensors,
+ * e.g. the sector products of several independent projector computations
+ * (construct_projectors, ctm_renormalize.py:6-136, called once per site and
+ * direction by the CTMRG schedules).
+ */
+int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups,
+                              const double *const *A, const double *const *B,
+                              double *const *C, double alpha, double beta,
+                              void *stream);
+
+/*
  * Compensated FP32 form: A and B are split into hi/lo FP32 parts on the fly
  * (x = hi + lo), the three products hi*hi + hi*lo + lo*hi run on FP32 MFMA
  * with FP32 accumulation per K chunk and FP64 accumulation across chunks.

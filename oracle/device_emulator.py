@@ -63,10 +63,13 @@ def _view(buf, off, rows, cols, rs, cs):
 
 
 def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
-    a, b, c = _np(A), _np(B), _np(C)
-    for m, n, k, ao, ars, acs, bo, brs, bcs, co, cld in groups:
+    n_g = len(groups)
+    As, Bs, Cs = (list(x) if isinstance(x, (list, tuple)) else [x] * n_g for x in (A, B, C))
+    assert len(As) == len(Bs) == len(Cs) == n_g
+    for (m, n, k, ao, ars, acs, bo, brs, bcs, co, cld), A1, B1, C1 in zip(groups, As, Bs, Cs, strict=True):
         if m == 0 or n == 0:
             continue
+        a, b, c = _np(A1), _np(B1), _np(C1)
         am = _view(a, ao, m, k, ars, acs)
         bm = _view(b, bo, k, n, brs, bcs)
         cm = _view(c, co, m, n, cld, 1)

@@ -113,3 +113,30 @@ def test_classical_ising_vs_oracle(backend, beta, chi, n_iter, sym):
     ref = np.sort(np.concatenate(o["s"].blocks))
     assert got.shape == ref.shape
     assert np.abs(got - ref).max() < 1e-10
+
+
+def test_batched_projectors_match_single(backend):
+    """construct_projectors_batch over the two sites of a move gives the
+    projector pairs of separate construct_projectors calls (gauge-invariant
+    products Pt P^T compared; kept multiplets identical)."""
+    from frostspin_amd.ctmrg.ctm_renormalize import construct_projectors, construct_projectors_batch
+
+    u1, _ = _cell_tensors(np.random.default_rng(5))
+    ctm = SequentialCTMRG.from_elementary_tensors("AB\nBA", u1, 20)
+    ctm.iterate()
+    e = ctm._env
+    problems = []
+    for x, y in ctm.site_coords:
+        problems.append((
+            ctm.construct_enlarged_dr(x, y), ctm.construct_enlarged_ur(x, y), ctm.construct_enlarged_ul(x, y),
+            ctm.construct_enlarged_dl(x, y), e.get_C2(x + 2, y + 1),
+        ))
+    kw = {"block_chi_ratio": 1.1, "ncv_ratio": 2.0, "rtol": 1e-11, "degen_ratio": 0.9999}
+    batched = construct_projectors_batch(problems, 20, **kw)
+    for prob, (Pb, Ptb) in zip(problems, batched, strict=True):
+        P, Pt = construct_projectors(*prob[:4], 20, prob[4], **kw)
+        assert (P.block_irreps == Pb.block_irreps).all()
+        assert [b.shape for b in P.blocks] == [b.shape for b in Pb.blocks]
+        ref = (Pt @ P.transpose()).toarray()
+        got = (Ptb @ Pb.transpose()).toarray()
+        assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())

@@ -151,3 +151,29 @@ def test_norm_trace_copy_fill(gpu):
     y2 = _ops.empty(1001)
     _ops.fill_normal_(y2, 5)
     assert np.array_equal(yy, _ops.download(y2))
+
+
+@pytest.mark.parametrize("sizes", [GEMM_SIZES_SMALL, GEMM_SIZES_LARGE], ids=["tile32", "tile64"])
+def test_mfma_gemm_per_group_operand_bases(gpu, sizes):
+    """fs_grouped_gemm_f64_multi: every group reads and writes its own buffers
+    (blocks of different tensors in one launch); 40 groups cover two dispatches."""
+    rng = np.random.default_rng(3)
+    shapes = (list(sizes) * 14)[:40]
+    As, Bs, Cs, groups, refs = [], [], [], [], []
+    for i, (m, n, k) in enumerate(shapes):
+        a = rng.standard_normal((m, k))
+        b = rng.standard_normal((k, n))
+        pad = int(rng.integers(0, 5))  # operands at nonzero offsets inside their buffers
+        As.append(_dev(np.concatenate((np.zeros(pad), a.T.reshape(-1) if i % 2 else a.reshape(-1)))))
+        Bs.append(_dev(np.concatenate((np.zeros(pad), b.reshape(-1)))))
+        Cs.append(_ops.upload(np.full(pad + m * n, 7.0)))
+        ars, acs = (1, m) if i % 2 else (k, 1)
+        groups.append((m, n, k, pad, ars, acs, pad, n, 1, pad, n))
+        refs.append((pad, a @ b))
+    _ops.grouped_gemm(groups, As, Bs, Cs, alpha=2.0, beta=0.5)
+    for (pad, ref), C, (m, n, _) in zip(refs, Cs, shapes, strict=True):
+        c = _ops.download(C)
+        assert np.all(c[:pad] == 7.0)
+        expect = 2.0 * ref + 3.5
+        err = float(np.abs(c[pad:].reshape(m, n) - expect).max() / max(1.0, np.abs(expect).max()))
+        assert err <= 1e-12
