@@ -14,6 +14,7 @@
 // lie_group_symmetric_tensor.py:96-265 (transpose-reshape + small GEMM +
 // slice) and the per-elementary-block projections of lie:856-1109.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -44,6 +45,96 @@ struct TileDev {
   int32_t task, e0;
 };
 
+// Decode the external multi-index of element e into its row/column offsets on
+// the input and output sides (mixed radix, last digit fastest).
+__device__ inline void decode_external(const DigitDev *__restrict__ digits, const TaskDev &tk,
+                                       int32_t e, int64_t &er_in, int64_t &ec_in,
+                                       int64_t &er_out, int64_t &ec_out) {
+  er_in = ec_in = er_out = ec_out = 0;
+  int32_t rem = e;
+  for (int k = tk.ndig - 1; k >= 0; --k) {
+    const DigitDev d = digits[tk.dig0 + k];
+    const int32_t q = rem / d.radix;
+    const int32_t r = rem - q * d.radix;
+    rem = q;
+    er_in += static_cast<int64_t>(r) * d.in_r;
+    ec_in += static_cast<int64_t>(r) * d.in_c;
+    er_out += static_cast<int64_t>(r) * d.out_r;
+    ec_out += static_cast<int64_t>(r) * d.out_c;
+  }
+}
+
+// Register form: each lane keeps its whole input column (n_in <= kRegIn) in
+// registers, loaded back to back so the gathers overlap; every output row is
+// then one wave-uniform dot product. Tasks with more input rows stream their
+// inputs from the cache once per chunk of kOutChunk outputs. No LDS, so the
+// occupancy is set by registers only.
+constexpr int kRegIn = 32;
+
+__global__ void __launch_bounds__(kLanes)
+lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restrict__ tasks,
+                         const RowDev *__restrict__ rows, const DigitDev *__restrict__ digits,
+                         const double *__restrict__ w, const double *__restrict__ src,
+                         double *__restrict__ dst) {
+  const TileDev tile = tiles[blockIdx.x];
+  const TaskDev tk = tasks[tile.task];
+  const int lane = threadIdx.x;
+  const int32_t e = tile.e0 + lane;
+  const bool active = e < tk.e_size;
+  int64_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
+  if (active) decode_external(digits, tk, e, er_in, ec_in, er_out, ec_out);
+  const RowDev *in_rows = rows + tk.row0;
+  const RowDev *out_rows = in_rows + tk.n_in;
+
+  if (tk.n_in <= kRegIn) {
+    double x[kRegIn];
+#pragma unroll
+    for (int i = 0; i < kRegIn; ++i) {
+      x[i] = 0.0;
+      if (i < tk.n_in) {
+        const RowDev r = in_rows[i];
+        if (active && r.base >= 0) x[i] = src[r.base + er_in * r.rs + ec_in * r.cs];
+      }
+    }
+    for (int o = 0; o < tk.n_out; ++o) {
+      const double *wo = w + tk.w_off + o;
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < kRegIn; ++i)
+        if (i < tk.n_in) acc = fma(wo[static_cast<int64_t>(i) * tk.n_out_pad], x[i], acc);
+      if (active) {
+        const RowDev ro = out_rows[o];
+        dst[ro.base + er_out * ro.rs + ec_out * ro.cs] = acc;
+      }
+    }
+    return;
+  }
+
+  constexpr int kChunk = 2 * kOutChunk;
+  for (int o0 = 0; o0 < tk.n_out; o0 += kChunk) {
+    double acc[kChunk];
+#pragma unroll
+    for (int r = 0; r < kChunk; ++r) acc[r] = 0.0;
+    const double *wt = w + tk.w_off + o0;
+    const int nr = min(kChunk, tk.n_out_pad - o0);  // W rows are padded to a multiple of 8
+    for (int i = 0; i < tk.n_in; ++i) {
+      const RowDev r = in_rows[i];
+      const double xv = (active && r.base >= 0) ? src[r.base + er_in * r.rs + ec_in * r.cs] : 0.0;
+      const double *wi = wt + static_cast<int64_t>(i) * tk.n_out_pad;
+#pragma unroll
+      for (int q = 0; q < kChunk; ++q)
+        if (q < nr) acc[q] = fma(wi[q], xv, acc[q]);
+    }
+    if (active) {
+      const int nw = min(kChunk, tk.n_out - o0);
+      for (int q = 0; q < nw; ++q) {
+        const RowDev ro = out_rows[o0 + q];
+        dst[ro.base + er_out * ro.rs + ec_out * ro.cs] = acc[q];
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kLanes)
 lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restrict__ tasks,
                      const RowDev *__restrict__ rows, const DigitDev *__restrict__ digits,
@@ -57,19 +148,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
   const bool active = e < tk.e_size;
 
   int64_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
-  if (active) {
-    int32_t rem = e;
-    for (int k = tk.ndig - 1; k >= 0; --k) {
-      const DigitDev d = digits[tk.dig0 + k];
-      const int32_t q = rem / d.radix;
-      const int32_t r = rem - q * d.radix;
-      rem = q;
-      er_in += static_cast<int64_t>(r) * d.in_r;
-      ec_in += static_cast<int64_t>(r) * d.in_c;
-      er_out += static_cast<int64_t>(r) * d.out_r;
-      ec_out += static_cast<int64_t>(r) * d.out_c;
-    }
-  }
+  if (active) decode_external(digits, tk, e, er_in, ec_in, er_out, ec_out);
 
   const RowDev *in_rows = rows + tk.row0;
   const RowDev *out_rows = in_rows + tk.n_in;
@@ -313,11 +392,22 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
     FS_HIP_CHECK(hipGetDevice(&dev));
     FS_CHECK_ARG(dev == plan->device, "lie plan was built on device %d, current device is %d",
                  plan->device, dev);
-    const size_t lds = static_cast<size_t>(plan->lds_rows) * kLanes * sizeof(double);
-    hipLaunchKernelGGL(lie_transform_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
-                       dim3(kLanes), lds, fs::as_stream(stream), plan->tiles, plan->tasks,
-                       plan->rows, plan->digits, plan->w, src, dst, plan->lds_rows);
-    FS_LAUNCH_CHECK("lie_transform_kernel");
+    static const int mode = [] {
+      const char *e = std::getenv("FS_LIE_KERNEL");  // tuning only: 1 = LDS-staged form
+      return e ? std::atoi(e) : 0;
+    }();
+    if (mode == 1) {
+      const size_t lds = static_cast<size_t>(plan->lds_rows) * kLanes * sizeof(double);
+      hipLaunchKernelGGL(lie_transform_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
+                         dim3(kLanes), lds, fs::as_stream(stream), plan->tiles, plan->tasks,
+                         plan->rows, plan->digits, plan->w, src, dst, plan->lds_rows);
+      FS_LAUNCH_CHECK("lie_transform_kernel");
+    } else {
+      hipLaunchKernelGGL(lie_transform_reg_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
+                         dim3(kLanes), 0, fs::as_stream(stream), plan->tiles, plan->tasks,
+                         plan->rows, plan->digits, plan->w, src, dst);
+      FS_LAUNCH_CHECK("lie_transform_reg_kernel");
+    }
     return FS_OK;
   });
 }
