@@ -347,6 +347,50 @@ def run_ctmrg(args):
     return res
 
 
+def run_serve(args):
+    """Serving workload: Poisson arrivals (mean 50 ms) of SU(2) Heisenberg
+    thermal CTMRG requests, beta uniform in [0.5, 4], chi in {32, 64, 128, 192,
+    256}, D in {6, 7, 8}, each solved by simple update + n CTMRG iterations on
+    one of `slots` concurrent slots (worker processes with their own HIP
+    context, or threads with their own stream; admission by HBM footprint).
+    value = aggregate CTMRG iterations/s over the whole run. In process mode
+    this process never touches the GPU itself."""
+    from frostspin_amd import serving
+
+    reqs = serving.poisson_requests(args.requests, n_iter=args.serve_iters, seed=args.seed)
+    with serving.ServingEngine(args.slots, mode=args.serve_mode) as engine:
+        # one small request per slot loads the library and warms each slot
+        engine.run([serving.Request(rid=-1 - i, beta=0.5, chi=32, D=6, n_iter=2) for i in range(args.slots)])
+        results, summary = engine.run(reqs)
+    return {
+        "metric": "CTMRG serving: aggregate iters/s, SU(2) thermal requests (continuous batching); p50/p99 latency",
+        "value": summary["ctmrg_iters_per_s"],
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 1,
+        "ms_per_step": 1000.0 * summary["wall_s"] / len(reqs),
+        "p50_ms": 1000.0 * summary["latency_p50_s"],
+        "p99_ms": 1000.0 * summary["latency_p99_s"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (simple-update thermal states, seeded Poisson arrivals)",
+        "config": {
+            "workload": f"{len(reqs)} SU(2) Heisenberg CTMRG requests, Poisson mean 50 ms, beta U[0.5,4], "
+            f"chi in {{32,64,128,192,256}}, D in {{6,7,8}}, {args.serve_iters} CTMRG iterations each, "
+            f"{args.slots} slots ({args.serve_mode}s)",
+            "slots": args.slots,
+            "mode": args.serve_mode,
+            "requests_per_s": summary["requests_per_s"],
+            "queue_p50_ms": 1000.0 * summary["queue_p50_s"],
+            "wall_s": summary["wall_s"],
+            "energies": [round(r.energy, 12) for r in results[:8]],
+        },
+    }
+
+
 def _one_more(ctm):
     ctm.iterate()
     return ctm
@@ -357,7 +401,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--workload", default="ctmrg", choices=["contract_permute", "ctmrg"])
+    ap.add_argument("--workload", default="ctmrg", choices=["contract_permute", "ctmrg", "serve"])
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
+    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent request slots")
+    ap.add_argument("--serve-iters", type=int, default=10, help="serve: CTMRG iterations per request")
+    ap.add_argument("--seed", type=int, default=0, help="serve: request stream seed")
+    ap.add_argument("--serve-mode", default="process", choices=["process", "thread"],
+                    help="serve: slots as worker processes or as threads with their own stream")
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--D", type=int, default=8)
     ap.add_argument("--beta", type=float, default=4.0)
@@ -374,18 +424,24 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
-    import torch
-
-    if not torch.cuda.is_available():
-        print("bench.py needs a HIP GPU", file=sys.stderr)
-        sys.exit(1)
     from frostspin_amd.symmetric_tensor.tools import config as fs_config
 
     # the timed path runs like the reference under python -O (README.md:46):
     # no debug norm/structure checks (each costs a device reduction + sync)
     fs_config["debug_checks"] = args.debug_checks
+    if args.workload == "serve" and args.serve_mode == "process":
+        # worker processes start before this process initializes the GPU
+        print(json.dumps(run_serve(args)))
+        return
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP GPU", file=sys.stderr)
+        sys.exit(1)
     if args.workload == "ctmrg":
         res = run_ctmrg(args)
+    elif args.workload == "serve":
+        res = run_serve(args)
     else:
         res = run_contract_permute(args)
     print(json.dumps(res))

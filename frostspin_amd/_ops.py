@@ -6,6 +6,7 @@ any other device are rejected: there is no host fallback.
 """
 
 import ctypes
+import threading
 from collections import OrderedDict
 
 import numpy as np
@@ -17,16 +18,28 @@ from ._lib import check, lib
 DTYPE = torch.float64
 
 
+_HIP_AVAILABLE = None
+_DEVICES = {}
+
+
 def device():
     """Device that holds all block buffers (the current HIP device)."""
-    if not torch.cuda.is_available():
+    global _HIP_AVAILABLE
+    if _HIP_AVAILABLE is None:
+        _HIP_AVAILABLE = torch.cuda.is_available()
+    if not _HIP_AVAILABLE:
         msg = "frostspin_amd device operations need a HIP GPU (torch.cuda is unavailable)"
         raise RuntimeError(msg)
-    return torch.device("cuda", torch.cuda.current_device())
+    idx = torch._C._cuda_getDevice()
+    dev = _DEVICES.get(idx)
+    if dev is None:
+        dev = _DEVICES[idx] = torch.device("cuda", idx)
+    return dev
 
 
 def _stream():
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """The calling thread's current stream on the current device (raw handle)."""
+    return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))
 
 
 class KernelTimer:
@@ -183,6 +196,7 @@ class PermutePlan:
 
 
 _PLAN_CACHE = OrderedDict()
+_PLAN_LOCK = threading.Lock()  # host threads of concurrent requests share the cache
 _PLAN_CACHE_SIZE = 2048
 
 
@@ -220,10 +234,11 @@ def abelian_permute_plan(
         bool(old_dense),
         bool(new_dense),
     )
-    plan = _PLAN_CACHE.get(key)
-    if plan is not None:
-        _PLAN_CACHE.move_to_end(key)
-        return plan
+    with _PLAN_LOCK:
+        plan = _PLAN_CACHE.get(key)
+        if plan is not None:
+            _PLAN_CACHE.move_to_end(key)
+            return plan
     ptrs = (_lib.c_i8p * len(charges))(*[_lib.i8p(c) for c in charges])
     handle = ctypes.c_void_p()
     check(
@@ -245,9 +260,10 @@ def abelian_permute_plan(
         )
     )
     plan = PermutePlan(handle.value)
-    _PLAN_CACHE[key] = plan
-    if len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
-        _PLAN_CACHE.popitem(last=False)
+    with _PLAN_LOCK:
+        _PLAN_CACHE[key] = plan
+        if len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+            _PLAN_CACHE.popitem(last=False)
     return plan
 
 
@@ -332,13 +348,12 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
             for a, b, c, gs in by.values():
                 grouped_gemm(gs, a, b, c, alpha=alpha, beta=beta, split=True)
             return
-    arr = (_lib.GemmGroup * n)(*[_lib.GemmGroup(*map(int, g)) for g in groups])
-    flop = 2.0 * sum(float(g[0]) * float(g[1]) * float(g[2]) for g in groups)
+    garr = np.array(groups, dtype=np.int64).reshape(n, 11)  # fs_gemm_group layout: 11 int64
+    arr = garr.ctypes.data_as(ctypes.POINTER(_lib.GemmGroup))
+    flop = 2.0 * float((garr[:, 0].astype(np.float64) * garr[:, 1] * garr[:, 2]).sum())
     # the library launches one kernel per 32 groups that hold a non-empty group
-    disp = sum(
-        1 for c in range(0, n, GEMM_GROUPS_PER_DISPATCH)
-        if any(g[0] > 0 and g[1] > 0 for g in groups[c : c + GEMM_GROUPS_PER_DISPATCH])
-    )
+    nonempty = (garr[:, 0] > 0) & (garr[:, 1] > 0)
+    disp = sum(1 for c in range(0, n, GEMM_GROUPS_PER_DISPATCH) if nonempty[c : c + GEMM_GROUPS_PER_DISPATCH].any())
     if multi:
         bases = [(_lib.c_vp * n)(*[_ptr(t) for t in lst]) for lst in (As, Bs, Cs)]
         with _timed("gemm", flop, disp):

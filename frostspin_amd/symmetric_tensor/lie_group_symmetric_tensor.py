@@ -19,6 +19,7 @@ lie:540-602) - and assembles per call the offset tables with vectorized
 numpy.
 """
 
+import threading
 from collections import OrderedDict
 from functools import reduce
 
@@ -90,7 +91,8 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
     _dense_structures = {}
     _side_cache = {}
     _plans = OrderedDict()
-    _PLAN_CACHE_SIZE = 64
+    _plans_lock = threading.Lock()  # requests served from several host threads share the cache
+    _PLAN_CACHE_SIZE = 512  # a few concurrent requests of different chi keep their plans
 
     # group hook ---------------------------------------------------------------
     @classmethod
@@ -281,10 +283,11 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
             self._block_irreps.tobytes(), self._boffs.tobytes(), self._bshapes.tobytes(),
             self._fortran,
         )
-        hit = cls._plans.get(key)
-        if hit is not None:
-            cls._plans.move_to_end(key)
-            return hit
+        with cls._plans_lock:
+            hit = cls._plans.get(key)
+            if hit is not None:
+                cls._plans.move_to_end(key)
+                return hit
         st = cls._permute_structure(irreps_per_leg, sig, nrr, axes, nrr_out)
         degens = [np.asarray(r[0], dtype=np.int64) for r in reps]
         ax = np.array(axes)
@@ -359,9 +362,10 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         )
         plan = _ops.LiePlan(tasks, rows, dig.reshape(-1, 5), st.W, self._data.numel(), int(sizes.sum()))
         hit = (plan, shapes, st.out_targets)
-        cls._plans[key] = hit
-        if len(cls._plans) > cls._PLAN_CACHE_SIZE:
-            cls._plans.popitem(last=False)
+        with cls._plans_lock:
+            cls._plans[key] = hit
+            if len(cls._plans) > cls._PLAN_CACHE_SIZE:
+                cls._plans.popitem(last=False)
         return hit
 
     def _permute_data(self, axes, nrr):

@@ -132,10 +132,10 @@ class SymmetricTensor:
         offs, _ = _packed_offsets(shapes)
         self._setup(row_reps, col_reps, data, shapes, offs, block_irreps, signature, False)
 
-    def _setup(self, row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran):
+    def _setup(self, row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran, shape=None):
         self._row_reps = tuple(row_reps)
         self._col_reps = tuple(col_reps)
-        self._shape = tuple(
+        self._shape = shape if shape is not None else tuple(
             self.representation_dimension(r) for r in self._row_reps + self._col_reps
         )
         self._ndim = len(self._shape)
@@ -155,9 +155,11 @@ class SymmetricTensor:
             assert self.check_blocks_fit_representations()
 
     @classmethod
-    def _build(cls, row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran=False):
+    def _build(cls, row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran=False, shape=None):
+        """Tensor around an existing buffer; shape (leg dimensions) may be passed
+        when the caller knows it, to skip recomputing it from the representations."""
         st = cls.__new__(cls)
-        st._setup(row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran)
+        st._setup(row_reps, col_reps, data, shapes, offs, block_irreps, signature, fortran, shape)
         return st
 
     def _with(self, data=None, *, fortran=None):
@@ -171,6 +173,7 @@ class SymmetricTensor:
             self._block_irreps,
             self._signature,
             self._fortran if fortran is None else fortran,
+            self._shape,
         )
 
     @classmethod
@@ -337,7 +340,7 @@ class SymmetricTensor:
         _ops.copy2d(d1, self._data, out)
         _ops.copy2d(d2, other._data, out)
         return type(self)._build(
-            self._row_reps, self._col_reps, out, shapes, offs, irreps, self._signature
+            self._row_reps, self._col_reps, out, shapes, offs, irreps, self._signature, shape=self._shape
         )
 
     def __sub__(self, other):
@@ -414,7 +417,8 @@ class SymmetricTensor:
         signature = np.empty((self._nrr + other._ndim - other._nrr,), dtype=bool)
         signature[: self._nrr] = self._signature[: self._nrr]
         signature[self._nrr :] = other._signature[other._nrr :]
-        return type(self)._build(self._row_reps, other._col_reps, out, shapes, offs, irreps, signature)
+        shape = self._shape[: self._nrr] + other._shape[other._nrr :]
+        return type(self)._build(self._row_reps, other._col_reps, out, shapes, offs, irreps, signature, shape=shape)
 
     # misc ------------------------------------------------------------------
     def match_representations(self, other):
@@ -465,7 +469,8 @@ class SymmetricTensor:
         else:
             data, shapes, block_irreps = self._permute_data(axes, nrr)
         offs, _ = _packed_offsets(shapes)
-        tp = type(self)._build(reps[:nrr], reps[nrr:], data, shapes, offs, block_irreps, signature)
+        shape = tuple(self._shape[a] for a in axes)
+        tp = type(self)._build(reps[:nrr], reps[nrr:], data, shapes, offs, block_irreps, signature, shape=shape)
         assert check_norm(self, tp)
         return tp
 
@@ -482,6 +487,7 @@ class SymmetricTensor:
             block_irreps,
             s,
             not self._fortran,
+            self._shape[self._nrr :] + self._shape[: self._nrr],
         )
 
     def dual(self):
@@ -505,6 +511,7 @@ class SymmetricTensor:
             self._block_irreps,
             s,
             not self._fortran,
+            self._shape[self._nrr :] + self._shape[: self._nrr],
         )
 
     def contiguous(self):
@@ -521,7 +528,7 @@ class SymmetricTensor:
         _ops.copy2d(descs, self._data, out)
         return type(self)._build(
             self._row_reps, self._col_reps, out, self._bshapes, offs, self._block_irreps,
-            self._signature,
+            self._signature, shape=self._shape,
         )
 
     # linear algebra ----------------------------------------------------------
@@ -610,7 +617,7 @@ class SymmetricTensor:
             _ops.copy2d(descs, self._data, out, col_w=vals)
         irreps = self._block_irreps[[b for b, _ in keep]]
         return type(self)._build(
-            self._row_reps, self._col_reps, out, shapes, offs, irreps, self._signature
+            self._row_reps, self._col_reps, out, shapes, offs, irreps, self._signature, shape=self._shape
         )
 
     # --- batched SVD core ---------------------------------------------------
