@@ -99,7 +99,8 @@ class _timed:
         if self.e0 is not None and KernelTimer.active is not None:
             e1 = torch.cuda.Event(enable_timing=True)
             e1.record()
-            KernelTimer.active.records.append((self.kind, self.work, self.e0, e1, self.dispatches))
+            disp = self.dispatches() if callable(self.dispatches) else self.dispatches
+            KernelTimer.active.records.append((self.kind, self.work, self.e0, e1, disp))
 
 
 def _ptr(t):
@@ -326,7 +327,6 @@ class LiePlan:
 # ---------------------------------------------------------------------------
 # grouped GEMM
 # ---------------------------------------------------------------------------
-GEMM_GROUPS_PER_DISPATCH = 32
 
 
 def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
@@ -352,15 +352,13 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
     arr = garr.ctypes.data_as(ctypes.POINTER(_lib.GemmGroup))
     flop = 2.0 * float((garr[:, 0].astype(np.float64) * garr[:, 1] * garr[:, 2]).sum())
     # the library launches one kernel per 32 groups that hold a non-empty group
-    nonempty = (garr[:, 0] > 0) & (garr[:, 1] > 0)
-    disp = sum(1 for c in range(0, n, GEMM_GROUPS_PER_DISPATCH) if nonempty[c : c + GEMM_GROUPS_PER_DISPATCH].any())
     if multi:
         bases = [(_lib.c_vp * n)(*[_ptr(t) for t in lst]) for lst in (As, Bs, Cs)]
-        with _timed("gemm", flop, disp):
+        with _timed("gemm", flop, lambda: lib.fs_gemm_last_dispatches()):
             check(lib.fs_grouped_gemm_f64_multi(n, arr, *bases, float(alpha), float(beta), _stream()))
         return
     fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
-    with _timed("gemm_split" if split else "gemm", flop, disp):
+    with _timed("gemm_split" if split else "gemm", flop, lambda: lib.fs_gemm_last_dispatches()):
         check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
 
 

@@ -31,10 +31,11 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 struct GroupDev {
   int32_t m, n, k, tiles_n;
   int32_t a_rs, a_cs, b_rs, b_cs;
-  int32_t c_ld, pad;
+  int32_t c_ld, kc;  // kc: K range of one split (multiple of the K panel)
   const double *a;
   const double *b;
   double *c;
+  int64_t ws_off;  // split-K: this group's m x n partial at ws + split * ws_stride + ws_off
 };
 
 struct Batch {
@@ -42,6 +43,9 @@ struct Batch {
   int32_t ngroups, xcd_remap;
   int32_t tile_prefix[kMaxGroups + 1];
   GroupDev g[kMaxGroups];
+  double *ws;  // null: no split-K, the epilogue writes C
+  int64_t ws_stride;
+  int64_t elem_prefix[kMaxGroups + 1];
 };
 
 // Map a linear tile id to its group (linear scan over <= 32 prefix entries).
@@ -66,14 +70,14 @@ __device__ inline int xcd_logical_tile(int bid, int nblocks) {
 // the operand is contiguous so every wavefront load is coalesced.
 template <int BM, int BK, int PER>
 __device__ inline void load_a(double (&r)[PER], const double *__restrict__ A, const GroupDev &G,
-                              int m0, int k0, bool kcontig) {
+                              int m0, int k0, int kend, bool kcontig) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int idx = threadIdx.x + kThreads * q;
     const int i = kcontig ? idx / BK : idx % BM;
     const int kk = kcontig ? idx % BK : idx / BM;
     const int gi = m0 + i, gk = k0 + kk;
-    r[q] = (gi < G.m && gk < G.k)
+    r[q] = (gi < G.m && gk < kend)
                ? A[static_cast<int64_t>(gi) * G.a_rs + static_cast<int64_t>(gk) * G.a_cs]
                : 0.0;
   }
@@ -81,14 +85,14 @@ __device__ inline void load_a(double (&r)[PER], const double *__restrict__ A, co
 
 template <int BN, int BK, int PER>
 __device__ inline void load_b(double (&r)[PER], const double *__restrict__ B, const GroupDev &G,
-                              int n0, int k0, bool ncontig) {
+                              int n0, int k0, int kend, bool ncontig) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int idx = threadIdx.x + kThreads * q;
     const int j = ncontig ? idx % BN : idx / BK;
     const int kk = ncontig ? idx / BN : idx % BK;
     const int gj = n0 + j, gk = k0 + kk;
-    r[q] = (gj < G.n && gk < G.k)
+    r[q] = (gj < G.n && gk < kend)
                ? B[static_cast<int64_t>(gk) * G.b_rs + static_cast<int64_t>(gj) * G.b_cs]
                : 0.0;
   }
@@ -165,16 +169,19 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
     }
   };
 
-  load_a<BM, BK, A_PER>(ra, A, G, m0, 0, a_kcontig);
-  load_b<BN, BK, B_PER>(rb, B, G, n0, 0, b_ncontig);
+  // split-K: blockIdx.y selects the K range [kb, ke) of this workgroup
+  const int kb = static_cast<int>(blockIdx.y) * G.kc;
+  const int ke = min(G.k, kb + G.kc);
+  load_a<BM, BK, A_PER>(ra, A, G, m0, kb, ke, a_kcontig);
+  load_b<BN, BK, B_PER>(rb, B, G, n0, kb, ke, b_ncontig);
   store(0);
   __syncthreads();
   int cur = 0;
-  for (int k0 = 0; k0 < G.k; k0 += BK) {
-    const bool next = k0 + BK < G.k;
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    const bool next = k0 + BK < ke;
     if (next) {
-      load_a<BM, BK, A_PER>(ra, A, G, m0, k0 + BK, a_kcontig);
-      load_b<BN, BK, B_PER>(rb, B, G, n0, k0 + BK, b_ncontig);
+      load_a<BM, BK, A_PER>(ra, A, G, m0, k0 + BK, ke, a_kcontig);
+      load_b<BN, BK, B_PER>(rb, B, G, n0, k0 + BK, ke, b_ncontig);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
@@ -199,6 +206,20 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
 
   // epilogue: for v_mfma_f64_16x16x4f64 lane l holds D[l/16 + 4*r][l%16] of
   // each 16x16 tile (rows interleaved by 4, measured on gfx950)
+  if (bt.ws) {  // split-K partial, reduced by gemm_splitk_reduce
+    double *__restrict__ P = bt.ws + static_cast<int64_t>(blockIdx.y) * bt.ws_stride + G.ws_off;
+#pragma unroll
+    for (int a = 0; a < TI; ++a)
+#pragma unroll
+      for (int b = 0; b < TJ; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * WM + a * 16 + (lane >> 4) + 4 * r;
+          const int col = n0 + wn * WN + b * 16 + (lane & 15);
+          if (row < G.m && col < G.n) P[static_cast<int64_t>(row) * G.n + col] = acc[a][b][r];
+        }
+    return;
+  }
   double *__restrict__ C = G.c;
   const double alpha = bt.alpha, beta = bt.beta;
 #pragma unroll
@@ -215,6 +236,76 @@ __global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
           *c = (beta == 0.0) ? v : v + beta * (*c);
         }
       }
+}
+
+// C = alpha * sum_s partial_s + beta * C, one thread per output element; the
+// splits are summed in a fixed order, so results do not depend on scheduling.
+constexpr int kReduceThreads = 256;
+
+__global__ void __launch_bounds__(kReduceThreads) gemm_splitk_reduce(Batch bt, int splits) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * kReduceThreads + threadIdx.x;
+  if (e >= bt.elem_prefix[bt.ngroups]) return;
+  int g = 0;
+  while (g + 1 < bt.ngroups && bt.elem_prefix[g + 1] <= e) ++g;
+  const GroupDev &G = bt.g[g];
+  const int64_t local = e - bt.elem_prefix[g];
+  const int64_t row = local / G.n, col = local - row * G.n;
+  const double *p = bt.ws + G.ws_off + local;
+  double sum = 0.0;
+  for (int s = 0; s < splits; ++s) sum += p[static_cast<int64_t>(s) * bt.ws_stride];
+  double *c = G.c + row * G.c_ld + col;
+  const double v = bt.alpha * sum;
+  *c = (bt.beta == 0.0) ? v : v + bt.beta * (*c);
+}
+
+// Skinny products (m <= 16 rows, k <= 128): C is a few rows over a long n
+// (a site tensor applied to a whole environment tensor). One thread per output
+// column keeps its m accumulators in registers and streams its column of B;
+// A is staged once per workgroup in LDS and read as broadcasts. HBM bound:
+// B is read once, C written once, both coalesced when their columns are
+// contiguous.
+constexpr int kSkinnyM = 16;
+constexpr int kSkinnyK = 128;
+constexpr int kSkinnyThreads = 256;
+
+struct SkinnyBatch {
+  double alpha, beta;
+  int32_t ngroups, pad;
+  int32_t block_prefix[kMaxGroups + 1];
+  GroupDev g[kMaxGroups];
+};
+
+__global__ void __launch_bounds__(kSkinnyThreads) gemm_skinny_kernel(SkinnyBatch bt) {
+  __shared__ double As[kSkinnyM * kSkinnyK];
+  int gi = 0;
+  while (gi + 1 < bt.ngroups && bt.block_prefix[gi + 1] <= static_cast<int>(blockIdx.x)) ++gi;
+  const GroupDev &G = bt.g[gi];
+  const int m = G.m, k = G.k;
+  for (int e = threadIdx.x; e < m * k; e += kSkinnyThreads) {
+    const int i = e / k, kk = e - i * k;
+    As[i * kSkinnyK + kk] = G.a[static_cast<int64_t>(i) * G.a_rs + static_cast<int64_t>(kk) * G.a_cs];
+  }
+  __syncthreads();
+  const int64_t j = static_cast<int64_t>(blockIdx.x - bt.block_prefix[gi]) * kSkinnyThreads + threadIdx.x;
+  if (j >= G.n) return;
+  double acc[kSkinnyM];
+#pragma unroll
+  for (int i = 0; i < kSkinnyM; ++i) acc[i] = 0.0;
+  const double *bcol = G.b + j * G.b_cs;
+  for (int kk = 0; kk < k; ++kk) {
+    const double b = bcol[static_cast<int64_t>(kk) * G.b_rs];
+#pragma unroll
+    for (int i = 0; i < kSkinnyM; ++i)
+      if (i < m) acc[i] = fma(As[i * kSkinnyK + kk], b, acc[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < kSkinnyM; ++i) {
+    if (i < m) {
+      double *c = G.c + static_cast<int64_t>(i) * G.c_ld + j;
+      const double v = bt.alpha * acc[i];
+      *c = (bt.beta == 0.0) ? v : v + bt.beta * (*c);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -363,6 +454,48 @@ constexpr int kBK = 16;
 // through L2. FS_GEMM_TILE=32/64/128 forces a size (tuning only).
 constexpr int kCUs = 256;
 
+// Split-K: a launch whose output tiles cannot fill the chip while its K is
+// long (projected corners: chi x chi outputs over chi D^2) runs `splits`
+// workgroups per tile over disjoint K ranges of at least kSplitMinK; partial
+// tiles go to a per-thread workspace and a second kernel sums them in split
+// order. FS_GEMM_SPLITK=0 disables it (tuning only).
+constexpr int kSplitMinK = 256;
+constexpr int kMaxSplits = 16;
+
+inline int pick_splits(int64_t tiles, int64_t kmax) {
+  static const bool off = [] {
+    const char *e = std::getenv("FS_GEMM_SPLITK");
+    return e && std::atoi(e) == 0;
+  }();
+  if (off || tiles * 2 > kCUs || kmax < 2 * kSplitMinK) return 1;
+  const int64_t by_cus = fs::ceil_div(kCUs, tiles);
+  const int64_t by_k = kmax / kSplitMinK;
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({by_cus, by_k, kMaxSplits})));
+}
+
+// Per-thread, per-device split-K workspace (grow-only): host threads that
+// serve requests on their own streams never share it.
+double *splitk_workspace(int64_t n_doubles) {
+  struct Ws {
+    int device = -1;
+    double *ptr = nullptr;
+    int64_t size = 0;
+  };
+  static thread_local Ws ws[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return nullptr;
+  Ws &w = ws[dev];
+  if (w.size < n_doubles) {
+    if (w.ptr) (void)hipFree(w.ptr);
+    w.ptr = nullptr;
+    w.size = 0;
+    const int64_t want = std::max<int64_t>(n_doubles, int64_t(1) << 20);
+    if (hipMalloc(&w.ptr, static_cast<size_t>(want) * sizeof(double)) != hipSuccess) return nullptr;
+    w.size = want;
+  }
+  return w.ptr;
+}
+
 inline int pick_tile(int ng, const fs_gemm_group *g) {
   static const int forced = [] {
     const char *e = std::getenv("FS_GEMM_TILE");
@@ -375,18 +508,83 @@ inline int pick_tile(int ng, const fs_gemm_group *g) {
   return tiles64 < kCUs ? 32 : 64;
 }
 
+thread_local int last_dispatches = 0;
+
+inline bool is_skinny(const fs_gemm_group &q) {
+  static const bool off = [] {
+    const char *e = std::getenv("FS_GEMM_SKINNY");
+    return e && std::atoi(e) == 0;
+  }();
+  return !off && q.m > 0 && q.m <= kSkinnyM && q.k <= kSkinnyK && q.n >= kSkinnyThreads;
+}
+
+// Fill one device group descriptor (operand bases already resolved).
+int fill_group(GroupDev &d, const fs_gemm_group &q, const double *a_base, const double *b_base,
+               double *c_base) {
+  FS_CHECK_ARG(q.m < (1ll << 31) && q.n < (1ll << 31) && q.k < (1ll << 31),
+               "grouped gemm: dimension too large");
+  FS_CHECK_ARG(a_base && b_base && c_base, "grouped gemm: null operand");
+  FS_CHECK_ARG(q.a_off >= 0 && q.b_off >= 0 && q.c_off >= 0, "grouped gemm: negative offset");
+  d.m = static_cast<int32_t>(q.m);
+  d.n = static_cast<int32_t>(q.n);
+  d.k = static_cast<int32_t>(q.k);
+  d.a_rs = static_cast<int32_t>(q.a_rs);
+  d.a_cs = static_cast<int32_t>(q.a_cs);
+  d.b_rs = static_cast<int32_t>(q.b_rs);
+  d.b_cs = static_cast<int32_t>(q.b_cs);
+  d.c_ld = static_cast<int32_t>(q.c_ld);
+  d.a = a_base + q.a_off;
+  d.b = b_base + q.b_off;
+  d.c = c_base + q.c_off;
+  d.kc = static_cast<int32_t>(fs::ceil_div(q.k, kBK) * kBK);
+  d.ws_off = 0;
+  return FS_OK;
+}
+
 // A, B, C: per-group operand bases (stride 1 for one shared base, 0 entries
-// otherwise unused); group offsets are relative to them.
+// otherwise unused); group offsets are relative to them. Skinny groups (few
+// rows over a long n) go to the streaming kernel, the others to the MFMA tile
+// kernel; each launch takes at most kMaxGroups groups.
 template <bool SPLIT>
 int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const *A,
                    const double *const *B, double *const *C, int base_stride, double alpha,
                    double beta, void *stream) {
   FS_CHECK_ARG(ngroups >= 0, "grouped gemm: negative group count");
   FS_CHECK_ARG(ngroups == 0 || groups, "grouped gemm: null groups");
-  int g0 = 0;
-  while (g0 < ngroups) {
-    const int g_end = std::min(ngroups, g0 + kMaxGroups);
-    const int tsz = SPLIT ? 64 : pick_tile(g_end - g0, groups + g0);
+  int dispatches = 0;
+  std::vector<int> tiled, skinny;
+  for (int i = 0; i < ngroups; ++i) {
+    const fs_gemm_group &q = groups[i];
+    FS_CHECK_ARG(q.m >= 0 && q.n >= 0 && q.k >= 0, "grouped gemm: negative dimension");
+    if (q.m == 0 || q.n == 0) continue;
+    (!SPLIT && is_skinny(q) ? skinny : tiled).push_back(i);
+  }
+  for (size_t c0 = 0; c0 < skinny.size(); c0 += kMaxGroups) {
+    SkinnyBatch sb;
+    std::memset(&sb, 0, sizeof(sb));
+    sb.alpha = alpha;
+    sb.beta = beta;
+    int ng = 0;
+    int64_t blocks = 0;
+    for (size_t t = c0; t < std::min(skinny.size(), c0 + kMaxGroups); ++t) {
+      const int i = skinny[t];
+      int st = fill_group(sb.g[ng], groups[i], A[i * base_stride], B[i * base_stride], C[i * base_stride]);
+      if (st != FS_OK) return st;
+      blocks += fs::ceil_div(groups[i].n, kSkinnyThreads);
+      FS_CHECK_ARG(blocks < (1ll << 31), "grouped gemm: too many blocks");
+      sb.block_prefix[++ng] = static_cast<int32_t>(blocks);
+    }
+    sb.ngroups = ng;
+    hipLaunchKernelGGL(gemm_skinny_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kSkinnyThreads), 0,
+                       fs::as_stream(stream), sb);
+    FS_LAUNCH_CHECK("gemm_skinny_kernel");
+    ++dispatches;
+  }
+  for (size_t c0 = 0; c0 < tiled.size(); c0 += kMaxGroups) {
+    const size_t c1 = std::min(tiled.size(), c0 + kMaxGroups);
+    std::vector<fs_gemm_group> sel;
+    for (size_t t = c0; t < c1; ++t) sel.push_back(groups[tiled[t]]);
+    const int tsz = SPLIT ? 64 : pick_tile(static_cast<int>(sel.size()), sel.data());
     Batch bt;
     std::memset(&bt, 0, sizeof(bt));
     bt.alpha = alpha;
@@ -394,39 +592,43 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     int ng = 0;
     int64_t tiles = 0;
     bt.tile_prefix[0] = 0;
-    for (; g0 < g_end; ++g0) {
-      const fs_gemm_group &q = groups[g0];
-      FS_CHECK_ARG(q.m >= 0 && q.n >= 0 && q.k >= 0, "grouped gemm: negative dimension");
-      if (q.m == 0 || q.n == 0) continue;
-      FS_CHECK_ARG(q.m < (1ll << 31) && q.n < (1ll << 31) && q.k < (1ll << 31),
-                   "grouped gemm: dimension too large");
-      const double *a_base = A[g0 * base_stride];
-      const double *b_base = B[g0 * base_stride];
-      double *c_base = C[g0 * base_stride];
-      FS_CHECK_ARG(a_base && b_base && c_base, "grouped gemm: null operand");
-      FS_CHECK_ARG(q.a_off >= 0 && q.b_off >= 0 && q.c_off >= 0, "grouped gemm: negative offset");
+    for (size_t t = c0; t < c1; ++t) {
+      const int i = tiled[t];
+      const fs_gemm_group &q = groups[i];
       GroupDev &d = bt.g[ng];
-      d.m = static_cast<int32_t>(q.m);
-      d.n = static_cast<int32_t>(q.n);
-      d.k = static_cast<int32_t>(q.k);
+      int st = fill_group(d, q, A[i * base_stride], B[i * base_stride], C[i * base_stride]);
+      if (st != FS_OK) return st;
       d.tiles_n = static_cast<int32_t>(fs::ceil_div(q.n, tsz));
-      d.a_rs = static_cast<int32_t>(q.a_rs);
-      d.a_cs = static_cast<int32_t>(q.a_cs);
-      d.b_rs = static_cast<int32_t>(q.b_rs);
-      d.b_cs = static_cast<int32_t>(q.b_cs);
-      d.c_ld = static_cast<int32_t>(q.c_ld);
-      d.a = a_base + q.a_off;
-      d.b = b_base + q.b_off;
-      d.c = c_base + q.c_off;
       tiles += fs::ceil_div(q.m, tsz) * d.tiles_n;
       FS_CHECK_ARG(tiles < (1ll << 31), "grouped gemm: too many tiles");
       bt.tile_prefix[ng + 1] = static_cast<int32_t>(tiles);
       ++ng;
     }
-    if (ng == 0) continue;
     bt.ngroups = ng;
     bt.xcd_remap = tiles >= 64 ? 1 : 0;
-    const dim3 grid(static_cast<unsigned>(tiles));
+    int splits = 1;
+    if (!SPLIT) {
+      int64_t kmax = 0, elems = 0;
+      for (int i = 0; i < ng; ++i) kmax = std::max<int64_t>(kmax, bt.g[i].k);
+      splits = pick_splits(tiles, kmax);
+      if (splits > 1) {
+        bt.elem_prefix[0] = 0;
+        for (int i = 0; i < ng; ++i) {
+          GroupDev &d = bt.g[i];
+          d.kc = static_cast<int32_t>(fs::ceil_div(fs::ceil_div(d.k, splits), kBK) * kBK);
+          d.ws_off = elems;
+          elems += static_cast<int64_t>(d.m) * d.n;
+          bt.elem_prefix[i + 1] = elems;
+        }
+        bt.ws = splitk_workspace(elems * splits);
+        if (bt.ws == nullptr) {
+          fs::set_error("grouped gemm: split-K workspace allocation failed");
+          return FS_ERR_HIP;
+        }
+        bt.ws_stride = elems;
+      }
+    }
+    const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(splits));
     if (SPLIT)
       hipLaunchKernelGGL((gemm_split_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,
                          fs::as_stream(stream), bt);
@@ -440,7 +642,16 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
       hipLaunchKernelGGL((gemm_f64_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,
                          fs::as_stream(stream), bt);
     FS_LAUNCH_CHECK(SPLIT ? "gemm_split_kernel" : "gemm_f64_kernel");
+    ++dispatches;
+    if (splits > 1) {
+      const int64_t elems = bt.elem_prefix[ng];
+      hipLaunchKernelGGL(gemm_splitk_reduce, dim3(static_cast<unsigned>(fs::ceil_div(elems, kReduceThreads))),
+                         dim3(kReduceThreads), 0, fs::as_stream(stream), bt, splits);
+      FS_LAUNCH_CHECK("gemm_splitk_reduce");
+      ++dispatches;
+    }
   }
+  last_dispatches = dispatches;
   return FS_OK;
 }
 
@@ -463,6 +674,8 @@ int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups, const do
     return launch_grouped<false>(ngroups, groups, A, B, C, 1, alpha, beta, stream);
   });
 }
+
+int fs_gemm_last_dispatches(void) { return last_dispatches; }
 
 int fs_grouped_gemm_f64_split(int ngroups, const fs_gemm_group *groups, const double *A,
                               const double *B, double *C, double alpha, double beta,

@@ -233,7 +233,12 @@ typedef struct fs_gemm_group {
   int64_t c_off, c_ld;
 } fs_gemm_group;
 
-/* FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 accumulation. */
+/*
+ * FP64 MFMA (v_mfma_f64_16x16x4f64), FP64 accumulation. Groups with at most
+ * 16 rows, K <= 128 and n >= 256 run on a streaming (HBM-bound) kernel;
+ * launches whose tiles cannot fill the chip while K is long are split over K
+ * (partials summed in a fixed order, so results are reproducible).
+ */
 int fs_grouped_gemm_f64(int ngroups, const fs_gemm_group *groups,
                         const double *A, const double *B, double *C,
                         double alpha, double beta, void *stream);
@@ -250,6 +255,13 @@ int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups,
                               const double *const *A, const double *const *B,
                               double *const *C, double alpha, double beta,
                               void *stream);
+
+/*
+ * Kernel dispatches issued by the calling thread's last grouped GEMM call
+ * (tile kernel per 32 groups, skinny-row kernel, split-K reduction); for
+ * per-dispatch timing. No counterpart in the reference (instrumentation).
+ */
+int fs_gemm_last_dispatches(void);
 
 /*
  * Compensated FP32 form: A and B are split into hi/lo FP32 parts on the fly

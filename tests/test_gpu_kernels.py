@@ -177,3 +177,57 @@ def test_mfma_gemm_per_group_operand_bases(gpu, sizes):
         expect = 2.0 * ref + 3.5
         err = float(np.abs(c[pad:].reshape(m, n) - expect).max() / max(1.0, np.abs(expect).max()))
         assert err <= 1e-12
+
+
+def _check_groups(shapes, layouts, rng, alpha=1.5, beta=0.25):
+    """Grouped GEMM of the given (m, n, k) shapes and (A col-major, B col-major)
+    layouts against numpy, with alpha / beta."""
+    A_parts, B_parts, groups, refs = [], [], [], []
+    ao = bo = co = 0
+    for (m, n, k), (a_f, b_f) in zip(shapes, layouts, strict=True):
+        a = rng.standard_normal((m, k))
+        b = rng.standard_normal((k, n))
+        A_parts.append((a.T if a_f else a).reshape(-1))
+        B_parts.append((b.T if b_f else b).reshape(-1))
+        ars, acs = (1, m) if a_f else (k, 1)
+        brs, bcs = (1, k) if b_f else (n, 1)
+        groups.append((m, n, k, ao, ars, acs, bo, brs, bcs, co, n))
+        refs.append((co, a @ b))
+        ao += m * k
+        bo += k * n
+        co += m * n
+    C0 = rng.standard_normal(co)
+    C = _ops.upload(C0)
+    _ops.grouped_gemm(groups, _dev(np.concatenate(A_parts)), _dev(np.concatenate(B_parts)), C, alpha=alpha, beta=beta)
+    c = _ops.download(C)
+    for (off, ref), (m, n, _) in zip(refs, shapes, strict=True):
+        expect = alpha * ref + beta * C0[off : off + m * n].reshape(m, n)
+        got = c[off : off + m * n].reshape(m, n)
+        err = float(np.abs(got - expect).max() / max(1.0, np.abs(expect).max()))
+        assert err <= 1e-12, f"shape {(m, n)} rel err {err:.2e}"
+
+
+def test_gemm_skinny_rows(gpu):
+    """m <= 16, k <= 128, n >= 256: the streaming kernel (both B layouts, mixed
+    with tile-kernel groups in one call)."""
+    rng = np.random.default_rng(11)
+    shapes = [(12, 5000, 36), (1, 300, 1), (16, 777, 128), (3, 2049, 7), (40, 300, 20), (12, 100, 36)]
+    layouts = [(False, False), (True, True), (False, True), (True, False), (False, False), (True, False)]
+    _check_groups(shapes, layouts, rng)
+
+
+def test_gemm_split_k(gpu):
+    """Few output tiles over a long K: split over K, partials reduced in order;
+    repeated calls give bit-identical results."""
+    rng = np.random.default_rng(12)
+    shapes = [(40, 50, 6000), (64, 30, 2500), (17, 70, 1025), (5, 5, 700)]
+    layouts = [(False, False), (True, False), (False, True), (True, True)]
+    _check_groups(shapes, layouts, rng)
+    a = _dev(rng.standard_normal(60 * 4000))
+    b = _dev(rng.standard_normal(4000 * 50))
+    outs = []
+    for _ in range(2):
+        C = _ops.empty(60 * 50)
+        _ops.grouped_gemm([(60, 50, 4000, 0, 4000, 1, 0, 50, 1, 0, 50)], a, b, C)
+        outs.append(_ops.download(C))
+    assert np.array_equal(outs[0], outs[1])
