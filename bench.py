@@ -35,6 +35,7 @@ METRIC = "CTMRG iters/s SU(2) χ=256 mixed vs plain FP64 GPU at 1e-8 obs err; p5
 PEAK_FP64_MFMA = 78.6e12
 PEAK_HBM = 8.0e12
 PROFILE_MARKER_CYCLES = 20_000_000
+COPY_BYTES = 1 << 30  # hbm_copy workload: 1 GiB read + 1 GiB written
 
 
 def leg_rep():
@@ -189,6 +190,51 @@ def run_contract_permute(args):
             f"vectorised scatter permute), BLAS threads = {cores}",
         }
     return res
+
+
+def run_hbm_copy(args):
+    """Reference copy: one copy2d_kernel launch moving COPY_BYTES of float64
+    (read + write), timed with HIP events; the byte count the PMC counters
+    FETCH_SIZE / WRITE_SIZE must report for it calibrates their scale."""
+    import torch
+
+    from frostspin_amd import _ops
+
+    n = COPY_BYTES // 8
+    src = _ops.upload(np.arange(n, dtype=np.float64))
+    dst = _ops.empty(n)
+    rows = 4096
+    desc = [(rows, n // rows, 0, n // rows, 1, 0, n // rows, 1, 0, 0, 0, 1.0, 0.0)]
+    for _ in range(args.warmup):
+        _ops.copy2d(desc, src, dst)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(args.steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _ops.copy2d(desc, src, dst)
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    assert float(_ops.download(dst[-1:])[0]) == float(n - 1)
+    t = float(np.median(ms))
+    return {
+        "metric": "HBM copy (copy2d_kernel), read + write GB/s",
+        "value": 2 * COPY_BYTES / (t * 1e-3) / 1e9,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (arange)",
+        "config": {"workload": f"copy of {COPY_BYTES} bytes ({n} float64) in one launch", "bytes_read": COPY_BYTES,
+                   "bytes_written": COPY_BYTES},
+    }
 
 
 def kernel_roofline(summary):
@@ -401,7 +447,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=6)
-    ap.add_argument("--workload", default="ctmrg", choices=["contract_permute", "ctmrg", "serve"])
+    ap.add_argument("--workload", default="ctmrg", choices=["contract_permute", "ctmrg", "serve", "hbm_copy"])
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--slots", type=int, default=4, help="serve: concurrent request slots")
     ap.add_argument("--serve-iters", type=int, default=10, help="serve: CTMRG iterations per request")
@@ -442,6 +488,8 @@ def main():
         res = run_ctmrg(args)
     elif args.workload == "serve":
         res = run_serve(args)
+    elif args.workload == "hbm_copy":
+        res = run_hbm_copy(args)
     else:
         res = run_contract_permute(args)
     print(json.dumps(res))

@@ -33,12 +33,17 @@ struct TaskDev {
   int32_t row0, ndig, dig0, pad;
 };
 
+// In-block offsets (er * rs + ec * cs) are validated below 2^31, so row and
+// column strides and the decoded external indices are 32-bit and every address
+// is one 64-bit multiply-add per stride.
 struct RowDev {
-  int64_t base, rs, cs;
+  int64_t base;
+  int32_t rs, cs;
 };
 
 struct DigitDev {
-  int32_t radix, in_r, in_c, out_r, out_c;
+  double inv;  // 1 / radix: division by the radix as a multiply plus one correction
+  int32_t radix, in_r, in_c, out_r, out_c, pad;
 };
 
 struct TileDev {
@@ -48,20 +53,32 @@ struct TileDev {
 // Decode the external multi-index of element e into its row/column offsets on
 // the input and output sides (mixed radix, last digit fastest).
 __device__ inline void decode_external(const DigitDev *__restrict__ digits, const TaskDev &tk,
-                                       int32_t e, int64_t &er_in, int64_t &ec_in,
-                                       int64_t &er_out, int64_t &ec_out) {
+                                       int32_t e, int32_t &er_in, int32_t &ec_in,
+                                       int32_t &er_out, int32_t &ec_out) {
   er_in = ec_in = er_out = ec_out = 0;
   int32_t rem = e;
   for (int k = tk.ndig - 1; k >= 0; --k) {
     const DigitDev d = digits[tk.dig0 + k];
-    const int32_t q = rem / d.radix;
-    const int32_t r = rem - q * d.radix;
+    // rem / radix through the reciprocal: exact up to one unit, corrected
+    int32_t q = static_cast<int32_t>(static_cast<double>(rem) * d.inv);
+    int32_t r = rem - q * d.radix;
+    if (r < 0) {
+      --q;
+      r += d.radix;
+    } else if (r >= d.radix) {
+      ++q;
+      r -= d.radix;
+    }
     rem = q;
-    er_in += static_cast<int64_t>(r) * d.in_r;
-    ec_in += static_cast<int64_t>(r) * d.in_c;
-    er_out += static_cast<int64_t>(r) * d.out_r;
-    ec_out += static_cast<int64_t>(r) * d.out_c;
+    er_in += r * d.in_r;
+    ec_in += r * d.in_c;
+    er_out += r * d.out_r;
+    ec_out += r * d.out_c;
   }
+}
+
+__device__ inline int64_t row_offset(const RowDev &r, int32_t er, int32_t ec) {
+  return r.base + static_cast<int64_t>(er) * r.rs + static_cast<int64_t>(ec) * r.cs;
 }
 
 // Register form: each lane keeps its whole input column (n_in <= kRegIn) in
@@ -81,7 +98,7 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
   const int lane = threadIdx.x;
   const int32_t e = tile.e0 + lane;
   const bool active = e < tk.e_size;
-  int64_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
+  int32_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
   if (active) decode_external(digits, tk, e, er_in, ec_in, er_out, ec_out);
   const RowDev *in_rows = rows + tk.row0;
   const RowDev *out_rows = in_rows + tk.n_in;
@@ -93,18 +110,31 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
       x[i] = 0.0;
       if (i < tk.n_in) {
         const RowDev r = in_rows[i];
-        if (active && r.base >= 0) x[i] = src[r.base + er_in * r.rs + ec_in * r.cs];
+        if (active && r.base >= 0) x[i] = src[row_offset(r, er_in, ec_in)];
       }
     }
-    for (int o = 0; o < tk.n_out; ++o) {
-      const double *wo = w + tk.w_off + o;
-      double acc = 0.0;
+    // outputs in chunks of 8: each input row i contributes W^T[i, o0 .. o0+7],
+    // a contiguous (wave-uniform) run of the transposed structural matrix
+    // (rows padded to a multiple of 8, so the run is always in bounds)
+    for (int o0 = 0; o0 < tk.n_out; o0 += kOutChunk) {
+      double acc[kOutChunk];
 #pragma unroll
-      for (int i = 0; i < kRegIn; ++i)
-        if (i < tk.n_in) acc = fma(wo[static_cast<int64_t>(i) * tk.n_out_pad], x[i], acc);
+      for (int q = 0; q < kOutChunk; ++q) acc[q] = 0.0;
+      const double *wt = w + tk.w_off + o0;
+#pragma unroll
+      for (int i = 0; i < kRegIn; ++i) {
+        if (i < tk.n_in) {
+          const double *wi = wt + static_cast<int64_t>(i) * tk.n_out_pad;
+#pragma unroll
+          for (int q = 0; q < kOutChunk; ++q) acc[q] = fma(wi[q], x[i], acc[q]);
+        }
+      }
       if (active) {
-        const RowDev ro = out_rows[o];
-        dst[ro.base + er_out * ro.rs + ec_out * ro.cs] = acc;
+        const int nw = min(kOutChunk, tk.n_out - o0);
+        for (int q = 0; q < nw; ++q) {
+          const RowDev ro = out_rows[o0 + q];
+          dst[row_offset(ro, er_out, ec_out)] = acc[q];
+        }
       }
     }
     return;
@@ -119,7 +149,7 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
     const int nr = min(kChunk, tk.n_out_pad - o0);  // W rows are padded to a multiple of 8
     for (int i = 0; i < tk.n_in; ++i) {
       const RowDev r = in_rows[i];
-      const double xv = (active && r.base >= 0) ? src[r.base + er_in * r.rs + ec_in * r.cs] : 0.0;
+      const double xv = (active && r.base >= 0) ? src[row_offset(r, er_in, ec_in)] : 0.0;
       const double *wi = wt + static_cast<int64_t>(i) * tk.n_out_pad;
 #pragma unroll
       for (int q = 0; q < kChunk; ++q)
@@ -129,7 +159,7 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
       const int nw = min(kChunk, tk.n_out - o0);
       for (int q = 0; q < nw; ++q) {
         const RowDev ro = out_rows[o0 + q];
-        dst[ro.base + er_out * ro.rs + ec_out * ro.cs] = acc[q];
+        dst[row_offset(ro, er_out, ec_out)] = acc[q];
       }
     }
   }
@@ -147,7 +177,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
   const int32_t e = tile.e0 + lane;
   const bool active = e < tk.e_size;
 
-  int64_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
+  int32_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
   if (active) decode_external(digits, tk, e, er_in, ec_in, er_out, ec_out);
 
   const RowDev *in_rows = rows + tk.row0;
@@ -157,7 +187,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
     for (int i = 0; i < tk.n_in; ++i) {
       const RowDev r = in_rows[i];
       double v = 0.0;
-      if (active && r.base >= 0) v = src[r.base + er_in * r.rs + ec_in * r.cs];
+      if (active && r.base >= 0) v = src[row_offset(r, er_in, ec_in)];
       xs[i * kLanes + lane] = v;
     }
   }
@@ -173,7 +203,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
         x = xs[i * kLanes + lane];
       } else {
         const RowDev r = in_rows[i];
-        x = (active && r.base >= 0) ? src[r.base + er_in * r.rs + ec_in * r.cs] : 0.0;
+        x = (active && r.base >= 0) ? src[row_offset(r, er_in, ec_in)] : 0.0;
       }
       const double *wi = wt + static_cast<int64_t>(i) * tk.n_out_pad;
 #pragma unroll
@@ -183,7 +213,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
       const int nr = min(kOutChunk, tk.n_out - o0);
       for (int r = 0; r < nr; ++r) {
         const RowDev ro = out_rows[o0 + r];
-        dst[ro.base + er_out * ro.rs + ec_out * ro.cs] = acc[r];
+        dst[row_offset(ro, er_out, ec_out)] = acc[r];
       }
     }
   }
@@ -236,11 +266,14 @@ int check_task(const fs_lie_task &t, int64_t nrows, const fs_lie_row *rows, int6
                  "lie plan: external index too large");
   }
   FS_CHECK_ARG(prod == t.e_size, "lie plan: digit radices do not multiply to the external size");
+  FS_CHECK_ARG(std::max(std::max(mr_in, mc_in), std::max(mr_out, mc_out)) < (int64_t(1) << 31),
+               "lie plan: external index beyond 32 bits");
   for (int64_t i = 0; i < t.n_in + t.n_out; ++i) {
     const fs_lie_row &r = rows[t.row0 + i];
     const bool input = i < t.n_in;
     if (input && r.base < 0) continue;  // absent input block
     FS_CHECK_ARG(r.base >= 0 && r.rs >= 0 && r.cs >= 0, "lie plan: bad row entry");
+    FS_CHECK_ARG(r.rs < (int64_t(1) << 31) && r.cs < (int64_t(1) << 31), "lie plan: stride beyond 32 bits");
     int64_t a = 0, b = 0, last = 0;
     const bool ovf = __builtin_mul_overflow(input ? mr_in : mr_out, r.rs, &a) ||
                      __builtin_mul_overflow(input ? mc_in : mc_out, r.cs, &b) ||
@@ -298,13 +331,14 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     }
     FS_CHECK_ARG(tiles.size() < (size_t(1) << 31), "lie plan: too many tiles");
     std::vector<RowDev> rdev(static_cast<size_t>(nrows));
-    for (int64_t i = 0; i < nrows; ++i) rdev[i] = {rows[i].base, rows[i].rs, rows[i].cs};
+    for (int64_t i = 0; i < nrows; ++i)
+      rdev[i] = {rows[i].base, static_cast<int32_t>(rows[i].rs), static_cast<int32_t>(rows[i].cs)};
     std::vector<DigitDev> ddev(static_cast<size_t>(ndigits));
     for (int64_t i = 0; i < ndigits; ++i) {
       const fs_lie_digit &d = digits[i];
-      ddev[i] = {static_cast<int32_t>(d.radix), static_cast<int32_t>(d.in_r),
-                 static_cast<int32_t>(d.in_c), static_cast<int32_t>(d.out_r),
-                 static_cast<int32_t>(d.out_c)};
+      ddev[i] = {1.0 / static_cast<double>(d.radix), static_cast<int32_t>(d.radix),
+                 static_cast<int32_t>(d.in_r), static_cast<int32_t>(d.in_c),
+                 static_cast<int32_t>(d.out_r), static_cast<int32_t>(d.out_c), 0};
     }
     // one device allocation, 256-byte aligned parts
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
