@@ -56,6 +56,7 @@ class Request:
     jperp: float = 1.0
     tau: float = 0.01
     arrival: float = 0.0  # seconds after the start of the run
+    record_iters: bool = False  # keep the completion time of every CTMRG iteration
 
 
 @dataclasses.dataclass
@@ -67,6 +68,7 @@ class Result:
     start: float
     finish: float
     slot: int
+    iter_times: list = dataclasses.field(default_factory=list)  # monotonic seconds
 
     @property
     def latency(self):
@@ -87,9 +89,11 @@ def footprint_bytes(req):
     return int(3 * 4 * side * side * 8 * frac)
 
 
-def solve(req):
+def solve(req, iter_times=None):
     """Simple update to beta, n_iter CTMRG iterations, bond energy (on the
-    current device stream)."""
+    current device stream). With a list in iter_times, the completion time
+    (time.monotonic, after a stream synchronization) of every iteration is
+    appended to it."""
     if req.symmetry == "SU2":
         h = models.heisenberg_bond("SU2")
     elif req.symmetry == "U1":
@@ -100,6 +104,9 @@ def solve(req):
     ctm = SequentialCTMRG.from_elementary_tensors("AB\nBA", tensors, req.chi, cutoff=1e-10, degen_ratio=0.99)
     for _ in range(req.n_iter):
         ctm.iterate()
+        if iter_times is not None:
+            torch.cuda.current_stream().synchronize()
+            iter_times.append(time.monotonic())
     return models.bond_energy(ctm, h), list(ctm.chi_values)
 
 
@@ -137,8 +144,9 @@ def _worker_main(slot, q_in, q_out, debug_checks):
             return
         start = time.monotonic()
         try:
-            energy, chis = solve(req)
-            q_out.put(("done", slot, req.rid, (energy, chis), start, time.monotonic()))
+            its = [] if req.record_iters else None
+            energy, chis = solve(req, its)
+            q_out.put(("done", slot, req.rid, (energy, chis, its or []), start, time.monotonic()))
         except Exception as err:  # noqa: BLE001 - reported to the engine
             q_out.put(("failed", slot, req.rid, repr(err), start, time.monotonic()))
 
@@ -241,13 +249,15 @@ class ServingEngine:
         start = self._now()
         try:
             stream = self._streams[slot]
+            its = [] if req.record_iters else None
             if stream is not None:
                 with torch.cuda.stream(stream):
-                    energy, chis = solve(req)
+                    energy, chis = solve(req, its)
                     stream.synchronize()
             else:
-                energy, chis = solve(req)
-            res = Result(req.rid, energy, chis, req.arrival, start, self._now(), slot)
+                energy, chis = solve(req, its)
+            offset = time.monotonic() - time.perf_counter() + self._t0
+            res = Result(req.rid, energy, chis, req.arrival, start, self._now(), slot, [t - offset for t in its or []])
         except Exception as err:  # noqa: BLE001 - reported by run()
             res = None
             with self._lock:
@@ -320,11 +330,25 @@ class ServingEngine:
             self._free_slots.append(slot)
             if kind != "done":
                 raise RuntimeError(f"request {rid} failed: {payload}")
-            energy, chis = payload
-            self._results[rid] = Result(rid, energy, chis, by_rid[rid].arrival, start - t0, finish - t0, slot)
+            energy, chis, its = payload
+            self._results[rid] = Result(rid, energy, chis, by_rid[rid].arrival, start - t0, finish - t0, slot,
+                                        [t - t0 for t in its])
         wall = time.monotonic() - t0
         results = [self._results[r.rid] for r in requests]
         return results, _summary(requests, results, wall, self.n_slots)
+
+
+def overlapped_rate(results, skip):
+    """Aggregate iterations/s over the interval in which every request was past
+    its first `skip` iterations and not yet finished: iterations completed by
+    all requests inside [max of their start times, min of their end times]
+    divided by its length (steady-state throughput of concurrent solves)."""
+    t_a = max(r.iter_times[skip - 1] if skip else r.start for r in results)
+    t_b = min(r.iter_times[-1] for r in results)
+    if t_b <= t_a:
+        raise ValueError("the requests' timed phases do not overlap")
+    n = sum(sum(1 for t in r.iter_times[skip:] if t_a < t <= t_b) for r in results)
+    return n / (t_b - t_a), t_b - t_a
 
 
 def poisson_requests(n, *, mean_interarrival=0.05, betas=(0.5, 4.0), chis=(32, 64, 128, 192, 256), Ds=(6, 7, 8),

@@ -87,3 +87,17 @@ def test_process_slots_match_in_process_results(gpu):
     assert summary["requests"] == len(reqs)
     for r in mixed:
         assert abs(r.energy - alone[r.rid]) <= 1e-12 * max(1.0, abs(alone[r.rid]))
+
+
+def test_overlapped_rate_counts_common_window():
+    """Aggregate rate over the interval where all requests are past warm-up."""
+    mk = lambda rid, ts: serving.Result(rid, 0.0, [], 0.0, 0.0, ts[-1], 0, list(ts))  # noqa: E731
+    a = mk(0, [1.0, 2.0, 3.0, 4.0, 5.0])
+    b = mk(1, [1.5, 2.5, 3.5, 4.5])
+    rate, window = serving.overlapped_rate([a, b], 1)
+    # both past their first iteration from 1.5 on, b stops at 4.5: window (1.5, 4.5]
+    # holds a's 2.0, 3.0, 4.0 and b's 2.5, 3.5, 4.5
+    assert window == pytest.approx(3.0)
+    assert rate == pytest.approx(6 / 3.0)
+    with pytest.raises(ValueError):
+        serving.overlapped_rate([mk(0, [1.0, 2.0]), mk(1, [3.0, 4.0])], 1)
