@@ -300,6 +300,9 @@ def run_ctmrg(args):
     su, ctm = models.thermal_ctmrg(h, args.D, args.beta, args.chi)
     torch.cuda.synchronize()
     t_su = time.perf_counter() - t0
+    precision = None
+    if args.precision == "split":
+        precision = _split_precision_reference(ctm, h, args)
     for _ in range(args.warmup):
         ctm.iterate()
     torch.cuda.synchronize()
@@ -375,6 +378,10 @@ def run_ctmrg(args):
             "frac": perm["rate"] / PEAK_HBM,
         },
     }
+    if precision is not None:
+        res["dtype"] = "f32 split (hi/lo) contractions, f64 projectors"
+        res["precision"] = precision | {"energy_split": energy,
+                                        "rel_err": abs(energy - precision["energy_fp64"]) / abs(precision["energy_fp64"])}
     if args.cpu_baseline:
         cores = len(os.sched_getaffinity(0))
         energy_gpu = res["config"]["energy_per_bond"]
@@ -482,6 +489,34 @@ def run_batch_ctmrg(args):
     }
 
 
+def _split_precision_reference(ctm, h, args):
+    """FP64 reference for the split-precision run: the same starting
+    environment (npz round trip) iterated warmup + steps + 1 times in FP64;
+    then switches the contractions to the FP32-split GEMM for the timed run."""
+    import tempfile
+
+    import torch
+
+    from frostspin_amd.ctmrg import SequentialCTMRG
+    from frostspin_amd.ctmrg import models
+    from frostspin_amd.symmetric_tensor.symmetric_tensor import set_gemm_precision
+
+    with tempfile.TemporaryDirectory(prefix="fs_bench_") as tmp:
+        path = os.path.join(tmp, "env.npz")
+        ctm.save_to_file(path)
+        ref = SequentialCTMRG.load_from_file(path)
+    t0 = time.perf_counter()
+    for _ in range(args.warmup + args.steps + 1):
+        ref.iterate()
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    set_gemm_precision("split")
+    return {
+        "energy_fp64": models.bond_energy(ref, h),
+        "fp64_iters_per_s": (args.warmup + args.steps + 1) / t_ref,
+    }
+
+
 def _one_more(ctm):
     ctm.iterate()
     return ctm
@@ -504,6 +539,8 @@ def main():
     ap.add_argument("--beta", type=float, default=4.0)
     ap.add_argument("--symmetry", default="SU2", choices=["SU2", "U1"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "split"],
+                    help="ctmrg: contraction GEMMs in FP64 or FP32-split (projectors stay FP64)")
     ap.add_argument("--debug-checks", action="store_true")
     ap.add_argument("--profile-markers", action="store_true",
                     help="bracket the timed iterations with two long spin kernels (kernel-trace cuts)")

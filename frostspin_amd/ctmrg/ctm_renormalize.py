@@ -14,7 +14,6 @@ import numpy as np
 from .. import _ops
 from ..misc_tools import block_linalg as bl
 from ..misc_tools.svd_tools import find_block_cuts
-from ..symmetric_tensor.symmetric_tensor import _precision_split
 
 # Blocks up to SMALL_DENSE (smallest side) always use the exact Jacobi SVD; above
 # DENSE_SVD_LIMIT the randomized truncated SVD is the only option.
@@ -83,7 +82,8 @@ def construct_projectors_batch(
         for bi in range(len(shared)):
             sectors.append((pi, shared[bi], int(block_chi[bi]), [int(x[bi]) for x in ind]))
 
-    split = _precision_split()
+    # projector construction stays FP64 whatever the contraction precision
+    split = False
     # sectors are ordered by problem: gather each corner's blocks problem by problem
     cmats = [
         bl.Mats.concat(
