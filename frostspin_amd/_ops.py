@@ -37,6 +37,24 @@ def device():
     return dev
 
 
+_SIDE = threading.local()
+
+
+def side_stream():
+    """A second stream of the calling thread on the current device (default
+    priority), for work independent of the current stream; None off-GPU."""
+    if device().type != "cuda":
+        return None
+    idx = torch._C._cuda_getDevice()
+    streams = getattr(_SIDE, "streams", None)
+    if streams is None:
+        streams = _SIDE.streams = {}
+    st = streams.get(idx)
+    if st is None:
+        st = streams[idx] = torch.cuda.Stream(device=idx)
+    return st
+
+
 def _stream():
     """The calling thread's current stream on the current device (raw handle)."""
     return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice()))

@@ -10,6 +10,7 @@ s^-1/2 scaling fused into the slicing copy.
 """
 
 import numpy as np
+import torch
 
 from .. import _ops
 from ..misc_tools import block_linalg as bl
@@ -115,15 +116,33 @@ def construct_projectors_batch(
     dense_ids = [i for i in range(n) if dense[i]]
     rand_ids = [i for i in range(n) if not dense[i]]
     U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
+    dense_done = None
     if dense_ids:
-        U, S, s_off, s_list, VT = bl.svd(M.select(dense_ids))
-        for j, i in enumerate(dense_ids):
-            U_of[i] = (U, j)
-            VT_of[i] = (VT, j)
-            s_of[i] = s_list[j]
+        # the exact SVDs of the small sectors are independent of the randomized
+        # chain of the large ones: on a GPU they run on a side stream meanwhile
+        side = _ops.side_stream() if rand_ids else None
+        if side is not None:
+            main = torch.cuda.current_stream()
+            side.wait_stream(main)
+            with torch.cuda.stream(side):
+                U, S, s_off, VT, finish = bl.svd_async(M.select(dense_ids))
+            for t in (U.buf, S, VT.buf):
+                t.record_stream(main)
+        else:
+            U, S, s_off, VT, finish = bl.svd_async(M.select(dense_ids))
+        dense_done = (U, VT, finish, side)
     if rand_ids:
         U, S, s_off, s_list, VT = bl.rsvd(M.select(rand_ids), k_need[rand_ids], seed=rsvd_seed)
         for j, i in enumerate(rand_ids):
+            U_of[i] = (U, j)
+            VT_of[i] = (VT, j)
+            s_of[i] = s_list[j]
+    if dense_done is not None:
+        U, VT, finish, side = dense_done
+        if side is not None:
+            torch.cuda.current_stream().wait_stream(side)
+        s_list = finish()
+        for j, i in enumerate(dense_ids):
             U_of[i] = (U, j)
             VT_of[i] = (VT, j)
             s_of[i] = s_list[j]

@@ -124,6 +124,14 @@ def scale_slice(A, nrows, ncols, *, row_w=None, col_w=None, w_off=None):
 def svd(A, *, max_sweeps=60):
     """Thin SVD of every A_i (batched one-sided Jacobi). Returns U (packed m x q),
     S (device, packed), host singular values list, VT (packed q x n)."""
+    U, S, s_off, VT, finish = svd_async(A, max_sweeps=max_sweeps)
+    return U, S, s_off, finish(), VT
+
+
+def svd_async(A, *, max_sweeps=60):
+    """svd() without the read-back: returns U, S, s_off, VT and a callable that
+    downloads the singular values (host list per matrix) when called, so the
+    caller can launch other work before waiting for them."""
     q = np.minimum(A.rows, A.cols)
     if q.size and q.max() > _ops.SVD_MAX_MIN_DIM:
         msg = "dense device SVD limited to min(m, n) <= 1024; use rsvd"
@@ -144,9 +152,12 @@ def svd(A, *, max_sweeps=60):
         for i in range(len(A))
     ]
     _ops.batched_svd(descs, A.buf, U.buf, S, VT.buf, work, max_sweeps=max_sweeps)
-    s_host = _ops.download(S) if S.numel() else np.zeros((0,))
-    s_list = [s_host[s_off[i] : s_off[i] + q[i]] for i in range(len(A))]
-    return U, S, s_off, s_list, VT
+
+    def finish():
+        s_host = _ops.download(S) if S.numel() else np.zeros((0,))
+        return [s_host[s_off[i] : s_off[i] + q[i]] for i in range(len(A))]
+
+    return U, S, s_off, VT, finish
 
 
 def qr(A, *, want_r=True):
