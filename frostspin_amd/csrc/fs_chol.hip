@@ -10,6 +10,7 @@
 // unshifted passes restore orthogonality to O(u).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "fs_hip.h"
@@ -309,6 +310,260 @@ __global__ void __launch_bounds__(kCholInvThreads) cholinv_kernel(CholInvBatch b
   if (threadIdx.x == 0) bt.status[blockIdx.x] = 0;
 }
 
+// Blocked form (default): four columns of R per barrier. The owners publish
+// four raw rows; every thread factors the 4 x 4 diagonal block redundantly,
+// derives the four panel rows at its own rows and columns by forward
+// substitution and applies the rank-4 update to its 8 x 4 register tile
+// (512 threads as 16 x 32, thread (ty, tx) owns rows ty + 16p, columns
+// tx + 32q). The inverse takes four rows of X per barrier the same way, with
+// the owners of four columns of R publishing them. The matrix is padded with
+// the identity up to a multiple of 4, so every step is branch-free.
+constexpr int kC4Rows = 16, kC4Cols = 32, kC4Blk = 4;
+constexpr int kC4P = kCholInvMaxOrder / kC4Rows;  // 8 rows per thread
+constexpr int kC4Q = kCholInvMaxOrder / kC4Cols;  // 4 columns per thread
+constexpr int kC4Threads = kC4Rows * kC4Cols;     // 512
+
+__global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
+  const CholInvDev D = bt.d[blockIdx.x];
+  const int l = D.l;
+  const double *G = bt.G + D.g_off;
+  const int tx = threadIdx.x % kC4Cols;
+  const int ty = threadIdx.x / kC4Cols;
+  __shared__ double rowbuf[2][kC4Blk][kCholInvMaxOrder];
+  __shared__ double colbuf[2][kC4Blk][kCholInvMaxOrder];
+  __shared__ double udiag[kCholInvMaxOrder / kC4Blk][kC4Blk * kC4Blk];
+  __shared__ double red[kC4Threads / 64];
+  auto g_at = [&](int i, int k) {
+    double v = 0.0;
+    for (int p = 0; p < D.n_part; ++p) v += G[p * D.g_stride + static_cast<int64_t>(i) * l + k];
+    return v;
+  };
+  double tr = 0.0;
+  for (int i = threadIdx.x; i < l; i += kC4Threads) tr += g_at(i, i);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tr += __shfl_xor(tr, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = tr;
+  __syncthreads();
+  double trace = 0.0;
+#pragma unroll
+  for (int w = 0; w < kC4Threads / 64; ++w) trace += red[w];
+  const double shift = D.shift_rel * trace;
+
+  double a[kC4P][kC4Q];
+#pragma unroll
+  for (int p = 0; p < kC4P; ++p) {
+#pragma unroll
+    for (int q = 0; q < kC4Q; ++q) {
+      const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
+      double v = (i == c) ? 1.0 : 0.0;  // identity padding beyond l
+      if (i < l && c < l) v = (i <= c) ? g_at(i, c) + (i == c ? shift : 0.0) : 0.0;
+      a[p][q] = v;
+    }
+  }
+  const int lpad = (l + kC4Blk - 1) / kC4Blk * kC4Blk;
+  int fail = 0;
+  for (int j0 = 0; j0 < lpad; j0 += kC4Blk) {
+    const int buf = (j0 / kC4Blk) & 1;
+    const int jp = j0 / kC4Rows, jt = j0 % kC4Rows;  // rows j0 .. j0+3: ty = jt .. jt+3, same p
+    if (ty >= jt && ty < jt + kC4Blk) {
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        double v = a[0][q];
+#pragma unroll
+        for (int p = 1; p < kC4P; ++p) v = (p == jp) ? a[p][q] : v;
+        rowbuf[buf][ty - jt][tx + kC4Cols * q] = v;
+      }
+    }
+    __syncthreads();
+    // 4 x 4 upper Cholesky of the diagonal block (every thread)
+    double U[kC4Blk][kC4Blk], uinv[kC4Blk];
+#pragma unroll
+    for (int r = 0; r < kC4Blk; ++r) {
+      double d = rowbuf[buf][r][j0 + r];
+#pragma unroll
+      for (int k = 0; k < r; ++k) d = fma(-U[k][r], U[k][r], d);
+      if (!(d > 0.0)) {  // uniform: same values in every thread
+        fail = j0 + r + 1;
+        break;
+      }
+      U[r][r] = sqrt(d);
+      uinv[r] = 1.0 / U[r][r];
+#pragma unroll
+      for (int t = r + 1; t < kC4Blk; ++t) {
+        double v = rowbuf[buf][r][j0 + t];
+#pragma unroll
+        for (int k = 0; k < r; ++k) v = fma(-U[k][r], U[k][t], v);
+        U[r][t] = v * uinv[r];
+      }
+    }
+    if (fail) break;
+    // panel rows at own columns (Rc) and own rows (Ri): forward substitution
+    double Rc[kC4Blk][kC4Q], Ri[kC4Blk][kC4P];
+#pragma unroll
+    for (int r = 0; r < kC4Blk; ++r) {
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        const int c = tx + kC4Cols * q;
+        double v = rowbuf[buf][r][c];
+#pragma unroll
+        for (int k = 0; k < r; ++k) v = fma(-U[k][r], Rc[k][q], v);
+        Rc[r][q] = (c >= j0 + kC4Blk) ? v * uinv[r] : 0.0;
+      }
+#pragma unroll
+      for (int p = 0; p < kC4P; ++p) {
+        const int i = ty + kC4Rows * p;
+        double v = rowbuf[buf][r][i];
+#pragma unroll
+        for (int k = 0; k < r; ++k) v = fma(-U[k][r], Ri[k][p], v);
+        Ri[r][p] = (i >= j0 + kC4Blk) ? v * uinv[r] : 0.0;
+      }
+    }
+    // rank-4 update (rows at or above the block get a zero factor)
+#pragma unroll
+    for (int p = 0; p < kC4P; ++p) {
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        double upd = a[p][q];
+#pragma unroll
+        for (int r = 0; r < kC4Blk; ++r) upd = fma(-Ri[r][p], Rc[r][q], upd);
+        a[p][q] = upd;
+      }
+    }
+    // the owners of the block's rows store them final: U inside the block,
+    // the panel row right of it, zero left of the diagonal
+    if (ty >= jt && ty < jt + kC4Blk) {
+      const int r = ty - jt;
+      double urow[kC4Blk], prow[kC4Q];
+#pragma unroll
+      for (int t = 0; t < kC4Blk; ++t) {
+        double v = U[0][t];
+#pragma unroll
+        for (int k = 1; k < kC4Blk; ++k) v = (k == r) ? U[k][t] : v;
+        urow[t] = (t >= r) ? v : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        double v = Rc[0][q];
+#pragma unroll
+        for (int k = 1; k < kC4Blk; ++k) v = (k == r) ? Rc[k][q] : v;
+        prow[q] = v;
+      }
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        const int c = tx + kC4Cols * q;
+        double fin = (c >= j0 + kC4Blk) ? prow[q] : 0.0;
+#pragma unroll
+        for (int t = 0; t < kC4Blk; ++t) fin = (c == j0 + t) ? urow[t] : fin;
+#pragma unroll
+        for (int p = 0; p < kC4P; ++p)
+          if (p == jp) a[p][q] = fin;
+      }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int r = 0; r < kC4Blk; ++r)
+#pragma unroll
+        for (int t = 0; t < kC4Blk; ++t) udiag[j0 / kC4Blk][r * kC4Blk + t] = t >= r ? U[r][t] : 0.0;
+    }
+  }
+  if (fail) {
+    if (threadIdx.x == 0) bt.status[blockIdx.x] = fail;
+    return;
+  }
+  if (bt.R) {
+    double *R = bt.R + D.r_off;
+#pragma unroll
+    for (int p = 0; p < kC4P; ++p) {
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
+        if (i < l && c < l) R[static_cast<int64_t>(i) * l + c] = (c >= i) ? a[p][q] : 0.0;
+      }
+    }
+  }
+  __syncthreads();  // udiag complete, row buffers free
+  double x[kC4P][kC4Q];
+#pragma unroll
+  for (int p = 0; p < kC4P; ++p) {
+#pragma unroll
+    for (int q = 0; q < kC4Q; ++q) x[p][q] = (ty + kC4Rows * p == tx + kC4Cols * q) ? 1.0 : 0.0;
+  }
+  for (int i0 = lpad - kC4Blk; i0 >= 0; i0 -= kC4Blk) {
+    const int buf = (i0 / kC4Blk) & 1;
+    const int ip = i0 / kC4Rows, it = i0 % kC4Rows;
+    const int cq = i0 / kC4Cols, ct = i0 % kC4Cols;  // columns i0 .. i0+3: tx = ct .. ct+3, same q
+    if (ty >= it && ty < it + kC4Blk) {  // current rows i0 .. i0+3 of B
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        double v = x[0][q];
+#pragma unroll
+        for (int p = 1; p < kC4P; ++p) v = (p == ip) ? x[p][q] : v;
+        rowbuf[buf][ty - it][tx + kC4Cols * q] = v;
+      }
+    }
+    if (tx >= ct && tx < ct + kC4Blk) {  // columns i0 .. i0+3 of R at rows k < i0
+#pragma unroll
+      for (int p = 0; p < kC4P; ++p) {
+        double v = a[p][0];
+#pragma unroll
+        for (int q = 1; q < kC4Q; ++q) v = (q == cq) ? a[p][q] : v;
+        const int k = ty + kC4Rows * p;
+        colbuf[buf][tx - ct][k] = (k < i0) ? v : 0.0;
+      }
+    }
+    __syncthreads();
+    double U[kC4Blk][kC4Blk];
+#pragma unroll
+    for (int r = 0; r < kC4Blk; ++r)
+#pragma unroll
+      for (int t = 0; t < kC4Blk; ++t) U[r][t] = udiag[i0 / kC4Blk][r * kC4Blk + t];
+    double Xc[kC4Blk][kC4Q];
+#pragma unroll
+    for (int r = kC4Blk - 1; r >= 0; --r) {
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        const int c = tx + kC4Cols * q;
+        double v = rowbuf[buf][r][c];
+#pragma unroll
+        for (int t = r + 1; t < kC4Blk; ++t) v = fma(-U[r][t], Xc[t][q], v);
+        Xc[r][q] = (c >= i0) ? v / U[r][r] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < kC4P; ++p) {
+      const int k = ty + kC4Rows * p;
+#pragma unroll
+      for (int r = 0; r < kC4Blk; ++r) {
+        const double ck = colbuf[buf][r][k];
+#pragma unroll
+        for (int q = 0; q < kC4Q; ++q) x[p][q] = fma(-ck, Xc[r][q], x[p][q]);
+      }
+    }
+    if (ty >= it && ty < it + kC4Blk) {  // the block's rows of X are final
+      const int r = ty - it;
+#pragma unroll
+      for (int q = 0; q < kC4Q; ++q) {
+        double v = Xc[0][q];
+#pragma unroll
+        for (int k = 1; k < kC4Blk; ++k) v = (k == r) ? Xc[k][q] : v;
+#pragma unroll
+        for (int p = 0; p < kC4P; ++p)
+          if (p == ip) x[p][q] = v;
+      }
+    }
+  }
+  double *X = bt.Rinv + D.rinv_off;
+#pragma unroll
+  for (int p = 0; p < kC4P; ++p) {
+#pragma unroll
+    for (int q = 0; q < kC4Q; ++q) {
+      const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
+      if (i < l && c < l) X[static_cast<int64_t>(i) * l + c] = x[p][q];
+    }
+  }
+  if (threadIdx.x == 0) bt.status[blockIdx.x] = 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -355,9 +610,18 @@ int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double 
         d.shift_rel = s.shift_rel;
       }
       bt.nmat = nb;
-      hipLaunchKernelGGL(cholinv_kernel, dim3(nb), dim3(kCholInvThreads), kCholInvLds,
-                         fs::as_stream(stream), bt);
-      FS_LAUNCH_CHECK("cholinv_kernel");
+      static const int mode = [] {
+        const char *e = std::getenv("FS_CHOLINV_KERNEL");  // tuning only: 1 = one column per step
+        return e ? std::atoi(e) : 0;
+      }();
+      if (mode == 1) {
+        hipLaunchKernelGGL(cholinv_kernel, dim3(nb), dim3(kCholInvThreads), kCholInvLds,
+                           fs::as_stream(stream), bt);
+        FS_LAUNCH_CHECK("cholinv_kernel");
+      } else {
+        hipLaunchKernelGGL(cholinv4_kernel, dim3(nb), dim3(kC4Threads), 0, fs::as_stream(stream), bt);
+        FS_LAUNCH_CHECK("cholinv4_kernel");
+      }
     }
     return FS_OK;
   });
