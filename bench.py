@@ -34,6 +34,12 @@ if ROOT not in sys.path:
 METRIC = "CTMRG iters/s SU(2) χ=256 mixed vs plain FP64 GPU at 1e-8 obs err; p50/p99"
 PEAK_FP64_MFMA = 78.6e12
 PEAK_HBM = 8.0e12
+# HBM bytes per grouped-GEMM dispatch on the default workload (SU(2), D=8,
+# chi=256, beta=4, FP64), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled
+# by the same counters on a 1 GiB device copy (factors 2.0 and 1.0); the
+# counters cannot be read live, so the figure applies to that workload only.
+GEMM_TRAFFIC_SU2_D8_CHI256 = 31.7e6
+GEMM_TRAFFIC_SOURCE = "profiles/ctmrg_su2_heisenberg_d8_chi256_beta4_hbm_traffic.md"
 PROFILE_MARKER_CYCLES = 20_000_000
 COPY_BYTES = 1 << 30  # hbm_copy workload: 1 GiB read + 1 GiB written
 
@@ -334,6 +340,8 @@ def run_ctmrg(args):
     ms = wall * 1000.0 / args.steps
     gemm = kr.get("gemm", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
     perm = kr.get("lie" if args.symmetry == "SU2" else "permute", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
+    default_workload = (args.symmetry, args.D, args.chi, args.beta, args.precision) == ("SU2", 8, 256, 4.0, "fp64")
+    traffic = GEMM_TRAFFIC_SU2_D8_CHI256 if default_workload else None
     res = {
         "metric": METRIC,
         "value": 1000.0 / ms,
@@ -364,7 +372,9 @@ def run_ctmrg(args):
             "peak": PEAK_FP64_MFMA / 1e12,
             "unit": "TFLOP/s",
             "frac": gemm["rate"] / PEAK_FP64_MFMA,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_unit": "bytes per dispatch",
+            "traffic_source": GEMM_TRAFFIC_SOURCE if traffic is not None else None,
             "launches": gemm["launches"],
             "dispatches": gemm["dispatches"],
             "avg_ms": gemm["ms"] / max(gemm["dispatches"], 1),
