@@ -4,19 +4,22 @@
 // Each task is an elementary block (one irrep per leg) holding n_in input rows
 // and n_out output rows of E coefficients; the kernel computes
 // out[o, :] = sum_i W[o, i] in[i, :] with every row gathered from / scattered
-// to its own strided place in the block (or dense) buffers. One 64-lane wave
-// owns one (task, tile of 64 external indices): each lane decodes its external
-// multi-index once, stages its input column in LDS (own column only, so no
-// barrier is needed), then produces the output rows 8 at a time with the
-// transposed structural matrix read as wave-uniform scalar loads. The lane
-// order follows the output's fastest external index, so stores coalesce.
+// to its own strided place in the block (or dense) buffers. A tile is (task,
+// 64 external indices), one per lane: each lane decodes its external
+// multi-index once, gathers its input column (n_in <= 32: into registers, one
+// wave per tile; 32 < n_in <= 128: into LDS, four waves per tile sharing it),
+// then produces the output rows 8 at a time with the transposed structural
+// matrix read as wave-uniform scalar loads. The lane order follows the
+// output's fastest external index, so stores coalesce.
 // This replaces the per-simple-block numba loops of
 // lie_group_symmetric_tensor.py:96-265 (transpose-reshape + small GEMM +
 // slice) and the per-elementary-block projections of lie:856-1109.
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <tuple>
 #include <vector>
 
 #include "fs_hip.h"
@@ -25,12 +28,16 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kOutChunk = 8;
+// The device copy of each task's W^T holds n_in rounded up to kRowGroup rows
+// (the extra rows zero), so the kernels read it kRowGroup rows at a time with
+// every scalar load of a group in flight together.
+constexpr int kRowGroup = 4;
 constexpr int kMaxLdsRows = 96;  // 96 * 64 * 8 B = 48 KiB of LDS per wave
 
 struct TaskDev {
   int64_t w_off;
   int32_t e_size, n_in, n_out, n_out_pad;
-  int32_t row0, ndig, dig0, pad;
+  int32_t row0, ndig, dig0, n_in_pad;  // n_in rounded up to kRowGroup (device W^T rows)
 };
 
 // In-block offsets (er * rs + ec * cs) are validated below 2^31, so row and
@@ -81,6 +88,36 @@ __device__ inline int64_t row_offset(const RowDev &r, int32_t er, int32_t ec) {
   return r.base + static_cast<int64_t>(er) * r.rs + static_cast<int64_t>(ec) * r.cs;
 }
 
+// Row descriptors travel one per lane: a wave reads up to 64 of them with one
+// vector load (lane l holds rows[first + min(l, count - 1)]; callers pass
+// count >= 1 or, for a task without input rows, first = 0, where the task's
+// first output row lies, so every read stays inside the task's rows), and
+// row_from_lane() broadcasts lane j's to the
+// wave through readlane. The gathers that follow then wait on no memory but
+// their own, instead of one scalar descriptor load per row. Must be called
+// with all lanes of the wave active.
+struct LaneRow {
+  uint32_t base_lo, base_hi;
+  int32_t rs, cs;
+};
+
+__device__ inline LaneRow load_lane_rows(const RowDev *__restrict__ rows, int first, int count,
+                                         int lane) {
+  const RowDev r = rows[first + min(lane, max(count - 1, 0))];
+  const uint64_t b = static_cast<uint64_t>(r.base);
+  return {static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), r.rs, r.cs};
+}
+
+__device__ inline RowDev row_from_lane(const LaneRow &l, int j) {
+  const uint64_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(l.base_lo), j));
+  const uint64_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(l.base_hi), j));
+  RowDev r;
+  r.base = static_cast<int64_t>((hi << 32) | lo);
+  r.rs = __builtin_amdgcn_readlane(l.rs, j);
+  r.cs = __builtin_amdgcn_readlane(l.cs, j);
+  return r;
+}
+
 // Register form: each lane keeps its whole input column (n_in <= kRegIn) in
 // registers, loaded back to back so the gathers overlap; every output row is
 // then one wave-uniform dot product. Tasks with more input rows stream their
@@ -105,11 +142,16 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
 
   if (tk.n_in <= kRegIn) {
     double x[kRegIn];
+    const LaneRow lin = load_lane_rows(in_rows, 0, tk.n_in, lane);
+    const LaneRow lout = load_lane_rows(out_rows, 0, tk.n_out, lane);
+    // both descriptor loads complete here, before any gather is issued: later
+    // readlanes then need no wait (which would also drain the gathers)
+    __builtin_amdgcn_s_waitcnt(0);
 #pragma unroll
     for (int i = 0; i < kRegIn; ++i) {
       x[i] = 0.0;
       if (i < tk.n_in) {
-        const RowDev r = in_rows[i];
+        const RowDev r = row_from_lane(lin, i);
         if (active && r.base >= 0) x[i] = src[row_offset(r, er_in, ec_in)];
       }
     }
@@ -122,18 +164,22 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
       for (int q = 0; q < kOutChunk; ++q) acc[q] = 0.0;
       const double *wt = w + tk.w_off + o0;
 #pragma unroll
-      for (int i = 0; i < kRegIn; ++i) {
-        if (i < tk.n_in) {
-          const double *wi = wt + static_cast<int64_t>(i) * tk.n_out_pad;
+      for (int i0 = 0; i0 < kRegIn; i0 += kRowGroup) {
+        if (i0 < tk.n_in) {  // rows n_in .. n_in_pad - 1: zero W, zero x
 #pragma unroll
-          for (int q = 0; q < kOutChunk; ++q) acc[q] = fma(wi[q], x[i], acc[q]);
+          for (int j = 0; j < kRowGroup; ++j) {
+            const double *wi = wt + static_cast<int64_t>(i0 + j) * tk.n_out_pad;
+#pragma unroll
+            for (int q = 0; q < kOutChunk; ++q) acc[q] = fma(wi[q], x[i0 + j], acc[q]);
+          }
         }
       }
-      if (active) {
-        const int nw = min(kOutChunk, tk.n_out - o0);
-        for (int q = 0; q < nw; ++q) {
-          const RowDev ro = out_rows[o0 + q];
-          dst[row_offset(ro, er_out, ec_out)] = acc[q];
+#pragma unroll
+      for (int q = 0; q < kOutChunk; ++q) {
+        if (o0 + q < tk.n_out) {
+          // output rows past the first 64 (rare) come straight from the table
+          const RowDev ro = o0 < kLanes ? row_from_lane(lout, o0 + q) : out_rows[o0 + q];
+          if (active) dst[row_offset(ro, er_out, ec_out)] = acc[q];
         }
       }
     }
@@ -160,6 +206,80 @@ lie_transform_reg_kernel(const TileDev *__restrict__ tiles, const TaskDev *__res
       for (int q = 0; q < nw; ++q) {
         const RowDev ro = out_rows[o0 + q];
         dst[row_offset(ro, er_out, ec_out)] = acc[q];
+      }
+    }
+  }
+}
+
+// Wide form for tasks with kRegIn < n_in <= kWideMaxIn: four waves share one
+// tile of 64 external indices. The waves gather the input column into LDS,
+// kWideBatch rows per wave with every load of a batch in flight at once, then
+// each wave produces every fourth chunk of 8 output rows from LDS. Against the
+// streamed path of the register form (one dependent gather per input row and
+// output chunk) this turns ~n_in * n_out / 16 serial memory latencies per wave
+// into ~n_in / 32.
+constexpr int kWideWaves = 4;
+constexpr int kWideMaxIn = 128;  // 128 * 64 * 8 B = 64 KiB of LDS per workgroup
+constexpr int kWideBatch = 8;
+
+__global__ void __launch_bounds__(kLanes * kWideWaves)
+lie_transform_wide_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restrict__ tasks,
+                          const RowDev *__restrict__ rows, const DigitDev *__restrict__ digits,
+                          const double *__restrict__ w, const double *__restrict__ src,
+                          double *__restrict__ dst) {
+  extern __shared__ double xs[];  // [n_in][kLanes]
+  const TileDev tile = tiles[blockIdx.x];
+  const TaskDev tk = tasks[tile.task];
+  const int lane = threadIdx.x % kLanes;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  const int32_t e = tile.e0 + lane;
+  const bool active = e < tk.e_size;
+  int32_t er_in = 0, ec_in = 0, er_out = 0, ec_out = 0;
+  if (active) decode_external(digits, tk, e, er_in, ec_in, er_out, ec_out);
+  const RowDev *in_rows = rows + tk.row0;
+  const RowDev *out_rows = in_rows + tk.n_in;
+
+  // rows 0..63 and 64..127 of the input column, one descriptor per lane
+  // (n_in > kRegIn >= 1 here; with n_in <= 64 the second read repeats row n_in - 1)
+  const LaneRow lin0 = load_lane_rows(in_rows, 0, tk.n_in, lane);
+  const LaneRow lin1 = load_lane_rows(in_rows, min(kLanes, tk.n_in - 1), max(tk.n_in - kLanes, 1), lane);
+  for (int i0 = wave * kWideBatch; i0 < tk.n_in; i0 += kWideWaves * kWideBatch) {
+    double v[kWideBatch];
+    const LaneRow &lr = i0 < kLanes ? lin0 : lin1;  // a batch never straddles row 64
+#pragma unroll
+    for (int j = 0; j < kWideBatch; ++j) {
+      v[j] = 0.0;
+      if (i0 + j < tk.n_in) {
+        const RowDev r = row_from_lane(lr, (i0 + j) % kLanes);
+        if (active && r.base >= 0) v[j] = src[row_offset(r, er_in, ec_in)];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kWideBatch; ++j)
+      if (i0 + j < tk.n_in_pad) xs[(i0 + j) * kLanes + lane] = v[j];  // zero past n_in
+  }
+  __syncthreads();
+
+  for (int o0 = wave * kOutChunk; o0 < tk.n_out; o0 += kWideWaves * kOutChunk) {
+    const LaneRow lout = load_lane_rows(out_rows, o0, tk.n_out - o0, lane);
+    double acc[kOutChunk];
+#pragma unroll
+    for (int q = 0; q < kOutChunk; ++q) acc[q] = 0.0;
+    const double *wt = w + tk.w_off + o0;  // W^T rows padded to a multiple of 8
+    for (int i0 = 0; i0 < tk.n_in_pad; i0 += kRowGroup) {
+#pragma unroll
+      for (int j = 0; j < kRowGroup; ++j) {
+        const double x = xs[(i0 + j) * kLanes + lane];
+        const double *wi = wt + static_cast<int64_t>(i0 + j) * tk.n_out_pad;
+#pragma unroll
+        for (int q = 0; q < kOutChunk; ++q) acc[q] = fma(wi[q], x, acc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kOutChunk; ++q) {
+      if (o0 + q < tk.n_out) {
+        const RowDev ro = row_from_lane(lout, q);
+        if (active) dst[row_offset(ro, er_out, ec_out)] = acc[q];
       }
     }
   }
@@ -229,8 +349,10 @@ struct fs_lie_plan {
   const RowDev *rows = nullptr;
   const DigitDev *digits = nullptr;
   const double *w = nullptr;
-  int64_t ntiles = 0;
+  int64_t ntiles = 0;       // all tiles: register-form tiles, then wide-form tiles
+  int64_t ntiles_wide = 0;  // tiles of tasks with kRegIn < n_in <= kWideMaxIn
   int32_t lds_rows = 0;
+  int32_t wide_rows = 0;    // largest n_in_pad among the wide-form tasks
   int64_t src_size = 0, dst_size = 0;
   double flops = 0.0, coeffs = 0.0;
 };
@@ -305,14 +427,29 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     auto p = std::make_unique<fs_lie_plan>();
     std::vector<TaskDev> tdev;
     std::vector<TileDev> tiles;
+    std::vector<int32_t> reg_ids, wide_ids;
+    // device copy of W^T with every task's rows padded to a multiple of
+    // kRowGroup; tasks sharing a structural matrix share its copy
+    std::vector<double> wdev;
+    std::map<std::tuple<int64_t, int64_t, int64_t>, int64_t> w_copies;
     tdev.reserve(static_cast<size_t>(ntasks));
-    int64_t max_in = 0;
+    int64_t max_in = 0, max_wide_in = 0;
     for (int64_t t = 0; t < ntasks; ++t) {
       const fs_lie_task &s = tasks[t];
       int st = check_task(s, nrows, rows, ndigits, digits, nw, src_size, dst_size);
       if (st != FS_OK) return st;
       TaskDev d;
-      d.w_off = s.w_off;
+      const int64_t n_in_pad = (s.n_in + kRowGroup - 1) / kRowGroup * kRowGroup;
+      const auto key = std::make_tuple(s.w_off, s.n_in, s.n_out_pad);
+      auto found = w_copies.find(key);
+      if (found == w_copies.end()) {
+        const int64_t off = static_cast<int64_t>(wdev.size());
+        if (s.n_in * s.n_out_pad > 0)
+          wdev.insert(wdev.end(), w + s.w_off, w + s.w_off + s.n_in * s.n_out_pad);
+        wdev.resize(static_cast<size_t>(off + n_in_pad * s.n_out_pad), 0.0);
+        found = w_copies.emplace(key, off).first;
+      }
+      d.w_off = found->second;
       d.e_size = static_cast<int32_t>(s.e_size);
       d.n_in = static_cast<int32_t>(s.n_in);
       d.n_out = static_cast<int32_t>(s.n_out);
@@ -320,15 +457,33 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
       d.row0 = static_cast<int32_t>(s.row0);
       d.ndig = static_cast<int32_t>(s.ndig);
       d.dig0 = static_cast<int32_t>(s.dig0);
-      d.pad = 0;
+      d.n_in_pad = static_cast<int32_t>(n_in_pad);
       const int32_t id = static_cast<int32_t>(tdev.size());
       tdev.push_back(d);
       if (s.n_out == 0) continue;
       max_in = std::max(max_in, s.n_in);
-      for (int64_t e0 = 0; e0 < s.e_size; e0 += kLanes) tiles.push_back({id, static_cast<int32_t>(e0)});
+      if (s.n_in > kRegIn && s.n_in <= kWideMaxIn) {
+        wide_ids.push_back(id);
+        max_wide_in = std::max(max_wide_in, n_in_pad);
+      } else {
+        reg_ids.push_back(id);
+      }
       p->flops += 2.0 * double(s.n_in) * double(s.n_out) * double(s.e_size);
       p->coeffs += double(s.n_in + s.n_out) * double(s.e_size);
     }
+    // register-form tasks heaviest first (n_in * n_out per element), so that
+    // the longest waves start early instead of forming the launch's tail
+    std::stable_sort(reg_ids.begin(), reg_ids.end(), [&](int32_t a, int32_t b) {
+      return int64_t(tdev[a].n_in) * tdev[a].n_out > int64_t(tdev[b].n_in) * tdev[b].n_out;
+    });
+    auto add_tiles = [&](const std::vector<int32_t> &ids) {
+      for (int32_t id : ids)
+        for (int64_t e0 = 0; e0 < tdev[id].e_size; e0 += kLanes)
+          tiles.push_back({id, static_cast<int32_t>(e0)});
+    };
+    add_tiles(reg_ids);
+    const size_t n_reg_tiles = tiles.size();
+    add_tiles(wide_ids);
     FS_CHECK_ARG(tiles.size() < (size_t(1) << 31), "lie plan: too many tiles");
     std::vector<RowDev> rdev(static_cast<size_t>(nrows));
     for (int64_t i = 0; i < nrows; ++i)
@@ -344,7 +499,7 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t b_tiles = tiles.size() * sizeof(TileDev), b_tasks = tdev.size() * sizeof(TaskDev);
     const size_t b_rows = rdev.size() * sizeof(RowDev), b_dig = ddev.size() * sizeof(DigitDev);
-    const size_t b_w = static_cast<size_t>(nw) * sizeof(double);
+    const size_t b_w = wdev.size() * sizeof(double);
     const size_t total = al(b_tiles) + al(b_tasks) + al(b_rows) + al(b_dig) + al(b_w) + 256;
     FS_HIP_CHECK(hipGetDevice(&p->device));
     char *mem = nullptr;
@@ -372,7 +527,7 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     copy(d_tasks, tdev.data(), b_tasks);
     copy(d_rows, rdev.data(), b_rows);
     copy(d_dig, ddev.data(), b_dig);
-    copy(d_w, w, b_w);
+    copy(d_w, wdev.data(), b_w);
     if (st != FS_OK) {
       (void)hipFree(mem);
       return st;
@@ -383,6 +538,8 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     p->digits = reinterpret_cast<const DigitDev *>(d_dig);
     p->w = reinterpret_cast<const double *>(d_w);
     p->ntiles = static_cast<int64_t>(tiles.size());
+    p->ntiles_wide = static_cast<int64_t>(tiles.size() - n_reg_tiles);
+    p->wide_rows = static_cast<int32_t>(max_wide_in);
     p->lds_rows = static_cast<int32_t>(std::min<int64_t>(max_in, kMaxLdsRows));
     p->src_size = src_size;
     p->dst_size = dst_size;
@@ -427,7 +584,9 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
     FS_CHECK_ARG(dev == plan->device, "lie plan was built on device %d, current device is %d",
                  plan->device, dev);
     static const int mode = [] {
-      const char *e = std::getenv("FS_LIE_KERNEL");  // tuning only: 1 = LDS-staged form
+      // tuning only: 1 = one-wave LDS-staged form for every task, 2 = register
+      // form for every task (wide tasks on its streamed path)
+      const char *e = std::getenv("FS_LIE_KERNEL");
       return e ? std::atoi(e) : 0;
     }();
     if (mode == 1) {
@@ -437,10 +596,23 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
                          plan->rows, plan->digits, plan->w, src, dst, plan->lds_rows);
       FS_LAUNCH_CHECK("lie_transform_kernel");
     } else {
-      hipLaunchKernelGGL(lie_transform_reg_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
-                         dim3(kLanes), 0, fs::as_stream(stream), plan->tiles, plan->tasks,
-                         plan->rows, plan->digits, plan->w, src, dst);
-      FS_LAUNCH_CHECK("lie_transform_reg_kernel");
+      const int64_t n_wide = mode == 2 ? 0 : plan->ntiles_wide;
+      const int64_t n_reg = plan->ntiles - n_wide;
+      if (n_wide > 0) {
+        // every wide task has n_in <= wide_rows <= kWideMaxIn: the LDS column fits
+        const size_t lds = static_cast<size_t>(plan->wide_rows) * kLanes * sizeof(double);
+        hipLaunchKernelGGL(lie_transform_wide_kernel, dim3(static_cast<unsigned>(n_wide)),
+                           dim3(kLanes * kWideWaves), lds, fs::as_stream(stream),
+                           plan->tiles + n_reg, plan->tasks, plan->rows, plan->digits, plan->w,
+                           src, dst);
+        FS_LAUNCH_CHECK("lie_transform_wide_kernel");
+      }
+      if (n_reg > 0) {
+        hipLaunchKernelGGL(lie_transform_reg_kernel, dim3(static_cast<unsigned>(n_reg)),
+                           dim3(kLanes), 0, fs::as_stream(stream), plan->tiles, plan->tasks,
+                           plan->rows, plan->digits, plan->w, src, dst);
+        FS_LAUNCH_CHECK("lie_transform_reg_kernel");
+      }
     }
     return FS_OK;
   });
