@@ -41,6 +41,19 @@ class GemmGroup(ctypes.Structure):
     ]
 
 
+class LiePattern(ctypes.Structure):
+    """fs_lie_pattern: irreps-only data of one permute pattern (int64 arrays)."""
+
+    _fields_ = [(n, c_i64) for n in ("ndim", "nrr", "nrr_out", "ntasks", "nt_in", "nt_out")] + [
+        (n, c_i64p)
+        for n in (
+            "axes", "nirr", "sig_idx", "ir_in", "ic_in", "ir_out", "ic_out", "n_in", "n_out",
+            "w_off", "n_out_pad", "idr_in", "idc_in", "idr_out", "idc_out",
+            "in_t", "in_a", "in_b", "out_t", "out_a", "out_b",
+        )
+    ]
+
+
 class SvdDesc(ctypes.Structure):
     _fields_ = [
         ("m", c_i64),
@@ -166,6 +179,11 @@ _SIGNATURES = {
     ),
     "fs_lie_plan_check": (
         c_int, [c_i64, c_i64p, c_i64, c_i64p, c_i64, c_i64p, c_i64, c_i64, c_i64],
+    ),
+    "fs_lie_plan_tables": (
+        c_int,
+        [ctypes.POINTER(LiePattern), c_i64, c_i64p, c_i64p, c_i64, c_i64p, c_i64p, c_i64p,
+         c_i64p, c_i64, c_i64p, c_i64p, c_i64p],
     ),
     "fs_lie_plan_info": (c_int, [c_vp, c_i64p, c_dblp, c_dblp]),
     "fs_lie_plan_execute_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),

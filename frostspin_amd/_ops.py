@@ -293,6 +293,26 @@ def permute(plan, src, dtype=DTYPE):
     return out
 
 
+def lie_plan_tables(pattern, degen, in_block, block_off, block_rs, block_cs, *, nrows):
+    """Host tables (tasks, rows, digits, new block shapes) of a structural
+    transform plan for one permute pattern and one set of leg degeneracies
+    (fs_lie_plan_tables)."""
+    c = np.ascontiguousarray
+    degen, in_block = c(degen, dtype=np.int64), c(in_block, dtype=np.int64)
+    block_off, block_rs, block_cs = (c(x, dtype=np.int64) for x in (block_off, block_rs, block_cs))
+    assert in_block.size == pattern.nt_in
+    tasks = np.empty((pattern.ntasks, 8), dtype=np.int64)
+    rows = np.empty((int(nrows), 3), dtype=np.int64)
+    digits = np.empty((pattern.ntasks * pattern.ndim, 5), dtype=np.int64)
+    shapes = np.empty((pattern.nt_out, 2), dtype=np.int64)
+    check(lib.fs_lie_plan_tables(
+        ctypes.byref(pattern), degen.size, _lib.i64p(degen), _lib.i64p(in_block), block_off.size,
+        _lib.i64p(block_off), _lib.i64p(block_rs), _lib.i64p(block_cs),
+        _lib.i64p(tasks), rows.shape[0], _lib.i64p(rows), _lib.i64p(digits), _lib.i64p(shapes),
+    ))
+    return tasks, rows, digits, shapes
+
+
 class LiePlan:
     """Owner of a native structural-transform plan (fs_lie_plan_*).
 

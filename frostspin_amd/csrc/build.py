@@ -27,6 +27,7 @@ SOURCES = [
     "fs_chol.hip",
     "fs_lie.hip",
     "fs_elementwise.hip",
+    "fs_runtime.hip",
 ]
 ARCH = os.environ.get("FS_OFFLOAD_ARCH", "gfx950")
 

@@ -343,7 +343,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
 
 struct fs_lie_plan {
   int device = -1;
-  void *d_mem = nullptr;
+  mutable fs::DeviceTables tables;  // tiles, tasks, rows, digits, W^T (256-byte aligned parts)
   const TileDev *tiles = nullptr;
   const TaskDev *tasks = nullptr;
   const RowDev *rows = nullptr;
@@ -495,43 +495,31 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
                  static_cast<int32_t>(d.in_r), static_cast<int32_t>(d.in_c),
                  static_cast<int32_t>(d.out_r), static_cast<int32_t>(d.out_c), 0};
     }
-    // one device allocation, 256-byte aligned parts
+    // one host image, 256-byte aligned parts, uploaded as one allocation
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t b_tiles = tiles.size() * sizeof(TileDev), b_tasks = tdev.size() * sizeof(TaskDev);
     const size_t b_rows = rdev.size() * sizeof(RowDev), b_dig = ddev.size() * sizeof(DigitDev);
     const size_t b_w = wdev.size() * sizeof(double);
-    const size_t total = al(b_tiles) + al(b_tasks) + al(b_rows) + al(b_dig) + al(b_w) + 256;
+    const size_t o_tasks = al(b_tiles), o_rows = o_tasks + al(b_tasks), o_dig = o_rows + al(b_rows);
+    const size_t o_w = o_dig + al(b_dig), total = o_w + al(b_w) + 256;
+    std::vector<char> image(total, 0);
+    auto put = [&](size_t off, const void *h, size_t bytes) {
+      if (bytes) std::memcpy(image.data() + off, h, bytes);
+    };
+    put(0, tiles.data(), b_tiles);
+    put(o_tasks, tdev.data(), b_tasks);
+    put(o_rows, rdev.data(), b_rows);
+    put(o_dig, ddev.data(), b_dig);
+    put(o_w, wdev.data(), b_w);
     FS_HIP_CHECK(hipGetDevice(&p->device));
-    char *mem = nullptr;
-    FS_HIP_CHECK(hipMalloc(&mem, total));
-    p->d_mem = mem;
-    size_t off = 0;
-    auto put = [&](size_t bytes) -> char * {
-      char *d = mem + off;
-      off += al(bytes);
-      return bytes ? d : nullptr;
-    };
-    char *d_tiles = put(b_tiles);
-    char *d_tasks = put(b_tasks);
-    char *d_rows = put(b_rows);
-    char *d_dig = put(b_dig);
-    char *d_w = put(b_w);
-    int st = FS_OK;
-    auto copy = [&](char *d, const void *h, size_t bytes) {
-      if (st == FS_OK && bytes && hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) {
-        fs::set_error("lie plan: upload failed");
-        st = FS_ERR_HIP;
-      }
-    };
-    copy(d_tiles, tiles.data(), b_tiles);
-    copy(d_tasks, tdev.data(), b_tasks);
-    copy(d_rows, rdev.data(), b_rows);
-    copy(d_dig, ddev.data(), b_dig);
-    copy(d_w, wdev.data(), b_w);
-    if (st != FS_OK) {
-      (void)hipFree(mem);
-      return st;
-    }
+    int st = p->tables.upload(image.data(), total);
+    if (st != FS_OK) return st;
+    const char *mem = p->tables.data();
+    const char *d_tiles = b_tiles ? mem : nullptr;
+    const char *d_tasks = b_tasks ? mem + o_tasks : nullptr;
+    const char *d_rows = b_rows ? mem + o_rows : nullptr;
+    const char *d_dig = b_dig ? mem + o_dig : nullptr;
+    const char *d_w = b_w ? mem + o_w : nullptr;
     p->tiles = reinterpret_cast<const TileDev *>(d_tiles);
     p->tasks = reinterpret_cast<const TaskDev *>(d_tasks);
     p->rows = reinterpret_cast<const RowDev *>(d_rows);
@@ -589,6 +577,7 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
       const char *e = std::getenv("FS_LIE_KERNEL");
       return e ? std::atoi(e) : 0;
     }();
+    plan->tables.note_stream(fs::as_stream(stream));
     if (mode == 1) {
       const size_t lds = static_cast<size_t>(plan->lds_rows) * kLanes * sizeof(double);
       hipLaunchKernelGGL(lie_transform_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
@@ -621,21 +610,9 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
 int fs_lie_plan_destroy(fs_lie_plan *plan) {
   return fs::guarded([&]() -> int {
     if (!plan) return FS_OK;
-    if (plan->d_mem) {
-      int cur = 0;
-      FS_HIP_CHECK(hipGetDevice(&cur));
-      if (cur != plan->device) FS_HIP_CHECK(hipSetDevice(plan->device));
-      hipError_t e = hipFree(plan->d_mem);
-      if (cur != plan->device) (void)hipSetDevice(cur);
-      plan->d_mem = nullptr;
-      if (e != hipSuccess) {
-        delete plan;
-        fs::set_error("hipFree failed: %s", hipGetErrorString(e));
-        return FS_ERR_HIP;
-      }
-    }
+    const int st = plan->tables.release();
     delete plan;
-    return FS_OK;
+    return st;
   });
 }
 

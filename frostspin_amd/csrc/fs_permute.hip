@@ -9,6 +9,7 @@
 // 103-250) inverted into destination order; coefficients whose old block is
 // missing read as zero exactly as the reference's zero-initialised blocks.
 #include <cstring>
+#include <memory>
 
 #include "fs_hip.h"
 #include "fs_plan.h"
@@ -68,8 +69,9 @@ template <typename V> size_t bytes_of(const std::vector<V> &v) { return v.size()
 int upload(fs_permute_plan *p) {
   int dev = 0;
   FS_HIP_CHECK(hipGetDevice(&dev));
-  if (p->d_mem && p->device == dev) return FS_OK;
-  if (p->d_mem) {
+  std::lock_guard<std::mutex> lk(p->upload_mu);
+  if (p->d_tables && p->device == dev) return FS_OK;
+  if (p->d_tables) {
     fs::set_error("permute plan was uploaded on device %d, current device is %d", p->device, dev);
     return FS_ERR_INVALID;
   }
@@ -100,17 +102,11 @@ int upload(fs_permute_plan *p) {
   std::vector<char> host(total, 0);
   for (size_t i = 0; i < parts.size(); ++i)
     if (parts[i].bytes) std::memcpy(host.data() + offs[i], parts[i].src, parts[i].bytes);
-  void *mem = nullptr;
-  FS_HIP_CHECK(hipMalloc(&mem, total));
-  hipError_t e = hipMemcpy(mem, host.data(), total, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    (void)hipFree(mem);
-    fs::set_error("hipMemcpy of permute tables failed: %s", hipGetErrorString(e));
-    return FS_ERR_HIP;
-  }
-  for (size_t i = 0; i < parts.size(); ++i)
-    *parts[i].dst = static_cast<const char *>(mem) + offs[i];
-  p->d_mem = mem;
+  auto tables = std::make_unique<fs::DeviceTables>();
+  int st = tables->upload(host.data(), total);
+  if (st != FS_OK) return st;
+  for (size_t i = 0; i < parts.size(); ++i) *parts[i].dst = tables->data() + offs[i];
+  p->d_tables = tables.release();
   p->device = dev;
   return FS_OK;
 }
@@ -145,6 +141,7 @@ int execute(fs_permute_plan *plan, const T *src, T *dst, void *stream) {
   p.nblocks = static_cast<int32_t>(nb);
   const int64_t ntiles = plan->tile_prefix.back();
   FS_CHECK_ARG(ntiles < (int64_t(1) << 31), "permute execute: too many tiles");
+  static_cast<fs::DeviceTables *>(plan->d_tables)->note_stream(fs::as_stream(stream));
   hipLaunchKernelGGL(permute_gather_kernel<T>, dim3(static_cast<unsigned>(ntiles)), dim3(256), 0,
                      fs::as_stream(stream), src, dst, p);
   FS_LAUNCH_CHECK("permute_gather_kernel");
@@ -179,15 +176,11 @@ int fs_permute_plan_execute_f32(fs_permute_plan *plan, const float *src, float *
 int fs_permute_plan_destroy(fs_permute_plan *plan) {
   return fs::guarded([&]() -> int {
     if (!plan) return FS_OK;
-    if (plan->d_mem) {
-      int cur = 0;
-      (void)hipGetDevice(&cur);
-      if (cur != plan->device) (void)hipSetDevice(plan->device);
-      (void)hipFree(plan->d_mem);
-      if (cur != plan->device) (void)hipSetDevice(cur);
-    }
+    auto *tables = static_cast<fs::DeviceTables *>(plan->d_tables);
+    const int st = tables ? tables->release() : FS_OK;
+    delete tables;
     delete plan;
-    return FS_OK;
+    return st;
   });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "frostspin_amd.h"
@@ -26,9 +27,11 @@ struct fs_permute_plan {
   int64_t n_old_rows = 0, n_old_cols = 0, n_new_rows_total = 0, n_new_cols_total = 0;
   std::vector<int64_t> tile_prefix;
 
-  // device mirror (one allocation), filled lazily by the executor
-  void *d_mem = nullptr;
+  // device mirror (one fs::DeviceTables, owned), filled lazily by the
+  // executor under upload_mu
+  void *d_tables = nullptr;
   int device = -1;
+  std::mutex upload_mu;
   const int32_t *d_old_row_block = nullptr, *d_old_row_pos = nullptr;
   const int32_t *d_old_col_block = nullptr, *d_old_col_pos = nullptr;
   const int32_t *d_row_a = nullptr, *d_col_b = nullptr;

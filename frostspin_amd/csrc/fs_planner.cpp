@@ -295,6 +295,41 @@ void fs_permute_plan::finalize_tiles() {
 }
 
 
+namespace {
+
+// Row-major outer product of the degeneracy vectors of `legs` (last leg
+// fastest): the degeneracy of every irreps combination of those legs.
+std::vector<int64_t> degen_outer(const std::vector<int64_t> &legs, const int64_t *degen,
+                                 const std::vector<int64_t> &doff, const int64_t *nirr) {
+  std::vector<int64_t> out(1, 1);
+  for (int64_t leg : legs) {
+    const int64_t n = nirr[leg];
+    std::vector<int64_t> next(out.size() * static_cast<size_t>(n));
+    for (size_t i = 0; i < out.size(); ++i)
+      for (int64_t j = 0; j < n; ++j) next[i * n + j] = out[i] * degen[doff[leg] + j];
+    out.swap(next);
+  }
+  return out;
+}
+
+// start[c][t]: first row (column) of combination c inside target t, i.e. the
+// exclusive running sum over c of ed[c] * id[c][t]; total[t] is the full sum.
+void block_starts(const std::vector<int64_t> &ed, const int64_t *id, int64_t nt,
+                  std::vector<int64_t> &start, std::vector<int64_t> &total) {
+  start.assign(ed.size() * static_cast<size_t>(nt), 0);
+  total.assign(static_cast<size_t>(nt), 0);
+  for (int64_t t = 0; t < nt; ++t) {
+    int64_t run = 0;
+    for (size_t c = 0; c < ed.size(); ++c) {
+      start[c * nt + t] = run;
+      run += ed[c] * id[c * nt + t];
+    }
+    total[t] = run;
+  }
+}
+
+} // namespace
+
 extern "C" {
 
 const char *fs_last_error(void) { return fs::g_last_error.c_str(); }
@@ -425,6 +460,134 @@ int fs_permute_plan_tables(const fs_permute_plan *plan, int64_t *sizes,
     cp64(old_off, plan->old_off);
     cp64(old_rs, plan->old_rs);
     cp64(old_cs, plan->old_cs);
+    return FS_OK;
+  });
+}
+
+int fs_lie_plan_tables(const fs_lie_pattern *pat, int64_t ndegen, const int64_t *degen,
+                       const int64_t *in_block, int64_t nblocks, const int64_t *block_off,
+                       const int64_t *block_rs, const int64_t *block_cs, int64_t *tasks,
+                       int64_t nrows, int64_t *rows, int64_t *digits, int64_t *shapes) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(pat && degen && in_block && tasks && rows && digits && shapes,
+                 "lie plan tables: null argument");
+    const int64_t nd = pat->ndim, nrr = pat->nrr, nro = pat->nrr_out, ns = pat->ntasks;
+    const int64_t nti = pat->nt_in, nto = pat->nt_out;
+    FS_CHECK_ARG(nd >= 1 && nd <= FS_LIE_MAX_LEGS && nrr >= 0 && nrr <= nd && nro >= 0 && nro <= nd,
+                 "lie plan tables: bad leg counts");
+    FS_CHECK_ARG(ns >= 0 && nti >= 0 && nto >= 0 && nblocks >= 0, "lie plan tables: negative size");
+    FS_CHECK_ARG(nblocks == 0 || (block_off && block_rs && block_cs), "lie plan tables: null block table");
+    std::vector<int64_t> doff(static_cast<size_t>(nd) + 1, 0);
+    std::vector<bool> seen(static_cast<size_t>(nd), false);
+    for (int64_t k = 0; k < nd; ++k) {
+      FS_CHECK_ARG(pat->nirr[k] >= 1, "lie plan tables: leg %lld has no irrep", static_cast<long long>(k));
+      doff[k + 1] = doff[k] + pat->nirr[k];
+      const int64_t a = pat->axes[k];
+      FS_CHECK_ARG(a >= 0 && a < nd && !seen[a], "lie plan tables: axes is not a permutation");
+      seen[a] = true;
+    }
+    FS_CHECK_ARG(doff[nd] == ndegen, "lie plan tables: %lld degeneracies for %lld irreps",
+                 static_cast<long long>(ndegen), static_cast<long long>(doff[nd]));
+    std::vector<int64_t> legs_ri, legs_ci, legs_ro, legs_co;
+    for (int64_t k = 0; k < nd; ++k) {
+      (k < nrr ? legs_ri : legs_ci).push_back(k);
+      (k < nro ? legs_ro : legs_co).push_back(pat->axes[k]);
+    }
+    const std::vector<int64_t> edr = degen_outer(legs_ri, degen, doff, pat->nirr);
+    const std::vector<int64_t> edc = degen_outer(legs_ci, degen, doff, pat->nirr);
+    const std::vector<int64_t> edr2 = degen_outer(legs_ro, degen, doff, pat->nirr);
+    const std::vector<int64_t> edc2 = degen_outer(legs_co, degen, doff, pat->nirr);
+    std::vector<int64_t> rstart, cstart, rstart2, cstart2, rtot, ctot, nrows_t, ncols_t;
+    block_starts(edr, pat->idr_in, nti, rstart, rtot);
+    block_starts(edc, pat->idc_in, nti, cstart, ctot);
+    block_starts(edr2, pat->idr_out, nto, rstart2, nrows_t);
+    block_starts(edc2, pat->idc_out, nto, cstart2, ncols_t);
+    std::vector<int64_t> noff(static_cast<size_t>(nto), 0);
+    for (int64_t t = 0, run = 0; t < nto; ++t) {
+      shapes[2 * t] = nrows_t[t];
+      shapes[2 * t + 1] = ncols_t[t];
+      noff[t] = run;
+      run += nrows_t[t] * ncols_t[t];
+    }
+    for (int64_t t = 0; t < nti; ++t)
+      FS_CHECK_ARG(in_block[t] >= -1 && in_block[t] < nblocks, "lie plan tables: bad stored block");
+
+    int64_t row0 = 0, ein = 0, eout = 0;
+    int64_t D[FS_LIE_MAX_LEGS], Dn[FS_LIE_MAX_LEGS], so[FS_LIE_MAX_LEGS], sn[FS_LIE_MAX_LEGS];
+    for (int64_t s = 0; s < ns; ++s) {
+      int64_t E = 1;
+      for (int64_t k = 0; k < nd; ++k) {
+        const int64_t i = pat->sig_idx[s * nd + k];
+        FS_CHECK_ARG(i >= 0 && i < pat->nirr[k], "lie plan tables: irrep index out of range");
+        D[k] = degen[doff[k] + i];
+        E *= D[k];
+      }
+      const int64_t ir = pat->ir_in[s], ic = pat->ic_in[s];
+      const int64_t ir2 = pat->ir_out[s], ic2 = pat->ic_out[s];
+      FS_CHECK_ARG(ir >= 0 && ir < static_cast<int64_t>(edr.size()) && ic >= 0 &&
+                       ic < static_cast<int64_t>(edc.size()) && ir2 >= 0 &&
+                       ir2 < static_cast<int64_t>(edr2.size()) && ic2 >= 0 &&
+                       ic2 < static_cast<int64_t>(edc2.size()),
+                   "lie plan tables: irreps combination out of range");
+      const int64_t nin = pat->n_in[s], nout = pat->n_out[s];
+      FS_CHECK_ARG(nin >= 0 && nout >= 0 && row0 + nin + nout <= nrows,
+                   "lie plan tables: row table too short");
+      for (int64_t j = 0; j < nin; ++j, ++ein) {
+        const int64_t t = pat->in_t[ein];
+        FS_CHECK_ARG(t >= 0 && t < nti, "lie plan tables: input target out of range");
+        const int64_t row = rstart[ir * nti + t] + pat->in_a[ein] * edr[ir];
+        const int64_t col = cstart[ic * nti + t] + pat->in_b[ein] * edc[ic];
+        const int64_t b = in_block[t];
+        int64_t *r = rows + 3 * (row0 + j);
+        if (b >= 0) {
+          r[0] = block_off[b] + row * block_rs[b] + col * block_cs[b];
+          r[1] = block_rs[b];
+          r[2] = block_cs[b];
+        } else {
+          r[0] = -1;
+          r[1] = r[2] = 0;
+        }
+      }
+      for (int64_t j = 0; j < nout; ++j, ++eout) {
+        const int64_t t = pat->out_t[eout];
+        FS_CHECK_ARG(t >= 0 && t < nto, "lie plan tables: output target out of range");
+        int64_t *r = rows + 3 * (row0 + nin + j);
+        r[0] = noff[t] + (rstart2[ir2 * nto + t] + pat->out_a[eout] * edr2[ir2]) * ncols_t[t] +
+               cstart2[ic2 * nto + t] + pat->out_b[eout] * edc2[ic2];
+        r[1] = ncols_t[t];
+        r[2] = 1;
+      }
+      // digits: one per NEW leg (the external index runs row-major in the new
+      // order), with its strides on the old and the new side
+      for (int64_t k = nd - 1, run_r = 1, run_c = 1; k >= 0; --k) {  // old strides
+        if (k >= nrr) { so[k] = run_c; run_c *= D[k]; }
+        else { so[k] = run_r; run_r *= D[k]; }
+      }
+      for (int64_t k = 0; k < nd; ++k) Dn[k] = D[pat->axes[k]];
+      for (int64_t k = nd - 1, run_r = 1, run_c = 1; k >= 0; --k) {  // new strides
+        if (k >= nro) { sn[k] = run_c; run_c *= Dn[k]; }
+        else { sn[k] = run_r; run_r *= Dn[k]; }
+      }
+      for (int64_t k = 0; k < nd; ++k) {
+        const int64_t a = pat->axes[k];
+        int64_t *d = digits + 5 * (s * nd + k);
+        d[0] = Dn[k];
+        d[1] = a < nrr ? so[a] : 0;
+        d[2] = a >= nrr ? so[a] : 0;
+        d[3] = k < nro ? sn[k] : 0;
+        d[4] = k >= nro ? sn[k] : 0;
+      }
+      int64_t *tk = tasks + 8 * s;
+      tk[0] = E;
+      tk[1] = pat->w_off[s];
+      tk[2] = nin;
+      tk[3] = nout;
+      tk[4] = pat->n_out_pad[s];
+      tk[5] = row0;
+      tk[6] = nd;
+      tk[7] = s * nd;
+      row0 += nin + nout;
+    }
     return FS_OK;
   });
 }

@@ -25,7 +25,7 @@ from functools import reduce
 
 import numpy as np
 
-from .. import _ops
+from .. import _lib, _ops
 from .abelian_symmetric_tensor import _dense_source
 from .non_abelian_symmetric_tensor import NonAbelianSymmetricTensor
 
@@ -71,8 +71,26 @@ class _PermuteStructure:
     __slots__ = (
         "W", "idc_in", "idc_out", "idr_in", "idr_out", "ic_in", "ic_out", "in_a", "in_b",
         "in_t", "in_targets", "ir_in", "ir_out", "n_in", "n_out", "n_out_pad", "out_a",
-        "out_b", "out_t", "out_targets", "sig_idx", "w_off",
+        "out_b", "out_t", "out_targets", "sig_idx", "w_off", "pattern", "_keep",
     )
+
+    def bind(self, ndim, nrr, nrr_out, axes, nirr):
+        """The C pattern (fs_lie_pattern) over this structure's arrays; the
+        arrays are kept alive with it."""
+        arrs = {
+            "axes": np.asarray(axes, dtype=np.int64), "nirr": np.asarray(nirr, dtype=np.int64),
+            "sig_idx": self.sig_idx, "ir_in": self.ir_in, "ic_in": self.ic_in, "ir_out": self.ir_out,
+            "ic_out": self.ic_out, "n_in": self.n_in, "n_out": self.n_out, "w_off": self.w_off,
+            "n_out_pad": self.n_out_pad, "idr_in": self.idr_in, "idc_in": self.idc_in,
+            "idr_out": self.idr_out, "idc_out": self.idc_out, "in_t": self.in_t, "in_a": self.in_a,
+            "in_b": self.in_b, "out_t": self.out_t, "out_a": self.out_a, "out_b": self.out_b,
+        }
+        arrs = {k: np.ascontiguousarray(v, dtype=np.int64) for k, v in arrs.items()}
+        self._keep = arrs
+        self.pattern = _lib.LiePattern(
+            ndim, nrr, nrr_out, len(self.n_in), len(self.in_targets), len(self.out_targets),
+            *(_lib.i64p(arrs[f]) for f, _ in _lib.LiePattern._fields_[6:]),
+        )
 
 
 class _DenseStructure:
@@ -250,6 +268,7 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         eo = np.concatenate(ent_out, axis=1) if ent_out else np.zeros((3, 0), dtype=np.int64)
         st.in_t, st.in_a, st.in_b = ei
         st.out_t, st.out_a, st.out_b = eo
+        st.bind(ndim, nrr, nrr_out, axes, nper)
         cls._permute_structures[key] = st
         return st
 
@@ -273,13 +292,13 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
     def _permute_plan(self, axes, nrr_out):
         cls = type(self)
         reps = self._row_reps + self._col_reps
-        ndim, nrr = self._ndim, self._nrr
-        irreps_per_leg = tuple(tuple(int(x) for x in r[1]) for r in reps)
-        sig = tuple(bool(s) for s in self._signature)
+        nrr = self._nrr
         axes = tuple(int(a) for a in axes)
+        # the key hashes raw bytes (representations: degeneracies and irreps of
+        # every leg); the irreps tuples are only built for a new plan
         key = (
-            cls, irreps_per_leg, sig, nrr, axes, nrr_out,
-            b"|".join(np.ascontiguousarray(r[0], dtype=np.int64).tobytes() for r in reps),
+            cls, tuple(np.ascontiguousarray(r, dtype=np.int64).tobytes() for r in reps),
+            self._signature.tobytes(), nrr, axes, nrr_out,
             self._block_irreps.tobytes(), self._boffs.tobytes(), self._bshapes.tobytes(),
             self._fortran,
         )
@@ -288,79 +307,17 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
             if hit is not None:
                 cls._plans.move_to_end(key)
                 return hit
+        irreps_per_leg = tuple(tuple(int(x) for x in r[1]) for r in reps)
+        sig = tuple(bool(s) for s in self._signature)
         st = cls._permute_structure(irreps_per_leg, sig, nrr, axes, nrr_out)
-        degens = [np.asarray(r[0], dtype=np.int64) for r in reps]
-        ax = np.array(axes)
-        nsig = st.sig_idx.shape[0]
-        D = np.empty((nsig, ndim), dtype=np.int64)
-        for k in range(ndim):
-            D[:, k] = degens[k][st.sig_idx[:, k]]
-        E = D.prod(axis=1)
-        # old side: elementary block offsets inside each block
-        edr = _outer(degens[:nrr])
-        edc = _outer(degens[nrr:])
-        rstart = _exclusive_cumsum0(edr[:, None] * st.idr_in)
-        cstart = _exclusive_cumsum0(edc[:, None] * st.idc_in)
-        pos = self._present_map(st.in_targets)
+        degen = np.concatenate([np.asarray(r[0], dtype=np.int64) for r in reps])
         rs_b, cs_b = self._block_strides()
-        sig_e = np.repeat(np.arange(nsig), st.n_in)
-        b = pos[st.in_t]
-        ir, ic = st.ir_in[sig_e], st.ic_in[sig_e]
-        row = rstart[ir, st.in_t] + st.in_a * edr[ir]
-        col = cstart[ic, st.in_t] + st.in_b * edc[ic]
-        bb = np.maximum(b, 0)
-        if self._nblocks:
-            in_base = np.where(b >= 0, self._boffs[bb] + row * rs_b[bb] + col * cs_b[bb], -1)
-            in_rs = np.where(b >= 0, rs_b[bb], 0)
-            in_cs = np.where(b >= 0, cs_b[bb], 0)
-        else:
-            in_base = np.full_like(row, -1)
-            in_rs = in_cs = np.zeros_like(row)
-        # new side: all allowed blocks, packed row-major
-        ndeg = [degens[a] for a in axes]
-        edr2 = _outer(ndeg[:nrr_out])
-        edc2 = _outer(ndeg[nrr_out:])
-        shapes = np.stack([(edr2[:, None] * st.idr_out).sum(axis=0), (edc2[:, None] * st.idc_out).sum(axis=0)], axis=1)
-        sizes = shapes[:, 0] * shapes[:, 1]
-        noff = _exclusive_cumsum0(sizes)
-        rstart2 = _exclusive_cumsum0(edr2[:, None] * st.idr_out)
-        cstart2 = _exclusive_cumsum0(edc2[:, None] * st.idc_out)
-        sig_o = np.repeat(np.arange(nsig), st.n_out)
-        ir2, ic2 = st.ir_out[sig_o], st.ic_out[sig_o]
-        t2 = st.out_t
-        out_base = noff[t2] + (rstart2[ir2, t2] + st.out_a * edr2[ir2]) * shapes[t2, 1] + cstart2[ic2, t2] + st.out_b * edc2[ic2]
-        # rows table: per task, its n_in input rows then its n_out output rows
-        nrow = st.n_in + st.n_out
-        row0 = _exclusive_cumsum0(nrow)
-        rows = np.zeros((int(nrow.sum()), 3), dtype=np.int64)
-        li = np.arange(sig_e.size) - np.repeat(_exclusive_cumsum0(st.n_in), st.n_in)
-        pi = row0[sig_e] + li
-        rows[pi, 0], rows[pi, 1], rows[pi, 2] = in_base, in_rs, in_cs
-        lo = np.arange(sig_o.size) - np.repeat(_exclusive_cumsum0(st.n_out), st.n_out)
-        po = row0[sig_o] + st.n_in[sig_o] + lo
-        rows[po, 0], rows[po, 1], rows[po, 2] = out_base, shapes[t2, 1], 1
-        # digits: one per leg in the NEW order (external index is row-major there)
-        s_old_r = _suffix_products(D[:, :nrr])
-        s_old_c = _suffix_products(D[:, nrr:])
-        Dn = D[:, ax]
-        s_new_r = _suffix_products(Dn[:, :nrr_out])
-        s_new_c = _suffix_products(Dn[:, nrr_out:])
-        dig = np.zeros((nsig, ndim, 5), dtype=np.int64)
-        dig[:, :, 0] = Dn
-        for k, a in enumerate(axes):
-            if a < nrr:
-                dig[:, k, 1] = s_old_r[:, a]
-            else:
-                dig[:, k, 2] = s_old_c[:, a - nrr]
-            if k < nrr_out:
-                dig[:, k, 3] = s_new_r[:, k]
-            else:
-                dig[:, k, 4] = s_new_c[:, k - nrr_out]
-        tasks = np.stack(
-            [E, st.w_off, st.n_in, st.n_out, st.n_out_pad, row0, np.full_like(E, ndim), np.arange(nsig) * ndim],
-            axis=1,
+        tasks, rows, dig, shapes = _ops.lie_plan_tables(
+            st.pattern, degen, self._present_map(st.in_targets), self._boffs, rs_b, cs_b,
+            nrows=int(st.n_in.sum() + st.n_out.sum()),
         )
-        plan = _ops.LiePlan(tasks, rows, dig.reshape(-1, 5), st.W, self._data.numel(), int(sizes.sum()))
+        sizes = shapes[:, 0] * shapes[:, 1]
+        plan = _ops.LiePlan(tasks, rows, dig, st.W, self._data.numel(), int(sizes.sum()))
         hit = (plan, shapes, st.out_targets)
         with cls._plans_lock:
             cls._plans[key] = hit

@@ -205,6 +205,52 @@ int fs_lie_plan_check(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
                       const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
                       int64_t nw, int64_t src_size, int64_t dst_size);
 
+/* Tables of a structural-transform plan for one permute, from the irreps-only
+ * pattern of the permute and the degeneracies of the legs. Replaces the
+ * per-call offset computation of the reference's numba permute
+ * (lie_group_symmetric_tensor.py:96-265: the in-block row / column of every
+ * elementary block and internal index, old and new side) by one host pass.
+ *
+ * The pattern (built once per irreps pattern; every array int64, row-major):
+ *   ndim, nrr, nrr_out      legs, row legs before and after the permute
+ *   ntasks                  elementary blocks (one irrep per leg) with data
+ *   axes[ndim]              new leg k is old leg axes[k]
+ *   nirr[ndim]              irreps per old leg
+ *   sig_idx[ntasks][ndim]   irrep index of each old leg of a task
+ *   ir_in, ic_in [ntasks]   row / column irreps combination (row-major over the
+ *                           old row / column legs); ir_out, ic_out on the new side
+ *   n_in, n_out, w_off, n_out_pad [ntasks]   as in fs_lie_task
+ *   nt_in, nt_out           target blocks of the old / new side
+ *   idr_in[rc_in][nt_in], idc_in[cc_in][nt_in]   internal degeneracy of each
+ *                           row / column combination in each target block
+ *                           (rc_in = prod of nirr over the old row legs, ...);
+ *                           idr_out, idc_out on the new side
+ *   in_t, in_a, in_b [sum n_in]      per input row of every task, in task order:
+ *                           target block, internal row, internal column;
+ *   out_t, out_a, out_b [sum n_out]  the same for the output rows
+ * degen: the degeneracies of the old legs, concatenated (nirr[k] each).
+ * in_block[nt_in]: stored block holding each old target (-1: absent, its rows
+ * read as zero); block_off / block_rs / block_cs [nblocks]: offset and strides
+ * of the stored blocks. The new side is packed row-major in target order.
+ * Outputs: tasks [ntasks][8], rows [sum(n_in + n_out)][3],
+ * digits [ntasks * ndim][5] (fs_lie_task / fs_lie_row / fs_lie_digit layouts)
+ * and the new block shapes [nt_out][2]. ndegen and nrows are the lengths of
+ * degen and of rows (in entries); the other arrays are sized from the pattern
+ * and every index is checked against it. */
+typedef struct fs_lie_pattern {
+  int64_t ndim, nrr, nrr_out, ntasks, nt_in, nt_out;
+  const int64_t *axes, *nirr, *sig_idx;
+  const int64_t *ir_in, *ic_in, *ir_out, *ic_out;
+  const int64_t *n_in, *n_out, *w_off, *n_out_pad;
+  const int64_t *idr_in, *idc_in, *idr_out, *idc_out;
+  const int64_t *in_t, *in_a, *in_b, *out_t, *out_a, *out_b;
+} fs_lie_pattern;
+
+int fs_lie_plan_tables(const fs_lie_pattern *pattern, int64_t ndegen, const int64_t *degen,
+                       const int64_t *in_block, int64_t nblocks, const int64_t *block_off,
+                       const int64_t *block_rs, const int64_t *block_cs, int64_t *tasks,
+                       int64_t nrows, int64_t *rows, int64_t *digits, int64_t *shapes);
+
 /* Work of one execution: flops (2 * sum n_in*n_out*E) and coefficients read
  * and written (sum (n_in + n_out) * E). */
 int fs_lie_plan_info(const fs_lie_plan *plan, int64_t *ntiles, double *flops, double *coeffs);
