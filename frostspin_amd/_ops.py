@@ -149,6 +149,38 @@ def download(t):
     return t.detach().to("cpu").numpy()
 
 
+def download_start(t):
+    """Start a device -> host copy on the current stream without waiting: the
+    returned host tensor (pinned) holds the values once the stream has been
+    synchronized (wait_current / stream.synchronize). Several readbacks then
+    share one host wait. Host tensors are returned as a copy."""
+    t = t.detach()
+    if not t.is_cuda:
+        return t.clone()
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
+    return h
+
+
+def wait_current():
+    """Host waits for the work queued so far on the current stream."""
+    if device().type == "cuda":
+        torch.cuda.current_stream().synchronize()
+
+
+def upload_async(arr, dtype=DTYPE):
+    """Host array -> new 1-D device tensor without a host wait: staged through
+    pinned memory, copied on the current stream (ordered before the kernels
+    launched after it)."""
+    a = torch.from_numpy(np.ascontiguousarray(arr).reshape(-1))
+    dev = device()
+    if dev.type != "cuda":
+        return a.to(dtype=dtype, copy=True)
+    h = torch.empty(a.shape, dtype=dtype, pin_memory=True)
+    h.copy_(a)
+    return h.to(dev, non_blocking=True)
+
+
 # ---------------------------------------------------------------------------
 # permute plans
 # ---------------------------------------------------------------------------

@@ -116,7 +116,10 @@ def construct_projectors_batch(
     dense_ids = [i for i in range(n) if dense[i]]
     rand_ids = [i for i in range(n) if not dense[i]]
     U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
-    dense_done = None
+    # every singular value the truncation needs reaches the host in one wait per
+    # stream: the readbacks are started where the values are produced and read
+    # after both streams have been waited for
+    dense_done = rand_done = None
     if dense_ids:
         # the exact SVDs of the small sectors are independent of the randomized
         # chain of the large ones: on a GPU they run on a side stream meanwhile
@@ -126,22 +129,29 @@ def construct_projectors_batch(
             side.wait_stream(main)
             with torch.cuda.stream(side):
                 U, S, s_off, VT, finish = bl.svd_async(M.select(dense_ids))
+                s_h = _ops.download_start(S)
             for t in (U.buf, S, VT.buf):
                 t.record_stream(main)
         else:
             U, S, s_off, VT, finish = bl.svd_async(M.select(dense_ids))
-        dense_done = (U, VT, finish, side)
+            s_h = _ops.download_start(S)
+        dense_done = (U, VT, finish, s_h, side)
     if rand_ids:
-        U, S, s_off, s_list, VT = bl.rsvd(M.select(rand_ids), k_need[rand_ids], seed=rsvd_seed)
+        rand_done = bl.rsvd_async(M.select(rand_ids), k_need[rand_ids], seed=rsvd_seed)
+    _ops.wait_current()
+    if rand_done is not None:
+        U, S, s_off, VT, finish = rand_done
+        s_list = finish()
         for j, i in enumerate(rand_ids):
             U_of[i] = (U, j)
             VT_of[i] = (VT, j)
             s_of[i] = s_list[j]
     if dense_done is not None:
-        U, VT, finish, side = dense_done
+        U, VT, finish, s_h, side = dense_done
         if side is not None:
+            side.synchronize()
             torch.cuda.current_stream().wait_stream(side)
-        s_list = finish()
+        s_list = finish(s_h)
         for j, i in enumerate(dense_ids):
             U_of[i] = (U, j)
             VT_of[i] = (VT, j)
@@ -166,7 +176,7 @@ def construct_projectors_batch(
     w_off = np.zeros((len(non_empty),), dtype=np.int64)
     if len(non_empty) > 1:
         w_off[1:] = np.cumsum(cuts[non_empty])[:-1]
-    w_dev = _ops.upload(w_host)
+    w_dev = _ops.upload_async(w_host)
 
     # Us = U[:, :cut] * s^-1/2 and Vs = s^-1/2 * VT[:cut, :] per sector (grouped by source buffer)
     kept = [int(cuts[i]) for i in non_empty]

@@ -153,8 +153,13 @@ def svd_async(A, *, max_sweeps=60):
     ]
     _ops.batched_svd(descs, A.buf, U.buf, S, VT.buf, work, max_sweeps=max_sweeps)
 
-    def finish():
-        s_host = _ops.download(S) if S.numel() else np.zeros((0,))
+    def finish(host=None):
+        """Host singular values per matrix; `host` is a readback of S already
+        started (download_start) and waited for."""
+        if host is not None:
+            s_host = host.numpy()
+        else:
+            s_host = _ops.download(S) if S.numel() else np.zeros((0,))
         return [s_host[s_off[i] : s_off[i] + q[i]] for i in range(len(A))]
 
     return U, S, s_off, VT, finish
@@ -297,6 +302,17 @@ def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
     (no host synchronisation inside the chain); the statuses are read once with
     the singular values, and any matrix whose factorisation broke down is redone
     with Householder QR (robust=True) and spliced into the results."""
+    U, S, s_off, VT, finish = rsvd_async(A, ks, oversampling=oversampling, power_iters=power_iters,
+                                         seed=seed, robust=robust)
+    _ops.wait_current()
+    return U, S, s_off, finish(), VT
+
+
+def rsvd_async(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
+    """rsvd() with its one host readback (singular values and CholeskyQR
+    statuses) started but not waited for: returns U, S, s_off, VT and a
+    callable giving the host singular values once the current stream has been
+    synchronized (it splices the redone matrices, if any, into U, S and VT)."""
     ks = np.asarray(ks, dtype=np.int64)
     mins = np.minimum(A.rows, A.cols)
     p = np.maximum(10, ks // 2) if oversampling is None else np.full_like(ks, oversampling)
@@ -319,14 +335,21 @@ def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
         Q2, R2 = qr(B.T())
     else:
         Q2, R2 = cholqr(B.T(), want_r=True, statuses=statuses)
-    Ur, S, s_off, s_list, Vrt = svd(R2.T())
+    Ur, S, s_off, Vrt, finish_small = svd_async(R2.T())
     U = gemm(Q, Ur)  # m x l
     VT = gemm(Vrt, Q2.T())  # l x n
-    if statuses:
-        failed = _ops.download(torch.stack(statuses)).any(axis=0).nonzero()[0]
-        if failed.size:
-            _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed)
-    return U, S, s_off, s_list, VT
+    s_h = _ops.download_start(S)
+    st_h = _ops.download_start(torch.stack(statuses)) if statuses else None
+
+    def finish():
+        s_list = finish_small(s_h)
+        if st_h is not None:
+            failed = st_h.numpy().any(axis=0).nonzero()[0]
+            if failed.size:
+                _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed)
+        return s_list
+
+    return U, S, s_off, VT, finish
 
 
 def _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed):
