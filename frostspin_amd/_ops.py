@@ -400,9 +400,10 @@ class LiePlan:
 
 
 def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
-    """groups: sequence of 11-tuples (m, n, k, a_off, a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld).
-    A, B, C: one device buffer each, or a list with one buffer per group (the
-    groups then read and write blocks of several tensors in one launch)."""
+    """groups: (n, 11) int64 array, or a sequence of 11-tuples, (m, n, k, a_off,
+    a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld). A, B, C: one device buffer
+    each, or a list with one buffer per group (the groups then read and write
+    blocks of several tensors in one launch)."""
     n = len(groups)
     if n == 0:
         return
@@ -418,18 +419,57 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
             for a, b, c, gs in by.values():
                 grouped_gemm(gs, a, b, c, alpha=alpha, beta=beta, split=True)
             return
-    garr = np.array(groups, dtype=np.int64).reshape(n, 11)  # fs_gemm_group layout: 11 int64
+    # fs_gemm_group layout: 11 int64 per group
+    garr = np.ascontiguousarray(groups, dtype=np.int64).reshape(n, 11)
     arr = garr.ctypes.data_as(ctypes.POINTER(_lib.GemmGroup))
     flop = 2.0 * float((garr[:, 0].astype(np.float64) * garr[:, 1] * garr[:, 2]).sum())
     # the library launches one kernel per 32 groups that hold a non-empty group
     if multi:
-        bases = [(_lib.c_vp * n)(*[_ptr(t) for t in lst]) for lst in (As, Bs, Cs)]
+        ptrs = {}
+        bases = [
+            (_lib.c_vp * n)(*[ptrs[id(t)] if id(t) in ptrs else ptrs.setdefault(id(t), _ptr(t)) for t in lst])
+            for lst in (As, Bs, Cs)
+        ]
         with _timed("gemm", flop, lambda: lib.fs_gemm_last_dispatches()):
             check(lib.fs_grouped_gemm_f64_multi(n, arr, *bases, float(alpha), float(beta), _stream()))
         return
     fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
     with _timed("gemm_split" if split else "gemm", flop, lambda: lib.fs_gemm_last_dispatches()):
         check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
+
+
+# ---------------------------------------------------------------------------
+# descriptor tables
+# ---------------------------------------------------------------------------
+_DESC_DTYPES = {}
+
+
+def _desc_table(ctype, descs):
+    """Descriptors as one numpy structured array laid out like `ctype` (all
+    fields 8 bytes: c_int64 or c_double): one conversion per field instead of
+    one ctypes object per descriptor. Returns (table, ctypes pointer)."""
+    dt = _DESC_DTYPES.get(ctype)
+    if dt is None:
+        dt = np.dtype([(f, "<i8" if t is ctypes.c_int64 else "<f8") for f, t in ctype._fields_])
+        assert dt.itemsize == ctypes.sizeof(ctype)
+        _DESC_DTYPES[ctype] = dt
+    if isinstance(descs, np.ndarray) and descs.dtype == dt:
+        table = descs
+    elif isinstance(descs, dict):  # one array (or scalar) per field
+        n = max(np.size(v) for v in descs.values())
+        table = np.empty((n,), dtype=dt)
+        for name in dt.names:
+            table[name] = descs[name]
+    else:
+        cols = descs.T if isinstance(descs, np.ndarray) else list(zip(*descs))
+        table = np.empty((len(descs),), dtype=dt)
+        for name, col in zip(dt.names, cols, strict=True):
+            table[name] = col
+    return table, table.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _ndesc(descs):
+    return max(np.size(v) for v in descs.values()) if isinstance(descs, dict) else len(descs)
 
 
 # ---------------------------------------------------------------------------
@@ -440,12 +480,13 @@ SVD_MAX_MIN_DIM = 1024
 
 def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
     """descs: sequence of 9-tuples (m, n, a_off, a_rs, a_cs, u_off, s_off, vt_off, work_off)."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.SvdDesc * n)(*[_lib.SvdDesc(*map(int, d)) for d in descs])
+    tab, arr = _desc_table(_lib.SvdDesc, descs)
     # work model: one Jacobi sweep ~ 3 q^2 p flop (dot products + rotations)
-    flop = sum(3.0 * min(d[0], d[1]) ** 2 * max(d[0], d[1]) for d in descs)
+    mn = np.minimum(tab["m"], tab["n"]).astype(np.float64)
+    flop = float((3.0 * mn * mn * np.maximum(tab["m"], tab["n"])).sum())
     with _timed("svd", flop):
         check(
             lib.fs_batched_svd_f64(
@@ -457,22 +498,23 @@ def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
 def batched_qr(descs, A, Q, R, work):
     """descs: sequence of 8-tuples (m, n, a_off, a_rs, a_cs, q_off, r_off, work_off);
     R may be None (only Q is formed)."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.QrDesc * n)(*[_lib.QrDesc(*map(int, d)) for d in descs])
-    flop = sum(4.0 * d[0] * min(d[0], d[1]) ** 2 for d in descs)  # geqrf + orgqr, m >= n
+    tab, arr = _desc_table(_lib.QrDesc, descs)
+    mn = np.minimum(tab["m"], tab["n"]).astype(np.float64)
+    flop = float((4.0 * tab["m"] * mn * mn).sum())  # geqrf + orgqr, m >= n
     with _timed("qr", flop):
         check(lib.fs_batched_qr_f64(n, arr, _ptr(A), _ptr(Q), _ptr(R), _ptr(work), _stream()))
 
 
 def batched_cholesky(descs, G, R, status):
     """descs: sequence of (l, g_off, r_off, shift_rel); status: device int32 (len(descs))."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.CholDesc * n)(*[_lib.CholDesc(int(d[0]), int(d[1]), int(d[2]), float(d[3])) for d in descs])
-    flop = sum(d[0] ** 3 / 3.0 for d in descs)
+    tab, arr = _desc_table(_lib.CholDesc, descs)
+    flop = float((tab["l"].astype(np.float64) ** 3 / 3.0).sum())
     with _timed("cholesky", flop):
         check(lib.fs_batched_cholesky_f64(n, arr, _ptr(G), _ptr(R), _ptr(status), _stream()))
 
@@ -484,13 +526,11 @@ def batched_cholinv(descs, G, R, Rinv, status):
     """descs: (l, g_off, g_stride, n_part, r_off, rinv_off, shift_rel): G is the sum
     of n_part l x l partials at g_off + p*g_stride; R may be None. Writes the upper
     Cholesky factor (if R) and its inverse; status: device int32 (len(descs))."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.CholInvDesc * n)(
-        *[_lib.CholInvDesc(*(int(x) for x in d[:6]), float(d[6])) for d in descs]
-    )
-    flop = sum(d[0] ** 3 / 3.0 + d[0] ** 3 / 3.0 for d in descs)
+    tab, arr = _desc_table(_lib.CholInvDesc, descs)
+    flop = float((2.0 * tab["l"].astype(np.float64) ** 3 / 3.0).sum())
     with _timed("cholinv", flop):
         check(lib.fs_batched_cholinv_f64(
             n, arr, _ptr(G), None if R is None else _ptr(R), _ptr(Rinv), _ptr(status), _stream()))
@@ -498,11 +538,13 @@ def batched_cholinv(descs, G, R, Rinv, status):
 
 def batched_trsm_right_upper(descs, Y, R, O):
     """descs: sequence of (m, l, y_off, y_rs, y_cs, r_off, o_off): O = Y R^-1."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.TrsmDesc * n)(*[_lib.TrsmDesc(*map(int, d)) for d in descs])
-    flop = sum(float(d[0]) * d[1] * d[1] for d in descs)
+    tab, arr = _desc_table(_lib.TrsmDesc, descs)
+    f0 = tab[tab.dtype.names[0]].astype(np.float64)
+    f1 = tab[tab.dtype.names[1]].astype(np.float64)
+    flop = float((f0 * f1 * f1).sum())
     with _timed("trsm", flop):
         check(lib.fs_batched_trsm_right_upper_f64(n, arr, _ptr(Y), _ptr(R), _ptr(O), _stream()))
 
@@ -543,13 +585,13 @@ def axpby_(alpha, x, beta, y):
 def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
     """descs: sequence of 13-tuples
     (rows, cols, s_off, s_rs, s_cs, d_off, d_rs, d_cs, rw_off, cw_off, ci_off, alpha, beta)."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.Copy2dDesc * n)(
-        *[_lib.Copy2dDesc(*(int(v) for v in d[:11]), float(d[11]), float(d[12])) for d in descs]
-    )
-    nbytes = sum((16.0 if d[12] == 0.0 else 24.0) * d[0] * d[1] for d in descs)
+    tab, arr = _desc_table(_lib.Copy2dDesc, descs)
+    names = tab.dtype.names
+    per = np.where(tab[names[12]] == 0.0, 16.0, 24.0)
+    nbytes = float((per * tab[names[0]] * tab[names[1]]).sum())
     with _timed("copy2d", nbytes):
         check(
             lib.fs_batched_copy2d_f64(
@@ -563,16 +605,17 @@ def gather2d(descs, idx, w, src, dst):
     (rows, cols, src_rows, src_cols, s_off, s_rs, s_cs, d_off, d_rs, d_cs, ri_off,
     rw_off, ci_off, cw_off, beta); idx (int64) and w (float64) are host arrays,
     validated against the buffer sizes before upload."""
-    n = len(descs)
+    n = _ndesc(descs)
     if n == 0:
         return
-    arr = (_lib.Gather2dDesc * n)(*[_lib.Gather2dDesc(*(int(x) for x in d[:14]), float(d[14])) for d in descs])
+    tab, arr = _desc_table(_lib.Gather2dDesc, descs)
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     w = np.ascontiguousarray(w, dtype=np.float64)
     check(lib.fs_gather2d_check(n, arr, _lib.i64p(idx), idx.size, w.size, src.numel(), dst.numel()))
     idx_dev = upload(idx, dtype=torch.int64)
     w_dev = upload(w)
-    moved = sum(float(d[0]) * float(d[1]) for d in descs)
+    names = tab.dtype.names
+    moved = float((tab[names[0]].astype(np.float64) * tab[names[1]]).sum())
     with _timed("gather2d", 16.0 * moved):
         check(lib.fs_batched_gather2d_f64(n, arr, _ptr(idx_dev), _ptr(w_dev), _ptr(src), _ptr(dst), _stream()))
 

@@ -51,12 +51,14 @@ class Mats:
 
     @classmethod
     def from_tensor_blocks(cls, st, ids):
-        ids = list(ids)
-        off = np.array([st._boffs[b] for b in ids], dtype=np.int64)
-        rows = np.array([st._bshapes[b, 0] for b in ids], dtype=np.int64)
-        cols = np.array([st._bshapes[b, 1] for b in ids], dtype=np.int64)
-        strides = np.array([st._strides(b) for b in ids], dtype=np.int64).reshape(-1, 2)
-        return cls(st._data, off, rows, cols, strides[:, 0].copy(), strides[:, 1].copy())
+        ids = np.asarray(ids, dtype=np.int64).reshape(-1)
+        rows = st._bshapes[ids, 0].copy()
+        cols = st._bshapes[ids, 1].copy()
+        if st._fortran:
+            rs, cs = np.ones_like(rows), rows.copy()
+        else:
+            rs, cs = cols.copy(), np.ones_like(cols)
+        return cls(st._data, st._boffs[ids].copy(), rows, cols, rs, cs)
 
     @classmethod
     def concat(cls, parts):
@@ -90,14 +92,9 @@ def gemm_into(A, B, C, *, alpha=1.0, split=False):
     """C_i = alpha * A_i @ B_i into given row-major outputs (C_i row stride c.rs,
     column stride 1); operands and outputs may span several buffers."""
     assert len(A) == len(B) == len(C)
-    assert np.array_equal(A.cols, B.rows)
-    assert np.array_equal(C.rows, A.rows) and np.array_equal(C.cols, B.cols)
+    assert (A.cols == B.rows).all() and (C.rows == A.rows).all() and (C.cols == B.cols).all()
     assert (C.cs[C.rows * C.cols > 0] == 1).all()
-    groups = [
-        (A.rows[i], B.cols[i], A.cols[i], A.off[i], A.rs[i], A.cs[i], B.off[i], B.rs[i], B.cs[i],
-         C.off[i], C.rs[i])
-        for i in range(len(A))
-    ]
+    groups = np.stack([A.rows, B.cols, A.cols, A.off, A.rs, A.cs, B.off, B.rs, B.cs, C.off, C.rs], axis=1)
     if any(isinstance(x.buf, list) for x in (A, B, C)):
         _ops.grouped_gemm(groups, A.bufs(), B.bufs(), C.bufs(), alpha=alpha, split=split)
     elif A.buf is B.buf or A.buf.data_ptr() == B.buf.data_ptr():
@@ -185,6 +182,14 @@ def qr(A, *, want_r=True):
     return Q, R
 
 
+def off_parts(counts):
+    """Exclusive running sum of counts (start of each matrix's parts)."""
+    out = np.zeros_like(counts)
+    if counts.size > 1:
+        out[1:] = np.cumsum(counts[:-1])
+    return out
+
+
 GRAM_ROWS_PER_PART = 192
 GRAM_MAX_PARTS = 16
 
@@ -201,15 +206,15 @@ def gram(Y):
     if sizes.size > 1:
         off[1:] = np.cumsum(sizes[:-1])
     buf = _ops.empty(int(sizes.sum()))
-    groups = []
-    for i in range(len(Y)):
-        m, l = int(Y.rows[i]), int(Y.cols[i])
-        bounds = np.linspace(0, m, int(nparts[i]) + 1).astype(np.int64)
-        for p in range(int(nparts[i])):
-            r0, r1 = int(bounds[p]), int(bounds[p + 1])
-            a_off = Y.off[i] + r0 * Y.rs[i]
-            groups.append((l, l, r1 - r0, a_off, Y.cs[i], Y.rs[i], a_off, Y.rs[i], Y.cs[i],
-                           off[i] + p * l2[i], l))
+    # part p of matrix i: rows floor(p m / n) .. floor((p + 1) m / n)
+    mat = np.repeat(np.arange(len(Y)), nparts)
+    part = np.arange(mat.size) - np.repeat(off_parts(nparts), nparts)
+    m, npi = Y.rows[mat], nparts[mat]
+    r0, r1 = part * m // npi, (part + 1) * m // npi
+    a_off = Y.off[mat] + r0 * Y.rs[mat]
+    l = Y.cols[mat]
+    groups = np.stack([l, l, r1 - r0, a_off, Y.cs[mat], Y.rs[mat], a_off, Y.rs[mat], Y.cs[mat],
+                       off[mat] + part * l2[mat], l], axis=1)
     A = Y.buf
     _ops.grouped_gemm(groups, A, A, buf)
     return Mats(buf, off, Y.cols.copy(), Y.cols.copy(), Y.cols.copy(), np.ones_like(Y.cols)), nparts
@@ -233,18 +238,17 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False, statuses=None):
     for p in range(passes):
         G, nparts = gram(cur)
         status = _ops.zeros(len(cur), dtype=torch.int32)
-        shift = [
-            11.0 * (float(cur.rows[i]) * cur.cols[i] + cur.cols[i] * (cur.cols[i] + 1)) * eps
-            if (shifted and p == 0) else 0.0
-            for i in range(len(cur))
-        ]
+        cols = cur.cols.astype(np.float64)
+        if shifted and p == 0:
+            shift = 11.0 * (cur.rows * cols + cols * (cols + 1.0)) * eps
+        else:
+            shift = np.zeros_like(cols)
         R = Mats.packed(cur.cols, cur.cols) if want_r else None
         if len(cur) and cur.cols.max() <= _ops.CHOLINV_MAX_ORDER:
             Rinv = Mats.packed(cur.cols, cur.cols)
             _ops.batched_cholinv(
-                [(cur.cols[i], G.off[i], cur.cols[i] ** 2, nparts[i], 0 if R is None else R.off[i],
-                  Rinv.off[i], shift[i])
-                 for i in range(len(cur))],
+                {"l": cur.cols, "g_off": G.off, "g_stride": cur.cols * cur.cols, "n_part": nparts,
+                 "r_off": 0 if R is None else R.off, "rinv_off": Rinv.off, "shift_rel": shift},
                 G.buf, None if R is None else R.buf, Rinv.buf, status,
             )
             out = gemm(cur, Rinv)
@@ -254,7 +258,7 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False, statuses=None):
             if np.any(nparts > 1):
                 G = gemm(cur.T(), cur)
             _ops.batched_cholesky(
-                [(cur.cols[i], G.off[i], R.off[i], shift[i]) for i in range(len(cur))], G.buf, R.buf, status
+                {"l": cur.cols, "g_off": G.off, "r_off": R.off, "shift_rel": shift}, G.buf, R.buf, status
             )
             out = Mats.packed(cur.rows, cur.cols)
             _ops.batched_trsm_right_upper(

@@ -395,24 +395,23 @@ class SymmetricTensor:
         assert type(self) is type(other)
         assert self._shape[self._nrr :] == other._shape[: other._nrr]
         assert (self._signature[self._nrr :] ^ other._signature[: other._nrr]).all()
-        ia = {int(r): i for i, r in enumerate(self._block_irreps)}
-        pairs = [(ia[int(r)], j) for j, r in enumerate(other.block_irreps) if int(r) in ia]
-        pairs.sort()
-        nb = len(pairs)
-        shapes = np.empty((nb, 2), dtype=np.int64)
-        irreps = np.empty((nb,), dtype=self._block_irreps.dtype)
-        for k, (i, j) in enumerate(pairs):
-            shapes[k] = (self._bshapes[i, 0], other._bshapes[j, 1])
-            irreps[k] = self._block_irreps[i]
+        # blocks present on both sides (block irreps are sorted), in irrep order
+        ia, ib = self._block_irreps, other._block_irreps
+        pos = np.minimum(np.searchsorted(ia, ib), max(ia.size - 1, 0))
+        match = (ia[pos] == ib) if ia.size else np.zeros(ib.shape, dtype=bool)
+        i_idx, j_idx = pos[match], np.flatnonzero(match)
+        m, kk = self._bshapes[i_idx, 0], self._bshapes[i_idx, 1]
+        n = other._bshapes[j_idx, 1]
+        shapes = np.stack([m, n], axis=1)
+        irreps = ia[i_idx]
         offs, total = _packed_offsets(shapes)
         out = _ops.empty(total)
-        groups = []
-        for k, (i, j) in enumerate(pairs):
-            m, kk = self._bshapes[i]
-            n = other._bshapes[j, 1]
-            ars, acs = self._strides(i)
-            brs, bcs = other._strides(j)
-            groups.append((m, n, kk, self._boffs[i], ars, acs, other._boffs[j], brs, bcs, offs[k], n))
+        kb = other._bshapes[j_idx, 0]
+        assert (kb == kk).all()
+        one = np.ones_like(m)
+        ars, acs = (one, m) if self._fortran else (kk, one)
+        brs, bcs = (one, kb) if other._fortran else (n, one)
+        groups = np.stack([m, n, kk, self._boffs[i_idx], ars, acs, other._boffs[j_idx], brs, bcs, offs, n], axis=1)
         _ops.grouped_gemm(groups, self._data, other._data, out, split=_precision_split())
         signature = np.empty((self._nrr + other._ndim - other._nrr,), dtype=bool)
         signature[: self._nrr] = self._signature[: self._nrr]

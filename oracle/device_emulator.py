@@ -106,8 +106,19 @@ def batched_qr(descs, A, Q, R, work):
             rb[ro : ro + q * n] = rr.reshape(-1)
 
 
+def _desc_rows(descs, names):
+    """Descriptor tuples from either form the product passes: a sequence of
+    tuples, or a dict of per-field columns (scalars broadcast)."""
+    if not isinstance(descs, dict):
+        return descs
+    n = max(np.size(v) for v in descs.values())
+    cols = [np.broadcast_to(np.asarray(descs[k]), (n,)) for k in names]
+    return [tuple(c[i].item() for c in cols) for i in range(n)]
+
+
 def batched_cholesky(descs, G, R, status):
     g, r, st = _np(G), _np(R), _np(status)
+    descs = _desc_rows(descs, ("l", "g_off", "r_off", "shift_rel"))
     for b, (l, go, ro, shift_rel) in enumerate(descs):
         gm = g[go : go + l * l].reshape(l, l)
         gm = np.triu(gm) + np.triu(gm, 1).T
@@ -124,6 +135,7 @@ def batched_cholesky(descs, G, R, status):
 def batched_cholinv(descs, G, R, Rinv, status):
     g, ri, st = _np(G), _np(Rinv), _np(status)
     r = None if R is None else _np(R)
+    descs = _desc_rows(descs, ("l", "g_off", "g_stride", "n_part", "r_off", "rinv_off", "shift_rel"))
     for b, (l, go, gs, npart, ro, rio, shift_rel) in enumerate(descs):
         assert 1 <= l <= _ops.CHOLINV_MAX_ORDER
         gm = sum(g[go + p * gs : go + p * gs + l * l].reshape(l, l) for p in range(npart))
