@@ -97,6 +97,10 @@ class CholInvDesc(ctypes.Structure):
     ]
 
 
+class TriInvDesc(ctypes.Structure):
+    _fields_ = [("l", c_i64), ("r_off", c_i64), ("x_off", c_i64)]
+
+
 class TrsmDesc(ctypes.Structure):
     _fields_ = [
         ("m", c_i64),
@@ -214,6 +218,7 @@ _SIGNATURES = {
         c_int,
         [c_int, ctypes.POINTER(CholDesc), c_vp, c_vp, c_vp, c_vp],
     ),
+    "fs_batched_triinv_f64": (c_int, [c_int, ctypes.POINTER(TriInvDesc), c_vp, c_vp, c_vp, c_vp]),
     "fs_batched_cholinv_f64": (
         c_int,
         [c_int, ctypes.POINTER(CholInvDesc), c_vp, c_vp, c_vp, c_vp, c_vp],

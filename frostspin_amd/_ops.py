@@ -533,7 +533,20 @@ def batched_cholinv(descs, G, R, Rinv, status):
     flop = float((2.0 * tab["l"].astype(np.float64) ** 3 / 3.0).sum())
     with _timed("cholinv", flop):
         check(lib.fs_batched_cholinv_f64(
-            n, arr, _ptr(G), None if R is None else _ptr(R), _ptr(Rinv), _ptr(status), _stream()))
+            n, arr, _ptr(G), None if R is None else _ptr(R), None if Rinv is None else _ptr(Rinv),
+            _ptr(status), _stream()))
+
+
+def batched_triinv(descs, R, X, status=None):
+    """descs: (l, r_off, x_off) per matrix (or a dict of columns): X = R^-1 for
+    upper triangular R; matrices whose status entry is nonzero get NaN."""
+    n = _ndesc(descs)
+    if n == 0:
+        return
+    tab, arr = _desc_table(_lib.TriInvDesc, descs)
+    flop = float((tab["l"].astype(np.float64) ** 3 / 3.0).sum())
+    with _timed("triinv", flop):
+        check(lib.fs_batched_triinv_f64(n, arr, _ptr(R), _ptr(X), _ptr(status), _stream()))
 
 
 def batched_trsm_right_upper(descs, Y, R, O):

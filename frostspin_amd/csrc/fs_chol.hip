@@ -481,6 +481,10 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
       }
     }
   }
+  if (!bt.Rinv) {  // factor only
+    if (threadIdx.x == 0) bt.status[blockIdx.x] = 0;
+    return;
+  }
   __syncthreads();  // udiag complete, row buffers free
   double x[kC4P][kC4Q];
 #pragma unroll
@@ -564,6 +568,141 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
   if (threadIdx.x == 0) bt.status[blockIdx.x] = 0;
 }
 
+// Inverse X = R^-1 of one upper triangular l x l matrix (l <= 128) per
+// workgroup, by block rows of 16 from the bottom: with B = I,
+//   X_I = D_I B_I  (D_I = R_II^-1),  then  B_K -= R_KI X_I  for K < I.
+// The diagonal-block inverses D_I are formed first (one thread per column of
+// each, back substitution), so the sequential part is nb = ceil(l / 16) steps
+// of two barriers each, instead of one barrier per row or four rows. 512
+// threads as 16 x 32, thread (ty, tx) owns rows ty + 16p and columns tx + 32q
+// of B / X in registers; block row I of B, the new X_I and the column block
+// R_:I travel through double-buffered LDS. R is padded with the identity up to
+// a multiple of 16. A matrix whose factorisation failed (status != 0) gets NaN.
+constexpr int kTiBlk = 16;
+constexpr int kTiRows = 16, kTiCols = 32;
+constexpr int kTiThreads = kTiRows * kTiCols;        // 512
+constexpr int kTiP = kCholInvMaxOrder / kTiRows;      // 8 rows per thread
+constexpr int kTiQ = kCholInvMaxOrder / kTiCols;      // 4 columns per thread
+constexpr int kTiNb = kCholInvMaxOrder / kTiBlk;      // 8 block rows
+
+struct TriInvDev {
+  int32_t l, pad;
+  int64_t r_off, x_off;
+};
+
+struct TriInvBatch {
+  const double *R;
+  double *X;
+  const int32_t *status;  // factorisation status per matrix (may be null)
+  int32_t nmat, pad;
+  TriInvDev d[kMaxMat];
+};
+
+__global__ void __launch_bounds__(kTiThreads) triinv_kernel(TriInvBatch bt) {
+  const TriInvDev D = bt.d[blockIdx.x];
+  const int l = D.l;
+  const double *R = bt.R + D.r_off;
+  double *X = bt.X + D.x_off;
+  const int tx = threadIdx.x % kTiCols;
+  const int ty = threadIdx.x / kTiCols;
+  if (bt.status && bt.status[blockIdx.x] != 0) {
+    for (int e = threadIdx.x; e < l * l; e += kTiThreads) X[e] = __builtin_nan("");
+    return;
+  }
+  const int nb = (l + kTiBlk - 1) / kTiBlk;
+  __shared__ double diag[kTiNb][kTiBlk][kTiBlk + 1];             // R_II, then D_I
+  __shared__ double rcol[2][kCholInvMaxOrder][kTiBlk + 1];       // R[k][16 I + t], k < 16 I
+  __shared__ double brow[2][kTiBlk][kCholInvMaxOrder];           // block row I of B
+  __shared__ double xrow[2][kTiBlk][kCholInvMaxOrder];           // X_I
+  auto r_at = [&](int i, int k) -> double {
+    if (i < l && k < l) return R[static_cast<int64_t>(i) * l + k];
+    return i == k ? 1.0 : 0.0;
+  };
+  for (int e = threadIdx.x; e < nb * kTiBlk * kTiBlk; e += kTiThreads) {
+    const int I = e / (kTiBlk * kTiBlk), r = (e / kTiBlk) % kTiBlk, t = e % kTiBlk;
+    diag[I][r][t] = (t >= r) ? r_at(I * kTiBlk + r, I * kTiBlk + t) : 0.0;
+  }
+  __syncthreads();
+  // D_I column c: R_II d = e_c by back substitution (thread (I, c))
+  const int di = threadIdx.x / kTiBlk, dc = threadIdx.x % kTiBlk;
+  double dcol[kTiBlk];
+  if (di < nb) {
+#pragma unroll
+    for (int r = kTiBlk - 1; r >= 0; --r) {
+      double v = (r == dc) ? 1.0 : 0.0;
+#pragma unroll
+      for (int t = r + 1; t < kTiBlk; ++t) v = fma(-diag[di][r][t], dcol[t], v);
+      dcol[r] = v / diag[di][r][r];
+    }
+  }
+  __syncthreads();  // every R_II read before D_I overwrites it
+  if (di < nb) {
+#pragma unroll
+    for (int r = 0; r < kTiBlk; ++r) diag[di][r][dc] = dcol[r];
+  }
+  double x[kTiP][kTiQ];
+#pragma unroll
+  for (int p = 0; p < kTiP; ++p)
+#pragma unroll
+    for (int q = 0; q < kTiQ; ++q) x[p][q] = (ty + kTiRows * p == tx + kTiCols * q) ? 1.0 : 0.0;
+
+  for (int I = nb - 1; I >= 0; --I) {
+    const int buf = I & 1;
+    // publish block row I of B (every thread owns one of its rows) and the
+    // column block R_:I above the diagonal block
+#pragma unroll
+    for (int q = 0; q < kTiQ; ++q) {
+      double v = x[0][q];
+#pragma unroll
+      for (int p = 1; p < kTiP; ++p) v = (p == I) ? x[p][q] : v;
+      brow[buf][ty][tx + kTiCols * q] = v;
+    }
+    for (int e = threadIdx.x; e < I * kTiBlk * kTiBlk; e += kTiThreads) {
+      const int k = e / kTiBlk, t = e % kTiBlk;
+      rcol[buf][k][t] = r_at(k, I * kTiBlk + t);
+    }
+    __syncthreads();
+    // X_I = D_I B_I at own row ty of the block and own columns
+    double xi[kTiQ];
+#pragma unroll
+    for (int q = 0; q < kTiQ; ++q) xi[q] = 0.0;
+    for (int t = ty; t < kTiBlk; ++t) {
+      const double dv = diag[I][ty][t];
+#pragma unroll
+      for (int q = 0; q < kTiQ; ++q) xi[q] = fma(dv, brow[buf][t][tx + kTiCols * q], xi[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kTiQ; ++q) {
+      xrow[buf][ty][tx + kTiCols * q] = xi[q];
+#pragma unroll
+      for (int p = 0; p < kTiP; ++p)
+        if (p == I) x[p][q] = xi[q];
+    }
+    __syncthreads();
+    // rows of the blocks above: B_K -= R_KI X_I
+#pragma unroll
+    for (int p = 0; p < kTiP; ++p) {
+      if (p < I) {
+        const int k = ty + kTiRows * p;
+#pragma unroll 4
+        for (int t = 0; t < kTiBlk; ++t) {
+          const double rk = rcol[buf][k][t];
+#pragma unroll
+          for (int q = 0; q < kTiQ; ++q) x[p][q] = fma(-rk, xrow[buf][t][tx + kTiCols * q], x[p][q]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < kTiP; ++p) {
+#pragma unroll
+    for (int q = 0; q < kTiQ; ++q) {
+      const int i = ty + kTiRows * p, c = tx + kTiCols * q;
+      if (i < l && c < l) X[static_cast<int64_t>(i) * l + c] = x[p][q];
+    }
+  }
+}
+
 } // namespace
 
 extern "C" {
@@ -572,7 +711,7 @@ int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double 
                            double *Rinv, int32_t *status, void *stream) {
   return fs::guarded([&]() -> int {
     FS_CHECK_ARG(nmat >= 0, "batched cholinv: negative count");
-    FS_CHECK_ARG(nmat == 0 || (descs && G && Rinv && status), "batched cholinv: null argument");
+    FS_CHECK_ARG(nmat == 0 || (descs && G && (Rinv || R) && status), "batched cholinv: null argument");
     for (int i = 0; i < nmat; ++i) {
       FS_CHECK_ARG(descs[i].l >= 1 && descs[i].l <= kCholInvMaxOrder,
                    "batched cholinv: order must be in [1, %d]", kCholInvMaxOrder);
@@ -614,7 +753,7 @@ int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double 
         const char *e = std::getenv("FS_CHOLINV_KERNEL");  // tuning only: 1 = one column per step
         return e ? std::atoi(e) : 0;
       }();
-      if (mode == 1) {
+      if (mode == 1 && Rinv) {
         hipLaunchKernelGGL(cholinv_kernel, dim3(nb), dim3(kCholInvThreads), kCholInvLds,
                            fs::as_stream(stream), bt);
         FS_LAUNCH_CHECK("cholinv_kernel");
@@ -694,6 +833,37 @@ int fs_batched_trsm_right_upper_f64(int nmat, const fs_trsm_desc *descs, const d
       hipLaunchKernelGGL(trsm_rows_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
                          fs::as_stream(stream), b);
       FS_LAUNCH_CHECK("trsm_rows_kernel");
+    }
+    return FS_OK;
+  });
+}
+
+int fs_batched_triinv_f64(int nmat, const fs_triinv_desc *descs, const double *R, double *X,
+                          const int32_t *status, void *stream) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(nmat >= 0, "batched triinv: negative count");
+    FS_CHECK_ARG(nmat == 0 || (descs && R && X), "batched triinv: null argument");
+    for (int i = 0; i < nmat; ++i)
+      FS_CHECK_ARG(descs[i].l >= 1 && descs[i].l <= kCholInvMaxOrder && descs[i].r_off >= 0 &&
+                       descs[i].x_off >= 0,
+                   "batched triinv: order must be in [1, %d]", kCholInvMaxOrder);
+    int i0 = 0;
+    while (i0 < nmat) {
+      TriInvBatch bt;
+      std::memset(&bt, 0, sizeof(bt));
+      bt.R = R;
+      bt.X = X;
+      bt.status = status ? status + i0 : nullptr;
+      int nb = 0;
+      for (; i0 < nmat && nb < kMaxMat; ++i0) {
+        TriInvDev &d = bt.d[nb++];
+        d.l = static_cast<int32_t>(descs[i0].l);
+        d.r_off = descs[i0].r_off;
+        d.x_off = descs[i0].x_off;
+      }
+      bt.nmat = nb;
+      hipLaunchKernelGGL(triinv_kernel, dim3(nb), dim3(kTiThreads), 0, fs::as_stream(stream), bt);
+      FS_LAUNCH_CHECK("triinv_kernel");
     }
     return FS_OK;
   });

@@ -237,24 +237,28 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False, statuses=None):
     Rtot = None
     for p in range(passes):
         G, nparts = gram(cur)
-        status = _ops.zeros(len(cur), dtype=torch.int32)
+        status = _ops.empty(len(cur), dtype=torch.int32)  # every factorisation kernel writes it
         cols = cur.cols.astype(np.float64)
         if shifted and p == 0:
             shift = 11.0 * (cur.rows * cols + cols * (cols + 1.0)) * eps
         else:
             shift = np.zeros_like(cols)
-        R = Mats.packed(cur.cols, cur.cols) if want_r else None
         if len(cur) and cur.cols.max() <= _ops.CHOLINV_MAX_ORDER:
+            # factor (one workgroup per matrix, four columns per barrier), then
+            # the blocked inverse, then the solve as one grouped GEMM
+            R = Mats.packed(cur.cols, cur.cols)
             Rinv = Mats.packed(cur.cols, cur.cols)
             _ops.batched_cholinv(
                 {"l": cur.cols, "g_off": G.off, "g_stride": cur.cols * cur.cols, "n_part": nparts,
-                 "r_off": 0 if R is None else R.off, "rinv_off": Rinv.off, "shift_rel": shift},
-                G.buf, None if R is None else R.buf, Rinv.buf, status,
+                 "r_off": R.off, "rinv_off": 0, "shift_rel": shift},
+                G.buf, R.buf, None, status,
             )
+            _ops.batched_triinv({"l": cur.cols, "r_off": R.off, "x_off": Rinv.off}, R.buf, Rinv.buf, status)
             out = gemm(cur, Rinv)
+            if not want_r:
+                R = None
         else:
-            if R is None:
-                R = Mats.packed(cur.cols, cur.cols)
+            R = Mats.packed(cur.cols, cur.cols)
             if np.any(nparts > 1):
                 G = gemm(cur.T(), cur)
             _ops.batched_cholesky(

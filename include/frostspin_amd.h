@@ -396,6 +396,8 @@ int fs_batched_cholesky_f64(int nmat, const fs_chol_desc *descs,
  * Y R^-1 of CholeskyQR then runs as a grouped GEMM with Rinv. Replaces the
  * QR step of the reference's randomized/ARPACK sector SVDs like
  * fs_batched_cholesky_f64 (misc_tools/svd_tools.py:154-197).
+ * R or Rinv may be NULL (not both): with Rinv NULL only the factor R is
+ * written (the factorisation alone; fs_batched_triinv_f64 then inverts it).
  */
 typedef struct fs_cholinv_desc {
   int64_t l;
@@ -408,6 +410,23 @@ typedef struct fs_cholinv_desc {
 
 int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double *G,
                            double *R, double *Rinv, int32_t *status, void *stream);
+
+/*
+ * Inverse X = R^-1 of upper triangular matrices (row-major l x l at R + r_off,
+ * X row-major at X + x_off, l <= 128), one workgroup per matrix, blocked by 16
+ * rows. status (may be NULL) is the per-matrix status of the factorisation
+ * that produced R: a nonzero entry makes that X all NaN. With the factor-only
+ * form of fs_batched_cholinv_f64 this gives R and R^-1 for CholeskyQR
+ * (misc_tools/svd_tools.py:154-197 of the reference orthonormalises with
+ * Householder QR; no counterpart of this split there).
+ */
+typedef struct fs_triinv_desc {
+  int64_t l;
+  int64_t r_off, x_off;
+} fs_triinv_desc;
+
+int fs_batched_triinv_f64(int nmat, const fs_triinv_desc *descs, const double *R, double *X,
+                          const int32_t *status, void *stream);
 
 typedef struct fs_trsm_desc {
   int64_t m, l, y_off, y_rs, y_cs, r_off, o_off;

@@ -13,6 +13,7 @@ The product never imports this module.
 """
 
 import numpy as np
+import scipy.linalg
 import torch
 
 from frostspin_amd import _ops
@@ -133,7 +134,8 @@ def batched_cholesky(descs, G, R, status):
 
 
 def batched_cholinv(descs, G, R, Rinv, status):
-    g, ri, st = _np(G), _np(Rinv), _np(status)
+    g, st = _np(G), _np(status)
+    ri = None if Rinv is None else _np(Rinv)
     r = None if R is None else _np(R)
     descs = _desc_rows(descs, ("l", "g_off", "g_stride", "n_part", "r_off", "rinv_off", "shift_rel"))
     for b, (l, go, gs, npart, ro, rio, shift_rel) in enumerate(descs):
@@ -148,13 +150,30 @@ def batched_cholinv(descs, G, R, Rinv, status):
             # the kernel stops at the failed pivot and leaves R / Rinv unwritten;
             # NaN stands for that unspecified content
             st[b] = 1
-            ri[rio : rio + l * l] = np.nan
+            if ri is not None:
+                ri[rio : rio + l * l] = np.nan
             if r is not None:
                 r[ro : ro + l * l] = np.nan
             continue
         if r is not None:
             r[ro : ro + l * l] = rr.reshape(-1)
-        ri[rio : rio + l * l] = np.linalg.inv(rr).reshape(-1)
+        if ri is not None:
+            ri[rio : rio + l * l] = np.linalg.inv(rr).reshape(-1)
+
+
+def batched_triinv(descs, R, X, status=None):
+    """X = R^-1 per upper triangular R (NaN where the status is nonzero)."""
+    r, x = _np(R), _np(X)
+    st = None if status is None else _np(status)
+    descs = _desc_rows(descs, ("l", "r_off", "x_off"))
+    for b, (l, ro, xo) in enumerate(descs):
+        if st is not None and st[b] != 0:
+            x[xo : xo + l * l] = np.nan
+            continue
+        rr = np.triu(r[ro : ro + l * l].reshape(l, l))
+        with np.errstate(all="ignore"):
+            x[xo : xo + l * l] = scipy.linalg.solve_triangular(
+                rr, np.eye(l), lower=False, check_finite=False).reshape(-1)
 
 
 def batched_trsm_right_upper(descs, Y, R, O):
@@ -342,7 +361,7 @@ def install(monkeypatch):
     monkeypatch.setattr(_ops, "LiePlan", EmulatedLiePlan)
     for name in (
         "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_cholinv",
-        "batched_trsm_right_upper",
+        "batched_triinv", "batched_trsm_right_upper",
         "weighted_norm2", "scale_", "scale_rsqrt_dev_",
         "axpby_", "copy2d", "gather2d", "weighted_trace", "fill_normal_",
     ):

@@ -134,3 +134,52 @@ def test_rsvd_recovers_from_cholesky_breakdown(backend, monkeypatch):
         rec = (us[i][:, :5] * s_list[i][:5]) @ vts[i][:5]
         ref = np.linalg.svd(a)
         assert np.abs(rec - (ref[0][:, :5] * ref[1][:5]) @ ref[2][:5]).max() < 1e-10 * ref[1][0]
+
+
+def test_triangular_inverse_blocked(backend):
+    """fs_batched_triinv: R^-1 of upper triangular matrices of every order up
+    to 128 across the 16-row block boundaries (identity padding), against
+    scipy's triangular solve; a nonzero factorisation status gives NaN and a
+    factor-only Cholesky followed by the inverse matches the fused form."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    orders = [1, 3, 15, 16, 17, 31, 33, 69, 90, 127, 128]
+    Rs = []
+    for l in orders:
+        r = np.triu(rng.standard_normal((l, l))) / np.sqrt(l)
+        r[np.diag_indices(l)] = 1.0 + rng.random(l)
+        Rs.append(r)
+    sizes = np.array([l * l for l in orders])
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    R = _ops.upload(np.concatenate([r.reshape(-1) for r in Rs]))
+    X = _ops.upload(np.full(int(sizes.sum()), 7.0))
+    status = _ops.upload(np.array([0] * (len(orders) - 1) + [3]), dtype=torch.int32)
+    _ops.batched_triinv({"l": np.array(orders), "r_off": offs, "x_off": offs}, R, X, status)
+    x = _ops.download(X)
+    for i, (l, r) in enumerate(zip(orders, Rs, strict=True)):
+        got = x[offs[i] : offs[i] + l * l].reshape(l, l)
+        if i == len(orders) - 1:
+            assert np.isnan(got).all()
+            continue
+        ref = lg.solve_triangular(r, np.eye(l), lower=False)
+        assert np.abs(got - ref).max() < 1e-13 * max(1.0, np.abs(ref).max()), l
+        assert np.abs(np.tril(got, -1)).max() == 0.0
+    # factor only + blocked inverse against the fused Cholesky + inverse
+    l, n = 90, 3
+    Gs = []
+    for _ in range(n):
+        y = rng.standard_normal((300, l))
+        Gs.append((y.T @ y).reshape(-1))
+    G = _ops.upload(np.concatenate(Gs))
+    base = {"l": np.full(n, l), "g_off": np.arange(n) * l * l, "g_stride": l * l, "n_part": 1,
+            "r_off": np.arange(n) * l * l, "rinv_off": np.arange(n) * l * l, "shift_rel": 0.0}
+    R1, X1, R2, X2 = (_ops.empty(n * l * l) for _ in range(4))
+    st1, st2 = (_ops.zeros(n, dtype=torch.int32) for _ in range(2))
+    _ops.batched_cholinv(base, G, R1, X1, st1)
+    _ops.batched_cholinv(base, G, R2, None, st2)
+    _ops.batched_triinv({"l": np.full(n, l), "r_off": base["r_off"], "x_off": base["r_off"]}, R2, X2, st2)
+    assert (_ops.download(st1) == 0).all() and (_ops.download(st2) == 0).all()
+    assert np.array_equal(_ops.download(R1), _ops.download(R2))
+    x1, x2 = _ops.download(X1), _ops.download(X2)
+    assert np.abs(x1 - x2).max() < 1e-12 * np.abs(x1).max()
