@@ -40,19 +40,67 @@ def device():
 _SIDE = threading.local()
 
 
-def side_stream():
-    """A second stream of the calling thread on the current device (default
-    priority), for work independent of the current stream; None off-GPU."""
+def side_stream(k=0):
+    """Extra stream number k of the calling thread on the current device
+    (default priority, created once), for work independent of the current
+    stream; None off-GPU."""
     if device().type != "cuda":
         return None
     idx = torch._C._cuda_getDevice()
     streams = getattr(_SIDE, "streams", None)
     if streams is None:
         streams = _SIDE.streams = {}
-    st = streams.get(idx)
+    st = streams.get((idx, k))
     if st is None:
-        st = streams[idx] = torch.cuda.Stream(device=idx)
+        st = streams[(idx, k)] = torch.cuda.Stream(device=idx)
     return st
+
+
+# Independent chains of small launches (the enlarged corners and the
+# renormalizations of different sites) can run on separate streams; results
+# are identical either way (same kernels on the same inputs).
+# Measured on the SU(2) chi=256 workload: 2-3 % slower than one stream (the
+# chains are already issued faster than they run), so off by default.
+CONCURRENCY = {"enabled": False}
+
+
+def _device_buffers(obj):
+    if isinstance(obj, torch.Tensor):
+        yield obj
+    elif isinstance(obj, (tuple, list)):
+        for o in obj:
+            yield from _device_buffers(o)
+    else:
+        data = getattr(obj, "_data", None)
+        if isinstance(data, torch.Tensor):
+            yield data
+
+
+def run_concurrently(thunks):
+    """Call each thunk (a function enqueueing device work and returning its
+    results), the first on the current stream and the others on side streams,
+    then make the current stream wait for all of them. Buffers produced on a
+    side stream are recorded on the current stream, so the allocator does not
+    hand them to other work while the current stream may still read them.
+    Inputs must be ready on the current stream when this is called (the side
+    streams wait for it first)."""
+    thunks = list(thunks)
+    if len(thunks) < 2 or not CONCURRENCY["enabled"] or KernelTimer.active is not None or device().type != "cuda":
+        return [t() for t in thunks]
+    main = torch.cuda.current_stream()
+    sides = [side_stream(1 + i) for i in range(len(thunks) - 1)]
+    for st in sides:
+        st.wait_stream(main)
+    results = [thunks[0]()]
+    for st, t in zip(sides, thunks[1:], strict=True):
+        with torch.cuda.stream(st):
+            results.append(t())
+    for st in sides:
+        main.wait_stream(st)
+    for r in results[1:]:
+        for buf in _device_buffers(r):
+            buf.record_stream(main)
+    return results
 
 
 def _stream():

@@ -473,18 +473,27 @@ inline int pick_splits(int64_t tiles, int64_t kmax) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({by_cus, by_k, kMaxSplits})));
 }
 
-// Per-thread, per-device split-K workspace (grow-only): host threads that
-// serve requests on their own streams never share it.
-double *splitk_workspace(int64_t n_doubles) {
+// Split-K workspace per (host thread, device, stream), grow-only: launches
+// that may run concurrently (other threads, or other streams of one thread)
+// never share one.
+double *splitk_workspace(int64_t n_doubles, hipStream_t stream) {
   struct Ws {
     int device = -1;
+    hipStream_t stream = nullptr;
     double *ptr = nullptr;
     int64_t size = 0;
   };
-  static thread_local Ws ws[16];
+  static thread_local std::vector<Ws> ws;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return nullptr;
-  Ws &w = ws[dev];
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+  Ws *found = nullptr;
+  for (Ws &x : ws)
+    if (x.device == dev && x.stream == stream) found = &x;
+  if (!found) {
+    ws.push_back(Ws{dev, stream, nullptr, 0});
+    found = &ws.back();
+  }
+  Ws &w = *found;
   if (w.size < n_doubles) {
     if (w.ptr) (void)hipFree(w.ptr);
     w.ptr = nullptr;
@@ -620,7 +629,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
           elems += static_cast<int64_t>(d.m) * d.n;
           bt.elem_prefix[i + 1] = elems;
         }
-        bt.ws = splitk_workspace(elems * splits);
+        bt.ws = splitk_workspace(elems * splits, fs::as_stream(stream));
         if (bt.ws == nullptr) {
           fs::set_error("grouped gemm: split-K workspace allocation failed");
           return FS_ERR_HIP;

@@ -54,3 +54,32 @@ def test_su2_simple_update_and_ctmrg_match_u1(backend):
     eU = models.bond_energy(ctmU, hU)
     assert abs(eS - eU) < 1e-12
     assert -0.75 < eS < 0.0
+
+
+@pytest.mark.gpu
+def test_concurrent_site_chains_give_identical_results(gpu):
+    """The enlarged corners and renormalizations of the two sites of a move run
+    on separate streams; the same iterations with one stream give the same bits
+    (energy, every corner and edge tensor)."""
+    from frostspin_amd import _ops
+
+    h = models.heisenberg_bond("SU2")
+    runs = []
+    for enabled in (True, False):
+        _ops.CONCURRENCY["enabled"] = enabled
+        try:
+            _, ctm = models.thermal_ctmrg(h, 6, 1.0, 48)
+            for _ in range(5):
+                ctm.iterate()
+            env = ctm._env
+            tensors = [g(x, y) for x, y in ctm._site_coords
+                       for g in (env.get_C1, env.get_C2, env.get_C3, env.get_C4,
+                                 env.get_T1, env.get_T2, env.get_T3, env.get_T4)]
+            runs.append((models.bond_energy(ctm, h), list(ctm.chi_values),
+                         [_ops.download(t._data) for t in tensors]))
+        finally:
+            _ops.CONCURRENCY["enabled"] = False
+    (e1, chi1, d1), (e2, chi2, d2) = runs
+    assert chi1 == chi2 and e1 == e2
+    for a, b in zip(d1, d2, strict=True):
+        assert np.array_equal(a, b)
