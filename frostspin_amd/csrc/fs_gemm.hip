@@ -112,8 +112,19 @@ __device__ inline void load_b(double (&r)[PER], const double *__restrict__ B, co
 //  * k-contiguous source (row-major A / col-major B): i-major rows [i][k] with
 //    odd stride BK+1: the 16 rows of an operand read map to distinct banks and
 //    the 4 k columns shift them by one bank.
+// Occupancy target per tile shape: 64 x 64 tiles fit 4 workgroups per CU in
+// LDS and, held to 128 registers (accumulators in VGPRs), 4 waves per SIMD
+// (3 otherwise): 35.7 vs 32.5 TFLOP/s on the projector products of a SU(2)
+// chi=256 move, 51 vs 45 on 4096^3. 32-deep K panels (2 workgroups per CU)
+// measured slower. The other shapes keep the compiler's choice.
 template <int BM, int BN, int BK>
-__global__ void __launch_bounds__(kThreads) gemm_f64_kernel(Batch bt) {
+constexpr int gemm_waves_per_eu() {
+  return (BM == 64 && BN == 64 && BK == 16) ? 4 : 1;
+}
+
+template <int BM, int BN, int BK>
+__global__ void __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK>()))) gemm_f64_kernel(Batch bt) {
   constexpr int PAD = 16;
   constexpr int A_SZ = (BK * (BM + PAD) > BM * (BK + 1)) ? BK * (BM + PAD) : BM * (BK + 1);
   constexpr int B_SZ = (BK * (BN + PAD) > BN * (BK + 1)) ? BK * (BN + PAD) : BN * (BK + 1);
