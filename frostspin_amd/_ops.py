@@ -393,14 +393,43 @@ def lie_plan_tables(pattern, degen, in_block, block_off, block_rs, block_cs, *, 
     return tasks, rows, digits, shapes
 
 
+def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, w, src_size):
+    """(plan, new block shapes) of one permute straight from its pattern and
+    leg degeneracies (fs_lie_plan_build: tables and device plan in one call)."""
+    c = np.ascontiguousarray
+    degen, in_block = c(degen, dtype=np.int64), c(in_block, dtype=np.int64)
+    block_off, block_rs, block_cs = (c(x, dtype=np.int64) for x in (block_off, block_rs, block_cs))
+    assert in_block.size == pattern.nt_in and w.dtype == np.float64 and w.flags.c_contiguous
+    shapes = np.empty((pattern.nt_out, 2), dtype=np.int64)
+    handle = ctypes.c_void_p()
+    check(lib.fs_lie_plan_build(
+        ctypes.byref(pattern), degen.size, _lib.i64p(degen), _lib.i64p(in_block), block_off.size,
+        _lib.i64p(block_off), _lib.i64p(block_rs), _lib.i64p(block_cs), w.size, _lib.dblp(w),
+        int(src_size), _lib.i64p(shapes), ctypes.byref(handle),
+    ))
+    dst = int((shapes[:, 0] * shapes[:, 1]).sum())
+    return LiePlan._from_handle(handle.value, src_size, dst), shapes
+
+
 class LiePlan:
     """Owner of a native structural-transform plan (fs_lie_plan_*).
 
     tasks (n, 8), rows (n, 3), digits (n, 5): int64 tables laid out as
     fs_lie_task / fs_lie_row / fs_lie_digit; w: transposed structural matrices.
-    The host tables are kept for inspection."""
+    Plans built from tables keep them for inspection (None for plans built by
+    lie_plan_build)."""
 
     __slots__ = ("_handle", "coeffs", "digits", "dst_size", "flops", "rows", "src_size", "tasks", "w")
+
+    @classmethod
+    def _from_handle(cls, handle, src_size, dst_size):
+        self = cls.__new__(cls)
+        self.tasks = self.rows = self.digits = self.w = None
+        self.src_size = int(src_size)
+        self.dst_size = int(dst_size)
+        self._handle = handle
+        self._info()
+        return self
 
     def __init__(self, tasks, rows, digits, w, src_size, dst_size):
         self._handle = None
@@ -421,6 +450,9 @@ class LiePlan:
             )
         )
         self._handle = handle.value
+        self._info()
+
+    def _info(self):
         nt = ctypes.c_int64()
         fl = ctypes.c_double()
         co = ctypes.c_double()

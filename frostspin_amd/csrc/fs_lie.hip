@@ -616,4 +616,35 @@ int fs_lie_plan_destroy(fs_lie_plan *plan) {
   });
 }
 
+int fs_lie_plan_build(const fs_lie_pattern *pattern, int64_t ndegen, const int64_t *degen,
+                      const int64_t *in_block, int64_t nblocks, const int64_t *block_off,
+                      const int64_t *block_rs, const int64_t *block_cs, int64_t nw,
+                      const double *w, int64_t src_size, int64_t *shapes, fs_lie_plan **plan) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(pattern && plan && shapes, "lie plan build: null argument");
+    *plan = nullptr;
+    const int64_t ns = pattern->ntasks, nd = pattern->ndim;
+    FS_CHECK_ARG(ns >= 0 && nd >= 1 && nd <= FS_LIE_MAX_LEGS, "lie plan build: bad pattern");
+    int64_t nrows = 0;
+    for (int64_t t = 0; t < ns; ++t) {
+      FS_CHECK_ARG(pattern->n_in[t] >= 0 && pattern->n_out[t] >= 0, "lie plan build: negative row count");
+      nrows += pattern->n_in[t] + pattern->n_out[t];
+    }
+    std::vector<int64_t> tasks(static_cast<size_t>(ns) * 8), rows(static_cast<size_t>(nrows) * 3);
+    std::vector<int64_t> digits(static_cast<size_t>(ns * nd) * 5);
+    int st = fs_lie_plan_tables(pattern, ndegen, degen, in_block, nblocks, block_off, block_rs,
+                                block_cs, tasks.data(), nrows, rows.data(), digits.data(), shapes);
+    if (st != FS_OK) return st;
+    int64_t dst_size = 0;
+    for (int64_t t = 0; t < pattern->nt_out; ++t) dst_size += shapes[2 * t] * shapes[2 * t + 1];
+    static_assert(sizeof(fs_lie_task) == 8 * sizeof(int64_t) && sizeof(fs_lie_row) == 3 * sizeof(int64_t) &&
+                      sizeof(fs_lie_digit) == 5 * sizeof(int64_t),
+                  "table layouts");
+    return fs_lie_plan_create(ns, reinterpret_cast<const fs_lie_task *>(tasks.data()), nrows,
+                              reinterpret_cast<const fs_lie_row *>(rows.data()), ns * nd,
+                              reinterpret_cast<const fs_lie_digit *>(digits.data()), nw, w, src_size,
+                              dst_size, plan);
+  });
+}
+
 } // extern "C"

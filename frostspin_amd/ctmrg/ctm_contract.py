@@ -10,6 +10,31 @@ Site tensor A legs: (physical, ancilla, up, right, down, left); C/T legs follow
 the clockwise convention documented in abstract_ctmrg.py:26-42.
 """
 
+import weakref
+
+# Permuted copies of site tensors, kept alongside the tensor they come from:
+# the site tensors of an environment do not change between set_tensors calls
+# and every move needs the same few permutations of them (two per enlarged
+# corner or renormalized edge). forget_site_permutes() drops them all.
+_SITE_PERMUTES = weakref.WeakKeyDictionary()
+
+
+def site_permute(A, row_axes, col_axes):
+    """A.permute(row_axes, col_axes) for a site tensor (or a permuted copy of
+    one), computed once and kept while A lives."""
+    kept = _SITE_PERMUTES.get(A)
+    if kept is None:
+        kept = _SITE_PERMUTES[A] = {}
+    key = (tuple(row_axes), tuple(col_axes))
+    t = kept.get(key)
+    if t is None:
+        t = kept[key] = A.permute(row_axes, col_axes)
+    return t
+
+
+def forget_site_permutes():
+    _SITE_PERMUTES.clear()
+
 
 def contract_enlarged_corner(C1, T1, T4, A):
     """Upper-left template C1-T1 // T4-A-A*; other corners are fed rotated legs
@@ -19,19 +44,19 @@ def contract_enlarged_corner(C1, T1, T4, A):
     ul = ul @ T4
     ul = A @ ul.permute((0, 3), (1, 4, 2, 5))
     ul = ul.permute((0, 1, 4, 5), (2, 3, 6, 7))
-    ul = A.permute((0, 1, 4, 5), (2, 3)).dagger() @ ul
+    ul = site_permute(A, (0, 1, 4, 5), (2, 3)).dagger() @ ul
     return ul.permute((2, 0, 4), (3, 1, 5))
 
 
 def contract_ul(C1, T1, T4, A):
     return contract_enlarged_corner(
-        C1, T1.permute((1, 2, 0), (3,)), T4.permute((0,), (1, 2, 3)), A.permute((0, 1, 3, 4), (2, 5))
+        C1, T1.permute((1, 2, 0), (3,)), T4.permute((0,), (1, 2, 3)), site_permute(A, (0, 1, 3, 4), (2, 5))
     )
 
 
 def contract_ur(T1, C2, A, T2):
     return contract_enlarged_corner(
-        C2, T2.permute((2, 3, 1), (0,)), T1.permute((0,), (1, 2, 3)), A.permute((0, 1, 4, 5), (3, 2))
+        C2, T2.permute((2, 3, 1), (0,)), T1.permute((0,), (1, 2, 3)), site_permute(A, (0, 1, 4, 5), (3, 2))
     )
 
 
@@ -39,13 +64,13 @@ def contract_dr(A, T2, T3, C3):
     """Transposed with respect to clockwise order (dr is contracted as dl-10-dr)."""
     return contract_enlarged_corner(
         C3.transpose(), T3.permute((0, 1, 3), (2,)), T2.permute((1,), (2, 3, 0)),
-        A.permute((0, 1, 5, 2), (4, 3)),
+        site_permute(A, (0, 1, 5, 2), (4, 3)),
     )
 
 
 def contract_dl(T4, A, C4, T3):
     return contract_enlarged_corner(
-        C4, T4.permute((1, 2, 0), (3,)), T3.permute((3,), (0, 1, 2)), A.permute((0, 1, 2, 3), (5, 4))
+        C4, T4.permute((1, 2, 0), (3,)), T3.permute((3,), (0, 1, 2)), site_permute(A, (0, 1, 2, 3), (5, 4))
     )
 
 

@@ -10,6 +10,7 @@ the site index indices[x mod Lx, y mod Ly].
 import numpy as np
 
 from ..symmetric_tensor.tools import get_symmetric_tensor_type
+from .ctm_contract import forget_site_permutes
 
 
 def initialize_dummy_env(A):
@@ -216,6 +217,7 @@ class CTMEnvironment:
             if not A.match_representations(self._unique_As[i]):
                 raise ValueError("Incompatible representation for new tensor")
         self._unique_As = tuple(tensors)
+        forget_site_permutes()
         self.reset_constructed_corners()
 
     def set_symmetry(self, symmetry):

@@ -15,6 +15,7 @@ import torch
 from .. import _ops
 from ..misc_tools import block_linalg as bl
 from ..misc_tools.svd_tools import find_block_cuts
+from .ctm_contract import site_permute
 
 # Blocks up to SMALL_DENSE (smallest side) always use the exact Jacobi SVD; above
 # DENSE_SVD_LIMIT the randomized truncated SVD is the only option.
@@ -255,7 +256,7 @@ def renormalize_T(Pt, T, A, P):
     nT = T @ P.permute((2,), (0, 1, 3))
     nT = A @ nT.permute((0, 3), (1, 4, 2, 5))
     nT = nT.permute((0, 1, 4, 5), (2, 3, 6, 7))
-    nT = A.permute((0, 1, 4, 5), (2, 3)).dagger() @ nT
+    nT = site_permute(A, (0, 1, 4, 5), (2, 3)).dagger() @ nT
     nT = Pt.transpose() @ nT.permute((3, 1, 4), (2, 0, 5))
     return nT.normalize_()
 
@@ -282,7 +283,7 @@ def renormalize_C1_up(C1, T4, P):
 
 
 def renormalize_T1(Pt, T1, A, P):
-    nT1 = renormalize_T(Pt, T1.permute((1, 2, 3), (0,)), A.permute((0, 1, 4, 5), (2, 3)), P)
+    nT1 = renormalize_T(Pt, T1.permute((1, 2, 3), (0,)), site_permute(A, (0, 1, 4, 5), (2, 3)), P)
     return nT1.permute((3,), (1, 2, 0))
 
 
@@ -295,7 +296,7 @@ def renormalize_C2_right(C2, T1, P):
 
 
 def renormalize_T2(Pt, T2, A, P):
-    nT2 = renormalize_T(Pt, T2.permute((2, 3, 0), (1,)), A.permute((0, 1, 5, 2), (3, 4)), P)
+    nT2 = renormalize_T(Pt, T2.permute((2, 3, 0), (1,)), site_permute(A, (0, 1, 5, 2), (3, 4)), P)
     return nT2.permute((0,), (3, 1, 2))
 
 
@@ -308,7 +309,7 @@ def renormalize_C3_down(C3, T2, P):
 
 
 def renormalize_T3(Pt, T3, A, P):
-    nT3 = renormalize_T(Pt, T3.permute((0, 1, 2), (3,)), A.permute((0, 1, 2, 3), (4, 5)), P)
+    nT3 = renormalize_T(Pt, T3.permute((0, 1, 2), (3,)), site_permute(A, (0, 1, 2, 3), (4, 5)), P)
     return nT3.permute((1, 2, 0), (3,))
 
 
@@ -321,7 +322,7 @@ def renormalize_C4_left(C4, T3, P):
 
 
 def renormalize_T4(Pt, T4, A, P):
-    nT4 = renormalize_T(Pt, T4.permute((1, 2, 3), (0,)), A.permute((0, 1, 3, 4), (5, 2)), P)
+    nT4 = renormalize_T(Pt, T4.permute((1, 2, 3), (0,)), site_permute(A, (0, 1, 3, 4), (5, 2)), P)
     return nT4.permute((3,), (1, 2, 0))
 
 

@@ -312,12 +312,10 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         st = cls._permute_structure(irreps_per_leg, sig, nrr, axes, nrr_out)
         degen = np.concatenate([np.asarray(r[0], dtype=np.int64) for r in reps])
         rs_b, cs_b = self._block_strides()
-        tasks, rows, dig, shapes = _ops.lie_plan_tables(
-            st.pattern, degen, self._present_map(st.in_targets), self._boffs, rs_b, cs_b,
-            nrows=int(st.n_in.sum() + st.n_out.sum()),
+        plan, shapes = _ops.lie_plan_build(
+            st.pattern, degen, self._present_map(st.in_targets), self._boffs, rs_b, cs_b, st.W,
+            self._data.numel(),
         )
-        sizes = shapes[:, 0] * shapes[:, 1]
-        plan = _ops.LiePlan(tasks, rows, dig, st.W, self._data.numel(), int(sizes.sum()))
         hit = (plan, shapes, st.out_targets)
         with cls._plans_lock:
             cls._plans[key] = hit

@@ -251,6 +251,15 @@ int fs_lie_plan_tables(const fs_lie_pattern *pattern, int64_t ndegen, const int6
                        const int64_t *block_rs, const int64_t *block_cs, int64_t *tasks,
                        int64_t nrows, int64_t *rows, int64_t *digits, int64_t *shapes);
 
+/* fs_lie_plan_tables followed by fs_lie_plan_create in one call: the plan of
+ * one permute straight from its pattern and degeneracies (the tables stay on
+ * the host side of the library). shapes: the new block shapes [nt_out][2];
+ * the destination size is their sum of products. */
+int fs_lie_plan_build(const fs_lie_pattern *pattern, int64_t ndegen, const int64_t *degen,
+                      const int64_t *in_block, int64_t nblocks, const int64_t *block_off,
+                      const int64_t *block_rs, const int64_t *block_cs, int64_t nw,
+                      const double *w, int64_t src_size, int64_t *shapes, fs_lie_plan **plan);
+
 /* Work of one execution: flops (2 * sum n_in*n_out*E) and coefficients read
  * and written (sum (n_in + n_out) * E). */
 int fs_lie_plan_info(const fs_lie_plan *plan, int64_t *ntiles, double *flops, double *coeffs);

@@ -326,6 +326,16 @@ class EmulatedLiePlan:
                 d[idx] = Y[o]
 
 
+def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, w, src_size):
+    """fs_lie_plan_build in emulation: the native host tables, then the numpy plan."""
+    n_in = np.ctypeslib.as_array(pattern.n_in, shape=(pattern.ntasks,))
+    n_out = np.ctypeslib.as_array(pattern.n_out, shape=(pattern.ntasks,))
+    tasks, rows, digits, shapes = _ops.lie_plan_tables(
+        pattern, degen, in_block, block_off, block_rs, block_cs, nrows=int(n_in.sum() + n_out.sum()))
+    dst = int((shapes[:, 0] * shapes[:, 1]).sum())
+    return EmulatedLiePlan(tasks, rows, digits, w, src_size, dst), shapes
+
+
 def _execute(self, src, dst):
     _permute_execute(self, src, dst)
 
@@ -361,7 +371,7 @@ def install(monkeypatch):
     monkeypatch.setattr(_ops, "LiePlan", EmulatedLiePlan)
     for name in (
         "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_cholinv",
-        "batched_triinv", "batched_trsm_right_upper",
+        "batched_triinv", "batched_trsm_right_upper", "lie_plan_build",
         "weighted_norm2", "scale_", "scale_rsqrt_dev_",
         "axpby_", "copy2d", "gather2d", "weighted_trace", "fill_normal_",
     ):
