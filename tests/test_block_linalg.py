@@ -180,6 +180,51 @@ def test_triangular_inverse_blocked(backend):
     _ops.batched_cholinv(base, G, R2, None, st2)
     _ops.batched_triinv({"l": np.full(n, l), "r_off": base["r_off"], "x_off": base["r_off"]}, R2, X2, st2)
     assert (_ops.download(st1) == 0).all() and (_ops.download(st2) == 0).all()
-    assert np.array_equal(_ops.download(R1), _ops.download(R2))
+    r1, r2 = _ops.download(R1), _ops.download(R2)
+    assert np.abs(r1 - r2).max() < 1e-13 * np.abs(r1).max()
     x1, x2 = _ops.download(X1), _ops.download(X2)
     assert np.abs(x1 - x2).max() < 1e-12 * np.abs(x1).max()
+
+
+def test_factor_only_cholesky(backend):
+    """Factor-only batched Cholesky (the four-columns-per-barrier kernel without
+    its inverse phase) against scipy: orders across block boundaries, Gram matrices
+    given as split-K partial sums, the relative shift, and the breakdown status
+    of a matrix that is not positive definite."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    orders = [1, 5, 15, 16, 17, 33, 69, 90, 127, 128]
+    nparts, shift_rel = 3, 1e-3
+    G_parts, refs, descs = [], [], {"l": [], "g_off": [], "g_stride": [], "n_part": [], "r_off": [],
+                                    "rinv_off": [], "shift_rel": []}
+    g_off = r_off = 0
+    for l in orders:
+        ys = [rng.standard_normal((40 + l, l)) for _ in range(nparts)]
+        parts = [y.T @ y for y in ys]
+        G_parts.extend(p.reshape(-1) for p in parts)
+        g = sum(parts)
+        refs.append(lg.cholesky(g + shift_rel * np.trace(g) * np.eye(l), lower=False))
+        for k, v in (("l", l), ("g_off", g_off), ("g_stride", l * l), ("n_part", nparts), ("r_off", r_off),
+                     ("rinv_off", 0), ("shift_rel", shift_rel)):
+            descs[k].append(v)
+        g_off += nparts * l * l
+        r_off += l * l
+    # a symmetric matrix with a negative eigenvalue breaks down
+    bad = np.diag(np.concatenate([np.ones(20), [-1.0], np.ones(19)]))
+    G_parts.append(bad.reshape(-1))
+    for k, v in (("l", 40), ("g_off", g_off), ("g_stride", 1600), ("n_part", 1), ("r_off", r_off),
+                 ("rinv_off", 0), ("shift_rel", 0.0)):
+        descs[k].append(v)
+    G = _ops.upload(np.concatenate(G_parts))
+    R = _ops.upload(np.zeros(r_off + 1600))
+    status = _ops.upload(np.full(len(orders) + 1, -7), dtype=torch.int32)
+    _ops.batched_cholinv({k: np.array(v) for k, v in descs.items()}, G, R, None, status)
+    st = _ops.download(status)
+    # the device reports the failing pivot (index 20, plus one); the emulation only that it failed
+    assert (st[:-1] == 0).all() and st[-1] > 0 and (backend != "gpu" or st[-1] == 21)
+    r = _ops.download(R)
+    for l, ref, off in zip(orders, refs, descs["r_off"], strict=False):
+        got = r[off : off + l * l].reshape(l, l)
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), l
+        assert np.abs(np.tril(got, -1)).max() == 0.0
