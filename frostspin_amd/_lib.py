@@ -189,10 +189,12 @@ _SIGNATURES = {
         [ctypes.POINTER(LiePattern), c_i64, c_i64p, c_i64p, c_i64, c_i64p, c_i64p, c_i64p,
          c_i64p, c_i64, c_i64p, c_i64p, c_i64p],
     ),
+    "fs_lie_weights_create": (c_int, [ctypes.POINTER(LiePattern), c_i64, c_dblp, ctypes.POINTER(c_vp)]),
+    "fs_lie_weights_destroy": (c_int, [c_vp]),
     "fs_lie_plan_build": (
         c_int,
         [ctypes.POINTER(LiePattern), c_i64, c_i64p, c_i64p, c_i64, c_i64p, c_i64p, c_i64p,
-         c_i64, c_dblp, c_i64, c_i64p, ctypes.POINTER(c_vp)],
+         c_vp, c_i64, c_i64p, ctypes.POINTER(c_vp)],
     ),
     "fs_lie_plan_info": (c_int, [c_vp, c_i64p, c_dblp, c_dblp]),
     "fs_lie_plan_execute_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),

@@ -393,22 +393,54 @@ def lie_plan_tables(pattern, degen, in_block, block_off, block_rs, block_cs, *, 
     return tasks, rows, digits, shapes
 
 
-def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, w, src_size):
+class LieWeights:
+    """The structural matrices W of one permute pattern on the current device
+    (fs_lie_weights), shared by every plan built from the pattern. One object
+    per device; plans keep a reference to the object they use."""
+
+    __slots__ = ("_handles", "pattern", "w")
+
+    def __init__(self, pattern, w):
+        self.pattern = pattern
+        self.w = np.ascontiguousarray(w, dtype=np.float64).reshape(-1)
+        self._handles = {}
+
+    def handle(self):
+        dev = torch._C._cuda_getDevice()
+        h = self._handles.get(dev)
+        if h is None:
+            out = ctypes.c_void_p()
+            check(lib.fs_lie_weights_create(ctypes.byref(self.pattern), self.w.size, _lib.dblp(self.w),
+                                            ctypes.byref(out)))
+            h = self._handles[dev] = out.value
+        return h
+
+    def __del__(self):
+        if lib is not None:
+            for h in self._handles.values():
+                lib.fs_lie_weights_destroy(h)
+        self._handles = {}
+
+
+def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, weights, src_size):
     """(plan, new block shapes) of one permute straight from its pattern and
-    leg degeneracies (fs_lie_plan_build: tables and device plan in one call)."""
+    leg degeneracies (fs_lie_plan_build: tables and device plan in one call,
+    W from the pattern's LieWeights)."""
     c = np.ascontiguousarray
     degen, in_block = c(degen, dtype=np.int64), c(in_block, dtype=np.int64)
     block_off, block_rs, block_cs = (c(x, dtype=np.int64) for x in (block_off, block_rs, block_cs))
-    assert in_block.size == pattern.nt_in and w.dtype == np.float64 and w.flags.c_contiguous
+    assert in_block.size == pattern.nt_in and weights.pattern is pattern
     shapes = np.empty((pattern.nt_out, 2), dtype=np.int64)
     handle = ctypes.c_void_p()
     check(lib.fs_lie_plan_build(
         ctypes.byref(pattern), degen.size, _lib.i64p(degen), _lib.i64p(in_block), block_off.size,
-        _lib.i64p(block_off), _lib.i64p(block_rs), _lib.i64p(block_cs), w.size, _lib.dblp(w),
+        _lib.i64p(block_off), _lib.i64p(block_rs), _lib.i64p(block_cs), weights.handle(),
         int(src_size), _lib.i64p(shapes), ctypes.byref(handle),
     ))
     dst = int((shapes[:, 0] * shapes[:, 1]).sum())
-    return LiePlan._from_handle(handle.value, src_size, dst), shapes
+    plan = LiePlan._from_handle(handle.value, src_size, dst)
+    plan.w = weights  # the plan reads the weights' device copy: keep it alive
+    return plan, shapes
 
 
 class LiePlan:

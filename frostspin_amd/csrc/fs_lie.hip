@@ -341,6 +341,17 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
 
 } // namespace
 
+// Structural matrices of one permute pattern on the device, shared by every
+// plan built from that pattern (they do not depend on the degeneracies):
+// W^T of each task padded to kRowGroup rows, tasks with the same matrix
+// sharing one copy.
+struct fs_lie_weights {
+  int device = -1;
+  fs::DeviceTables tables;
+  int64_t nw = 0;                // length of the unpadded W it was built from
+  std::vector<int64_t> off;      // per pattern task: offset of its padded copy
+};
+
 struct fs_lie_plan {
   int device = -1;
   mutable fs::DeviceTables tables;  // tiles, tasks, rows, digits, W^T (256-byte aligned parts)
@@ -407,20 +418,45 @@ int check_task(const fs_lie_task &t, int64_t nrows, const fs_lie_row *rows, int6
   return FS_OK;
 }
 
-} // namespace
+// Padded device copy of W (see fs_lie_weights) for tasks given by their
+// (w_off, n_in, n_out_pad); off[t] receives task t's offset in wdev.
+template <class Get>
+void repack_w(int64_t ntasks, Get get, const double *w, std::vector<double> &wdev,
+              std::vector<int64_t> &off) {
+  std::map<std::tuple<int64_t, int64_t, int64_t>, int64_t> copies;
+  off.resize(static_cast<size_t>(ntasks));
+  for (int64_t t = 0; t < ntasks; ++t) {
+    int64_t w_off, n_in, n_out_pad;
+    get(t, w_off, n_in, n_out_pad);
+    const int64_t n_in_pad = (n_in + kRowGroup - 1) / kRowGroup * kRowGroup;
+    const auto key = std::make_tuple(w_off, n_in, n_out_pad);
+    auto found = copies.find(key);
+    if (found == copies.end()) {
+      const int64_t o = static_cast<int64_t>(wdev.size());
+      if (n_in * n_out_pad > 0) wdev.insert(wdev.end(), w + w_off, w + w_off + n_in * n_out_pad);
+      wdev.resize(static_cast<size_t>(o + n_in_pad * n_out_pad), 0.0);
+      found = copies.emplace(key, o).first;
+    }
+    off[t] = found->second;
+  }
+}
 
-extern "C" {
-
-int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
-                       const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
-                       int64_t nw, const double *w, int64_t src_size, int64_t dst_size,
-                       fs_lie_plan **plan) {
-  return fs::guarded([&]() -> int {
+// Plan from tables; W either uploaded with the plan (shared == nullptr) or
+// taken from a shared weights object built from the same pattern.
+int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const fs_lie_row *rows,
+                int64_t ndigits, const fs_lie_digit *digits, int64_t nw, const double *w,
+                const fs_lie_weights *shared, int64_t src_size, int64_t dst_size,
+                fs_lie_plan **plan) {
+  {
     FS_CHECK_ARG(plan != nullptr, "lie plan: null output");
     *plan = nullptr;
+    if (shared) {
+      FS_CHECK_ARG(static_cast<int64_t>(shared->off.size()) == ntasks, "lie plan: weights of another pattern");
+      nw = shared->nw;
+    }
     FS_CHECK_ARG(ntasks >= 0 && nrows >= 0 && ndigits >= 0 && nw >= 0, "lie plan: negative size");
     FS_CHECK_ARG(ntasks == 0 || (tasks && rows && digits), "lie plan: null table");
-    FS_CHECK_ARG(nw == 0 || w, "lie plan: null W");
+    FS_CHECK_ARG(shared || nw == 0 || w, "lie plan: null W");
     FS_CHECK_ARG(src_size >= 0 && dst_size >= 0, "lie plan: negative buffer size");
     FS_CHECK_ARG(nrows < (int64_t(1) << 31) && ndigits < (int64_t(1) << 31),
                  "lie plan: tables too large");
@@ -428,28 +464,27 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     std::vector<TaskDev> tdev;
     std::vector<TileDev> tiles;
     std::vector<int32_t> reg_ids, wide_ids;
+    for (int64_t t = 0; t < ntasks; ++t) {
+      int st = check_task(tasks[t], nrows, rows, ndigits, digits, nw, src_size, dst_size);
+      if (st != FS_OK) return st;
+    }
     // device copy of W^T with every task's rows padded to a multiple of
-    // kRowGroup; tasks sharing a structural matrix share its copy
+    // kRowGroup (tasks sharing a structural matrix share its copy), unless
+    // the pattern's shared copy is used
     std::vector<double> wdev;
-    std::map<std::tuple<int64_t, int64_t, int64_t>, int64_t> w_copies;
+    std::vector<int64_t> w_off_own;
+    if (!shared)
+      repack_w(ntasks, [&](int64_t t, int64_t &wo, int64_t &ni, int64_t &nop) {
+        wo = tasks[t].w_off; ni = tasks[t].n_in; nop = tasks[t].n_out_pad;
+      }, w, wdev, w_off_own);
+    const std::vector<int64_t> &w_offs = shared ? shared->off : w_off_own;
     tdev.reserve(static_cast<size_t>(ntasks));
     int64_t max_in = 0, max_wide_in = 0;
     for (int64_t t = 0; t < ntasks; ++t) {
       const fs_lie_task &s = tasks[t];
-      int st = check_task(s, nrows, rows, ndigits, digits, nw, src_size, dst_size);
-      if (st != FS_OK) return st;
       TaskDev d;
       const int64_t n_in_pad = (s.n_in + kRowGroup - 1) / kRowGroup * kRowGroup;
-      const auto key = std::make_tuple(s.w_off, s.n_in, s.n_out_pad);
-      auto found = w_copies.find(key);
-      if (found == w_copies.end()) {
-        const int64_t off = static_cast<int64_t>(wdev.size());
-        if (s.n_in * s.n_out_pad > 0)
-          wdev.insert(wdev.end(), w + s.w_off, w + s.w_off + s.n_in * s.n_out_pad);
-        wdev.resize(static_cast<size_t>(off + n_in_pad * s.n_out_pad), 0.0);
-        found = w_copies.emplace(key, off).first;
-      }
-      d.w_off = found->second;
+      d.w_off = w_offs[t];
       d.e_size = static_cast<int32_t>(s.e_size);
       d.n_in = static_cast<int32_t>(s.n_in);
       d.n_out = static_cast<int32_t>(s.n_out);
@@ -512,6 +547,9 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     put(o_dig, ddev.data(), b_dig);
     put(o_w, wdev.data(), b_w);
     FS_HIP_CHECK(hipGetDevice(&p->device));
+    if (shared)
+      FS_CHECK_ARG(shared->device == p->device, "lie plan: weights on device %d, plan on device %d",
+                   shared->device, p->device);
     int st = p->tables.upload(image.data(), total);
     if (st != FS_OK) return st;
     const char *mem = p->tables.data();
@@ -524,7 +562,8 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     p->tasks = reinterpret_cast<const TaskDev *>(d_tasks);
     p->rows = reinterpret_cast<const RowDev *>(d_rows);
     p->digits = reinterpret_cast<const DigitDev *>(d_dig);
-    p->w = reinterpret_cast<const double *>(d_w);
+    p->w = shared ? reinterpret_cast<const double *>(shared->tables.data())
+                  : reinterpret_cast<const double *>(d_w);
     p->ntiles = static_cast<int64_t>(tiles.size());
     p->ntiles_wide = static_cast<int64_t>(tiles.size() - n_reg_tiles);
     p->wide_rows = static_cast<int32_t>(max_wide_in);
@@ -533,6 +572,55 @@ int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
     p->dst_size = dst_size;
     *plan = p.release();
     return FS_OK;
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int fs_lie_plan_create(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows,
+                       const fs_lie_row *rows, int64_t ndigits, const fs_lie_digit *digits,
+                       int64_t nw, const double *w, int64_t src_size, int64_t dst_size,
+                       fs_lie_plan **plan) {
+  return fs::guarded([&]() -> int {
+    return create_impl(ntasks, tasks, nrows, rows, ndigits, digits, nw, w, nullptr, src_size,
+                       dst_size, plan);
+  });
+}
+
+int fs_lie_weights_create(const fs_lie_pattern *pattern, int64_t nw, const double *w,
+                          fs_lie_weights **weights) {
+  return fs::guarded([&]() -> int {
+    FS_CHECK_ARG(pattern && weights, "lie weights: null argument");
+    *weights = nullptr;
+    const int64_t ns = pattern->ntasks;
+    FS_CHECK_ARG(ns >= 0 && nw >= 0 && (nw == 0 || w), "lie weights: bad W");
+    for (int64_t t = 0; t < ns; ++t) {
+      const int64_t wo = pattern->w_off[t], ni = pattern->n_in[t], nop = pattern->n_out_pad[t];
+      FS_CHECK_ARG(wo >= 0 && ni >= 0 && nop >= 0 && nop % kOutChunk == 0 && wo + ni * nop <= nw,
+                   "lie weights: task %lld out of W", static_cast<long long>(t));
+    }
+    auto q = std::make_unique<fs_lie_weights>();
+    std::vector<double> wdev;
+    repack_w(ns, [&](int64_t t, int64_t &wo, int64_t &ni, int64_t &nop) {
+      wo = pattern->w_off[t]; ni = pattern->n_in[t]; nop = pattern->n_out_pad[t];
+    }, w, wdev, q->off);
+    FS_HIP_CHECK(hipGetDevice(&q->device));
+    q->nw = nw;
+    int st = q->tables.upload(wdev.data(), wdev.size() * sizeof(double));
+    if (st != FS_OK) return st;
+    *weights = q.release();
+    return FS_OK;
+  });
+}
+
+int fs_lie_weights_destroy(fs_lie_weights *weights) {
+  return fs::guarded([&]() -> int {
+    if (!weights) return FS_OK;
+    const int st = weights->tables.release();
+    delete weights;
+    return st;
   });
 }
 
@@ -618,10 +706,11 @@ int fs_lie_plan_destroy(fs_lie_plan *plan) {
 
 int fs_lie_plan_build(const fs_lie_pattern *pattern, int64_t ndegen, const int64_t *degen,
                       const int64_t *in_block, int64_t nblocks, const int64_t *block_off,
-                      const int64_t *block_rs, const int64_t *block_cs, int64_t nw,
-                      const double *w, int64_t src_size, int64_t *shapes, fs_lie_plan **plan) {
+                      const int64_t *block_rs, const int64_t *block_cs,
+                      const fs_lie_weights *weights, int64_t src_size, int64_t *shapes,
+                      fs_lie_plan **plan) {
   return fs::guarded([&]() -> int {
-    FS_CHECK_ARG(pattern && plan && shapes, "lie plan build: null argument");
+    FS_CHECK_ARG(pattern && plan && shapes && weights, "lie plan build: null argument");
     *plan = nullptr;
     const int64_t ns = pattern->ntasks, nd = pattern->ndim;
     FS_CHECK_ARG(ns >= 0 && nd >= 1 && nd <= FS_LIE_MAX_LEGS, "lie plan build: bad pattern");
@@ -640,10 +729,10 @@ int fs_lie_plan_build(const fs_lie_pattern *pattern, int64_t ndegen, const int64
     static_assert(sizeof(fs_lie_task) == 8 * sizeof(int64_t) && sizeof(fs_lie_row) == 3 * sizeof(int64_t) &&
                       sizeof(fs_lie_digit) == 5 * sizeof(int64_t),
                   "table layouts");
-    return fs_lie_plan_create(ns, reinterpret_cast<const fs_lie_task *>(tasks.data()), nrows,
-                              reinterpret_cast<const fs_lie_row *>(rows.data()), ns * nd,
-                              reinterpret_cast<const fs_lie_digit *>(digits.data()), nw, w, src_size,
-                              dst_size, plan);
+    return create_impl(ns, reinterpret_cast<const fs_lie_task *>(tasks.data()), nrows,
+                       reinterpret_cast<const fs_lie_row *>(rows.data()), ns * nd,
+                       reinterpret_cast<const fs_lie_digit *>(digits.data()), 0, nullptr, weights,
+                       src_size, dst_size, plan);
   });
 }
 

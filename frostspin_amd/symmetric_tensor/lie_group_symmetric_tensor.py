@@ -71,7 +71,7 @@ class _PermuteStructure:
     __slots__ = (
         "W", "idc_in", "idc_out", "idr_in", "idr_out", "ic_in", "ic_out", "in_a", "in_b",
         "in_t", "in_targets", "ir_in", "ir_out", "n_in", "n_out", "n_out_pad", "out_a",
-        "out_b", "out_t", "out_targets", "sig_idx", "w_off", "pattern", "_keep",
+        "out_b", "out_t", "out_targets", "sig_idx", "w_off", "pattern", "weights", "_keep",
     )
 
     def bind(self, ndim, nrr, nrr_out, axes, nirr):
@@ -91,6 +91,7 @@ class _PermuteStructure:
             ndim, nrr, nrr_out, len(self.n_in), len(self.in_targets), len(self.out_targets),
             *(_lib.i64p(arrs[f]) for f, _ in _lib.LiePattern._fields_[6:]),
         )
+        self.weights = _ops.LieWeights(self.pattern, self.W)
 
 
 class _DenseStructure:
@@ -106,6 +107,7 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
 
     _unitary_cache = {}
     _permute_structures = {}
+    _structures_by_bytes = {}
     _dense_structures = {}
     _side_cache = {}
     _plans = OrderedDict()
@@ -307,13 +309,19 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
             if hit is not None:
                 cls._plans.move_to_end(key)
                 return hit
-        irreps_per_leg = tuple(tuple(int(x) for x in r[1]) for r in reps)
-        sig = tuple(bool(s) for s in self._signature)
-        st = cls._permute_structure(irreps_per_leg, sig, nrr, axes, nrr_out)
+        # irreps-only structure, found by the irreps' bytes; the tuples the
+        # structure is built from are formed only the first time
+        skey = (cls, tuple(np.ascontiguousarray(r[1], dtype=np.int64).tobytes() for r in reps),
+                key[2], nrr, axes, nrr_out)
+        st = cls._structures_by_bytes.get(skey)
+        if st is None:
+            irreps_per_leg = tuple(tuple(int(x) for x in r[1]) for r in reps)
+            sig = tuple(bool(s) for s in self._signature)
+            st = cls._structures_by_bytes[skey] = cls._permute_structure(irreps_per_leg, sig, nrr, axes, nrr_out)
         degen = np.concatenate([np.asarray(r[0], dtype=np.int64) for r in reps])
         rs_b, cs_b = self._block_strides()
         plan, shapes = _ops.lie_plan_build(
-            st.pattern, degen, self._present_map(st.in_targets), self._boffs, rs_b, cs_b, st.W,
+            st.pattern, degen, self._present_map(st.in_targets), self._boffs, rs_b, cs_b, st.weights,
             self._data.numel(),
         )
         hit = (plan, shapes, st.out_targets)

@@ -251,14 +251,26 @@ int fs_lie_plan_tables(const fs_lie_pattern *pattern, int64_t ndegen, const int6
                        const int64_t *block_rs, const int64_t *block_cs, int64_t *tasks,
                        int64_t nrows, int64_t *rows, int64_t *digits, int64_t *shapes);
 
-/* fs_lie_plan_tables followed by fs_lie_plan_create in one call: the plan of
- * one permute straight from its pattern and degeneracies (the tables stay on
- * the host side of the library). shapes: the new block shapes [nt_out][2];
- * the destination size is their sum of products. */
+/* Structural matrices of one pattern (W as for fs_lie_plan_create, indexed by
+ * the pattern's w_off / n_in / n_out_pad) uploaded once to the current device
+ * and shared by every plan built from that pattern: they depend on the irreps
+ * only, not on the degeneracies. Must outlive the plans that use it. */
+typedef struct fs_lie_weights fs_lie_weights;
+
+int fs_lie_weights_create(const fs_lie_pattern *pattern, int64_t nw, const double *w,
+                          fs_lie_weights **weights);
+int fs_lie_weights_destroy(fs_lie_weights *weights);
+
+/* fs_lie_plan_tables followed by plan creation in one call: the plan of one
+ * permute straight from its pattern and degeneracies, the tables staying on
+ * the host side of the library and W taken from the pattern's weights.
+ * shapes: the new block shapes [nt_out][2]; the destination size is the sum
+ * of their products. */
 int fs_lie_plan_build(const fs_lie_pattern *pattern, int64_t ndegen, const int64_t *degen,
                       const int64_t *in_block, int64_t nblocks, const int64_t *block_off,
-                      const int64_t *block_rs, const int64_t *block_cs, int64_t nw,
-                      const double *w, int64_t src_size, int64_t *shapes, fs_lie_plan **plan);
+                      const int64_t *block_rs, const int64_t *block_cs,
+                      const fs_lie_weights *weights, int64_t src_size, int64_t *shapes,
+                      fs_lie_plan **plan);
 
 /* Work of one execution: flops (2 * sum n_in*n_out*E) and coefficients read
  * and written (sum (n_in + n_out) * E). */

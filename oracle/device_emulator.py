@@ -326,8 +326,9 @@ class EmulatedLiePlan:
                 d[idx] = Y[o]
 
 
-def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, w, src_size):
+def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, weights, src_size):
     """fs_lie_plan_build in emulation: the native host tables, then the numpy plan."""
+    w = weights.w
     n_in = np.ctypeslib.as_array(pattern.n_in, shape=(pattern.ntasks,))
     n_out = np.ctypeslib.as_array(pattern.n_out, shape=(pattern.ntasks,))
     tasks, rows, digits, shapes = _ops.lie_plan_tables(
