@@ -347,7 +347,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
 // sharing one copy.
 struct fs_lie_weights {
   int device = -1;
-  fs::DeviceTables tables;
+  mutable fs::DeviceTables tables;  // note_stream() from the plans' launches
   int64_t nw = 0;                // length of the unpadded W it was built from
   std::vector<int64_t> off;      // per pattern task: offset of its padded copy
 };
@@ -355,6 +355,7 @@ struct fs_lie_weights {
 struct fs_lie_plan {
   int device = -1;
   mutable fs::DeviceTables tables;  // tiles, tasks, rows, digits, W^T (256-byte aligned parts)
+  const fs_lie_weights *weights = nullptr;  // shared W^T (then not in `tables`)
   const TileDev *tiles = nullptr;
   const TaskDev *tasks = nullptr;
   const RowDev *rows = nullptr;
@@ -564,6 +565,7 @@ int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const f
     p->digits = reinterpret_cast<const DigitDev *>(d_dig);
     p->w = shared ? reinterpret_cast<const double *>(shared->tables.data())
                   : reinterpret_cast<const double *>(d_w);
+    p->weights = shared;
     p->ntiles = static_cast<int64_t>(tiles.size());
     p->ntiles_wide = static_cast<int64_t>(tiles.size() - n_reg_tiles);
     p->wide_rows = static_cast<int32_t>(max_wide_in);
@@ -666,6 +668,7 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
       return e ? std::atoi(e) : 0;
     }();
     plan->tables.note_stream(fs::as_stream(stream));
+    if (plan->weights) plan->weights->tables.note_stream(fs::as_stream(stream));  // for its release
     if (mode == 1) {
       const size_t lds = static_cast<size_t>(plan->lds_rows) * kLanes * sizeof(double);
       hipLaunchKernelGGL(lie_transform_kernel, dim3(static_cast<unsigned>(plan->ntiles)),
