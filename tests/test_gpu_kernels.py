@@ -86,17 +86,20 @@ def test_split_gemm_precision(gpu):
     assert err < 1e-5, f"split gemm rel err {err:.2e}"  # FP32 accumulation inside each K chunk
 
 
-def test_jacobi_svd_batched(gpu):
-    rng = np.random.default_rng(3)
-    shapes = [(1, 1), (5, 3), (3, 5), (40, 40), (100, 37), (20, 130), (200, 200)]
+def _graded_lower(rng, n, decay):
+    """Transposed R factor of a matrix with singular values exp(-decay k / n):
+    the lower-triangular small matrix the randomized SVD hands to the Jacobi SVD."""
+    s = np.exp(-np.arange(n) * (decay / n))
+    u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return np.ascontiguousarray(np.linalg.qr(((u * s) @ v.T).T, mode="r").T)
+
+
+def _check_batched_svd(mats):
     parts, descs = [], []
     ao = uo = so = vo = wo = 0
-    mats = []
-    for m, n in shapes:
-        a = rng.standard_normal((m, n))
-        if m == 40:
-            a[:, 5] = a[:, 6]  # rank deficient
-        mats.append(a)
+    for a in mats:
+        m, n = a.shape
         q = min(m, n)
         parts.append(a.reshape(-1))
         descs.append((m, n, ao, n, 1, uo, so, vo, wo))
@@ -123,6 +126,43 @@ def test_jacobi_svd_batched(gpu):
         e_u = float(np.abs(uu[:, nz].T @ uu[:, nz] - np.eye(nz.sum())).max())
         e_v = float(np.abs(vv @ vv.T - np.eye(q)).max())
         assert e_s < 1e-13 and e_r < 1e-12 and e_u < 1e-12 and e_v < 1e-12, (tag, e_s, e_r, e_u, e_v)
+
+
+def test_jacobi_svd_batched(gpu):
+    # p <= 128 in LDS takes the register form, (20, 130) the strided form in
+    # LDS and (200, 200) the strided form in the work buffer, all in one launch
+    rng = np.random.default_rng(3)
+    shapes = [(1, 1), (5, 3), (3, 5), (40, 40), (100, 37), (20, 130), (200, 200)]
+    mats = []
+    for m, n in shapes:
+        a = rng.standard_normal((m, n))
+        if m == 40:
+            a[:, 5] = a[:, 6]  # rank deficient
+        mats.append(a)
+    _check_batched_svd(mats)
+
+
+@pytest.mark.parametrize(
+    "shapes",
+    [
+        [(24, 24), (7, 31), (2, 2)],  # register length 2 (p <= 32)
+        [(45, 45), (64, 50), (33, 17)],  # 4 (p <= 64)
+        [(70, 70), (96, 12), (65, 65)],  # 6 (p <= 96)
+        [(128, 128), (127, 3), (97, 97)],  # 8 (p <= 128)
+    ],
+)
+def test_jacobi_svd_register_lengths(gpu, shapes):
+    rng = np.random.default_rng(len(shapes) + shapes[0][0])
+    mats = []
+    for m, n in shapes:
+        if m == n:
+            a = _graded_lower(rng, m, 20.0)
+            if m > 40:
+                a[:, 3] = a[:, 4]  # rank deficient
+        else:
+            a = rng.standard_normal((m, n))
+        mats.append(a)
+    _check_batched_svd(mats)
 
 
 def test_norm_trace_copy_fill(gpu):
