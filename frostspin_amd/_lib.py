@@ -101,6 +101,18 @@ class TriInvDesc(ctypes.Structure):
     _fields_ = [("l", c_i64), ("r_off", c_i64), ("x_off", c_i64)]
 
 
+class CholQrDesc(ctypes.Structure):
+    _fields_ = [
+        ("m", c_i64),
+        ("l", c_i64),
+        ("y_off", c_i64),
+        ("y_rs", c_i64),
+        ("y_cs", c_i64),
+        ("q_off", c_i64),
+        ("rt_off", c_i64),
+    ]
+
+
 class TrsmDesc(ctypes.Structure):
     _fields_ = [
         ("m", c_i64),
@@ -226,6 +238,11 @@ _SIGNATURES = {
         [c_int, ctypes.POINTER(CholDesc), c_vp, c_vp, c_vp, c_vp],
     ),
     "fs_batched_triinv_f64": (c_int, [c_int, ctypes.POINTER(TriInvDesc), c_vp, c_vp, c_vp, c_vp]),
+    "fs_cholqr_work_size": (c_int, [c_int, ctypes.POINTER(CholQrDesc), c_int, c_int, ctypes.POINTER(c_i64)]),
+    "fs_cholqr_f64": (
+        c_int,
+        [c_int, ctypes.POINTER(CholQrDesc), c_vp, c_vp, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_vp],
+    ),
     "fs_batched_cholinv_f64": (
         c_int,
         [c_int, ctypes.POINTER(CholInvDesc), c_vp, c_vp, c_vp, c_vp, c_vp],

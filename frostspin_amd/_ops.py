@@ -661,6 +661,28 @@ def batched_triinv(descs, R, X, status=None):
         check(lib.fs_batched_triinv_f64(n, arr, _ptr(R), _ptr(X), _ptr(status), _stream()))
 
 
+def cholqr(descs, Y, Q, Rt, status, *, passes=3, shifted=True):
+    """descs: (m, l, y_off, y_rs, y_cs, q_off, rt_off) per matrix (or a dict of
+    columns): multi-pass shifted CholeskyQR of every Y_i (m x l, l <= 128) in
+    one native call, Q packed row-major at q_off, the total triangular factor
+    at rt_off when Rt is given (passes >= 2); status: device int32 of
+    passes * len(descs), pass-major."""
+    n = _ndesc(descs)
+    if n == 0:
+        return
+    tab, arr = _desc_table(_lib.CholQrDesc, descs)
+    size = ctypes.c_int64()
+    check(lib.fs_cholqr_work_size(n, arr, int(passes), int(Rt is not None), ctypes.byref(size)))
+    work = empty(int(size.value))
+    m = tab["m"].astype(np.float64)
+    l = tab["l"].astype(np.float64)
+    flop = float(passes * (4.0 * m * l * l + 2.0 * l**3 / 3.0).sum())
+    with _timed("cholqr", flop):
+        check(lib.fs_cholqr_f64(
+            n, arr, _ptr(Y), _ptr(Q), None if Rt is None else _ptr(Rt), _ptr(work), int(size.value),
+            int(passes), int(bool(shifted)), _ptr(status), _stream()))
+
+
 def batched_trsm_right_upper(descs, Y, R, O):
     """descs: sequence of (m, l, y_off, y_rs, y_cs, r_off, o_off): O = Y R^-1."""
     n = _ndesc(descs)

@@ -20,6 +20,7 @@ INCLUDE = ROOT / "include"
 LIB_NAME = "libfrostspin_amd.so"
 SOURCES = [
     "fs_planner.cpp",
+    "fs_cholqr.cpp",
     "fs_permute.hip",
     "fs_gemm.hip",
     "fs_svd.hip",

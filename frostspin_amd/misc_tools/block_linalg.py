@@ -231,7 +231,13 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False, statuses=None):
 
     statuses: if a list, the per-pass device status vectors are appended to it
     instead of being read back (no host synchronisation); matrices whose
-    factorisation broke down are then left to the caller to redo."""
+    factorisation broke down are then left to the caller to redo.
+
+    With every l_i <= 128 the passes run as one native call (fs_cholqr_f64:
+    the same launches, issued without returning to Python between them),
+    except under a KernelTimer, which times each launch on its own."""
+    if NATIVE_CHOLQR and _ops.KernelTimer.active is None and _chain_applies(Y, passes, want_r):
+        return _cholqr_chain(Y, shifted, passes, want_r, statuses)
     eps = np.finfo(np.float64).eps
     cur = Y
     Rtot = None
@@ -278,20 +284,58 @@ def cholqr(Y, *, shifted=True, passes=3, want_r=False, statuses=None):
             continue
         bad = np.nonzero(_ops.download(status))[0]
         if bad.size:
-            # breakdown (numerically rank-deficient beyond the shift): Householder
-            # on the original inputs of those matrices
-            Qh, Rh = qr(Y.select(bad), want_r=want_r)
-            for src, dst in ((Qh, out), (Rh, Rtot)):
-                if src is None:
-                    continue
-                descs = [
-                    (src.rows[j], src.cols[j], src.off[j], src.rs[j], src.cs[j], dst.off[i], dst.cols[i], 1,
-                     0, 0, 0, 1.0, 0.0)
-                    for j, i in enumerate(bad)
-                ]
-                _ops.copy2d(descs, src.buf, dst.buf)
+            _householder_splice(Y, bad, out, Rtot, want_r)
         cur = out
     return (cur, Rtot) if want_r else cur
+
+
+NATIVE_CHOLQR = True  # tests switch it off to compare with the launch-by-launch path
+
+
+def _chain_applies(Y, passes, want_r):
+    return (
+        len(Y) > 0
+        and not isinstance(Y.buf, list)
+        and 1 <= passes <= 3
+        and (passes >= 2 or not want_r)
+        and Y.rows.min() >= 1
+        and Y.cols.min() >= 1
+        and Y.cols.max() <= _ops.CHOLINV_MAX_ORDER
+    )
+
+
+def _cholqr_chain(Y, shifted, passes, want_r, statuses):
+    n = len(Y)
+    Q = Mats.packed(Y.rows, Y.cols)
+    Rt = Mats.packed(Y.cols, Y.cols) if want_r else None
+    status = _ops.empty(passes * n, dtype=torch.int32)
+    _ops.cholqr(
+        {"m": Y.rows, "l": Y.cols, "y_off": Y.off, "y_rs": Y.rs, "y_cs": Y.cs, "q_off": Q.off,
+         "rt_off": Rt.off if want_r else 0},
+        Y.buf, Q.buf, None if Rt is None else Rt.buf, status, passes=passes, shifted=shifted,
+    )
+    if statuses is not None:
+        statuses.extend(status.view(passes, n).unbind(0))
+    else:
+        bad = np.nonzero(_ops.download(status).reshape(passes, n).any(axis=0))[0]
+        if bad.size:
+            _householder_splice(Y, bad, Q, Rt, want_r)
+    return (Q, Rt) if want_r else Q
+
+
+def _householder_splice(Y, bad, out, Rtot, want_r):
+    """Breakdown (numerically rank-deficient beyond the shift): Householder QR
+    of the original inputs of those matrices, copied over their results."""
+    Qh, Rh = qr(Y.select(bad), want_r=want_r)
+    for src, dst in ((Qh, out), (Rh, Rtot)):
+        if src is None:
+            continue
+        descs = [
+            (src.rows[j], src.cols[j], src.off[j], src.rs[j], src.cs[j], dst.off[i], dst.cols[i], 1,
+             0, 0, 0, 1.0, 0.0)
+            for j, i in enumerate(bad)
+        ]
+        _ops.copy2d(descs, src.buf, dst.buf)
 
 
 def _orthonormal_columns(Y, passes=3, statuses=None, robust=False):

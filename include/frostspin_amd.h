@@ -449,6 +449,35 @@ typedef struct fs_triinv_desc {
 int fs_batched_triinv_f64(int nmat, const fs_triinv_desc *descs, const double *R, double *X,
                           const int32_t *status, void *stream);
 
+/*
+ * Multi-pass shifted CholeskyQR of tall matrices in one call: Y_i (m x l,
+ * 1 <= l <= 128, element (r, c) at Y + y_off + r*y_rs + c*y_cs) -> Q_i (m x l
+ * row-major at Q + q_off) with orthonormal columns. Each pass issues, on one
+ * stream, the split-K Gram product G = Y^T Y (fs_grouped_gemm_f64; row chunks
+ * of 192, at most 16), the factor-only fs_batched_cholinv_f64 of G + s I
+ * (s = 11 (m l + l (l + 1)) eps relative on the first pass when shifted),
+ * fs_batched_triinv_f64 and Y <- Y R^-1 (fs_grouped_gemm_f64); the next pass
+ * reads the packed result. Rt (may be NULL; needs passes >= 2) receives
+ * R = R_{passes-1} ... R_0 (l x l row-major at Rt + rt_off), so Y = Q R.
+ * status (passes * nmat int32) holds each pass's factorisation status; a
+ * breakdown turns that matrix's Q into NaN, leaves its R unspecified and
+ * leaves the fallback to the caller. work holds fs_cholqr_work_size doubles. The orthonormalisations of
+ * the randomized SVD (misc_tools/svd_tools.py:154-197 of the reference, which
+ * uses Householder QR there); no single counterpart in the reference.
+ */
+typedef struct fs_cholqr_desc {
+  int64_t m, l;
+  int64_t y_off, y_rs, y_cs;
+  int64_t q_off, rt_off;
+} fs_cholqr_desc;
+
+int fs_cholqr_work_size(int nmat, const fs_cholqr_desc *descs, int passes, int want_r,
+                        int64_t *doubles);
+
+int fs_cholqr_f64(int nmat, const fs_cholqr_desc *descs, const double *Y, double *Q, double *Rt,
+                  double *work, int64_t work_doubles, int passes, int shifted, int32_t *status,
+                  void *stream);
+
 typedef struct fs_trsm_desc {
   int64_t m, l, y_off, y_rs, y_cs, r_off, o_off;
 } fs_trsm_desc;

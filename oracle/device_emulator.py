@@ -176,6 +176,39 @@ def batched_triinv(descs, R, X, status=None):
                 rr, np.eye(l), lower=False, check_finite=False).reshape(-1)
 
 
+def cholqr(descs, Y, Q, Rt, status, *, passes=3, shifted=True):
+    """Multi-pass shifted CholeskyQR per matrix: G = Y^T Y, R = chol(G + s tr(G) I)
+    (s = 11 (m l + l (l + 1)) eps on the first pass when shifted), Y <- Y R^-1;
+    Q = the last Y, Rt = R_{passes-1} ... R_0; a failed factorisation sets that
+    pass's status and turns the matrix into NaN."""
+    y, q, st = _np(Y), _np(Q), _np(status)
+    rt = None if Rt is None else _np(Rt)
+    rows = _desc_rows(descs, ("m", "l", "y_off", "y_rs", "y_cs", "q_off", "rt_off"))
+    n = len(rows)
+    eps = np.finfo(np.float64).eps
+    for i, (m, l, yo, yrs, ycs, qo, rto) in enumerate(rows):
+        cur = np.array(_view(y, yo, m, l, yrs, ycs))
+        rtot = None
+        for p in range(passes):
+            gm = cur.T @ cur
+            gm = np.triu(gm) + np.triu(gm, 1).T
+            shift = 11.0 * (m * l + l * (l + 1.0)) * eps if shifted and p == 0 else 0.0
+            gm = gm + shift * np.trace(gm) * np.eye(l)
+            try:
+                r = np.linalg.cholesky(gm).T
+                st[p * n + i] = 0
+                x = scipy.linalg.solve_triangular(r, np.eye(l), lower=False, check_finite=False)
+            except np.linalg.LinAlgError:
+                st[p * n + i] = 1
+                r = np.full((l, l), np.nan)
+                x = r
+            cur = cur @ x
+            rtot = r if rtot is None else r @ rtot
+        q[qo : qo + m * l] = cur.reshape(-1)
+        if rt is not None:
+            rt[rto : rto + l * l] = rtot.reshape(-1)
+
+
 def batched_trsm_right_upper(descs, Y, R, O):
     y, r, o = _np(Y), _np(R), _np(O)
     for m, l, yo, yrs, ycs, ro, oo in descs:
@@ -372,7 +405,7 @@ def install(monkeypatch):
     monkeypatch.setattr(_ops, "LiePlan", EmulatedLiePlan)
     for name in (
         "grouped_gemm", "batched_svd", "batched_qr", "batched_cholesky", "batched_cholinv",
-        "batched_triinv", "batched_trsm_right_upper", "lie_plan_build",
+        "batched_triinv", "cholqr", "batched_trsm_right_upper", "lie_plan_build",
         "weighted_norm2", "scale_", "scale_rsqrt_dev_",
         "axpby_", "copy2d", "gather2d", "weighted_trace", "fill_normal_",
     ):

@@ -110,6 +110,59 @@ def test_cholqr_q_and_r(backend):
         assert np.abs(q @ r - a).max() < 1e-12 * np.abs(a).max()
 
 
+def test_cholqr_one_call_matches_launch_by_launch(backend, monkeypatch):
+    """The one-call CholeskyQR (fs_cholqr_f64) issues the launches of the
+    launch-by-launch path (split-K Gram, factor-only Cholesky, blocked inverse,
+    Y R^-1, running product of the factors) with the same descriptors: on the
+    device Q, R and every pass's status words are bit-identical, for 1, 2 and 3
+    passes, with and without R, on packed and transposed (strided) inputs whose
+    Gram products split into 1 to 16 parts, including an all-zero block that
+    breaks the factorisation down (its Q is NaN on both paths, its R unspecified)."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    shapes = ((5, 1), (64, 64), (250, 37), (3100, 20), (4000, 128), (130, 90))
+    mats = [rng.standard_normal(sh) for sh in shapes]
+    deficient = rng.standard_normal((200, 30))
+    deficient[:, 5:10] = 0.0
+    mats += [deficient, np.zeros((120, 16))]
+    inputs = [_pack(mats), _pack([rng.standard_normal((40, 300)), rng.standard_normal((7, 200))]).T()]
+    exact = backend == "gpu"
+    for Y in inputs:
+        for passes, want_r in ((1, False), (2, False), (2, True), (3, False), (3, True)):
+            res = []
+            for native in (True, False):
+                monkeypatch.setattr(bl, "NATIVE_CHOLQR", native)
+                statuses = []
+                out = bl.cholqr(Y, passes=passes, want_r=want_r, statuses=statuses)
+                Q, R = out if want_r else (out, None)
+                res.append((_unpack(Q), None if R is None else _unpack(R), _ops.download(torch.stack(statuses))))
+            (q1, r1, s1), (q2, r2, s2) = res
+            tag = (len(Y), passes, want_r)
+            assert np.array_equal(s1, s2), tag
+            # a breakdown leaves Q all NaN and R unspecified
+            broken = s1.reshape(passes, len(Y)).any(axis=0)
+            for i in np.nonzero(broken)[0]:
+                assert np.isnan(q1[i]).all() and np.isnan(q2[i]).all(), tag
+            pairs = [(a, b) for i, (a, b) in enumerate(zip(q1, q2, strict=True)) if not broken[i]]
+            if want_r:
+                pairs += [(a, b) for i, (a, b) in enumerate(zip(r1, r2, strict=True)) if not broken[i]]
+            for a, b in pairs:
+                if exact:
+                    assert np.array_equal(a, b, equal_nan=True), tag
+                else:
+                    ok = np.isfinite(a)
+                    assert np.array_equal(ok, np.isfinite(b)), tag
+                    assert np.abs(a[ok] - b[ok]).max(initial=0.0) < 1e-12 * max(1.0, np.abs(a[ok]).max(initial=0.0)), tag
+    # read-back mode: the zero block is redone with Householder QR on both paths
+    for native in (True, False):
+        monkeypatch.setattr(bl, "NATIVE_CHOLQR", native)
+        Q, R = bl.cholqr(inputs[0], want_r=True)
+        for a, q, r in zip(mats, _unpack(Q), _unpack(R), strict=True):
+            assert np.abs(q.T @ q - np.eye(q.shape[1])).max() < 1e-12
+            assert np.abs(q @ r - a).max() <= 1e-12 * max(1.0, np.abs(a).max())
+
+
 def test_rsvd_recovers_from_cholesky_breakdown(backend, monkeypatch):
     """An all-zero block breaks CholeskyQR down (zero pivot even with the shift);
     the deferred status check redoes it with Householder QR and splices the
