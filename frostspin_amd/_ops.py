@@ -143,7 +143,28 @@ class KernelTimer:
         return out
 
 
-class _timed:
+class _Untimed:
+    __slots__ = ()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+
+_UNTIMED = _Untimed()
+
+
+def _timed(kind, work, dispatches=1):
+    """Event pair around one native call while a KernelTimer is active; a shared
+    no-op otherwise. work may be a callable, evaluated only under a timer."""
+    if KernelTimer.active is None:
+        return _UNTIMED
+    return _Timed(kind, work() if callable(work) else work, dispatches)
+
+
+class _Timed:
     __slots__ = ("dispatches", "e0", "kind", "work")
 
     def __init__(self, kind, work, dispatches=1):
@@ -511,6 +532,10 @@ class LiePlan:
 # ---------------------------------------------------------------------------
 
 
+def _gemm_dispatches():
+    return lib.fs_gemm_last_dispatches()
+
+
 def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
     """groups: (n, 11) int64 array, or a sequence of 11-tuples, (m, n, k, a_off,
     a_rs, a_cs, b_off, b_rs, b_cs, c_off, c_ld). A, B, C: one device buffer
@@ -534,7 +559,9 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
     # fs_gemm_group layout: 11 int64 per group
     garr = np.ascontiguousarray(groups, dtype=np.int64).reshape(n, 11)
     arr = garr.ctypes.data_as(ctypes.POINTER(_lib.GemmGroup))
-    flop = 2.0 * float((garr[:, 0].astype(np.float64) * garr[:, 1] * garr[:, 2]).sum())
+    def flop():
+        return 2.0 * float((garr[:, 0].astype(np.float64) * garr[:, 1] * garr[:, 2]).sum())
+
     # the library launches one kernel per 32 groups that hold a non-empty group
     if multi:
         ptrs = {}
@@ -542,11 +569,11 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
             (_lib.c_vp * n)(*[ptrs[id(t)] if id(t) in ptrs else ptrs.setdefault(id(t), _ptr(t)) for t in lst])
             for lst in (As, Bs, Cs)
         ]
-        with _timed("gemm", flop, lambda: lib.fs_gemm_last_dispatches()):
+        with _timed("gemm", flop, _gemm_dispatches):
             check(lib.fs_grouped_gemm_f64_multi(n, arr, *bases, float(alpha), float(beta), _stream()))
         return
     fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
-    with _timed("gemm_split" if split else "gemm", flop, lambda: lib.fs_gemm_last_dispatches()):
+    with _timed("gemm_split" if split else "gemm", flop, _gemm_dispatches):
         check(fn(n, arr, _ptr(A), _ptr(B), _ptr(C), float(alpha), float(beta), _stream()))
 
 
@@ -597,8 +624,10 @@ def batched_svd(descs, A, U, S, VT, work, *, max_sweeps=60):
         return
     tab, arr = _desc_table(_lib.SvdDesc, descs)
     # work model: one Jacobi sweep ~ 3 q^2 p flop (dot products + rotations)
-    mn = np.minimum(tab["m"], tab["n"]).astype(np.float64)
-    flop = float((3.0 * mn * mn * np.maximum(tab["m"], tab["n"])).sum())
+    def flop():
+        mn = np.minimum(tab["m"], tab["n"]).astype(np.float64)
+        return float((3.0 * mn * mn * np.maximum(tab["m"], tab["n"])).sum())
+
     with _timed("svd", flop):
         check(
             lib.fs_batched_svd_f64(
@@ -642,8 +671,7 @@ def batched_cholinv(descs, G, R, Rinv, status):
     if n == 0:
         return
     tab, arr = _desc_table(_lib.CholInvDesc, descs)
-    flop = float((2.0 * tab["l"].astype(np.float64) ** 3 / 3.0).sum())
-    with _timed("cholinv", flop):
+    with _timed("cholinv", lambda: float((2.0 * tab["l"].astype(np.float64) ** 3 / 3.0).sum())):
         check(lib.fs_batched_cholinv_f64(
             n, arr, _ptr(G), None if R is None else _ptr(R), None if Rinv is None else _ptr(Rinv),
             _ptr(status), _stream()))
@@ -656,8 +684,7 @@ def batched_triinv(descs, R, X, status=None):
     if n == 0:
         return
     tab, arr = _desc_table(_lib.TriInvDesc, descs)
-    flop = float((tab["l"].astype(np.float64) ** 3 / 3.0).sum())
-    with _timed("triinv", flop):
+    with _timed("triinv", lambda: float((tab["l"].astype(np.float64) ** 3 / 3.0).sum())):
         check(lib.fs_batched_triinv_f64(n, arr, _ptr(R), _ptr(X), _ptr(status), _stream()))
 
 
@@ -674,9 +701,11 @@ def cholqr(descs, Y, Q, Rt, status, *, passes=3, shifted=True):
     size = ctypes.c_int64()
     check(lib.fs_cholqr_work_size(n, arr, int(passes), int(Rt is not None), ctypes.byref(size)))
     work = empty(int(size.value))
-    m = tab["m"].astype(np.float64)
-    l = tab["l"].astype(np.float64)
-    flop = float(passes * (4.0 * m * l * l + 2.0 * l**3 / 3.0).sum())
+    def flop():
+        m = tab["m"].astype(np.float64)
+        l = tab["l"].astype(np.float64)
+        return float(passes * (4.0 * m * l * l + 2.0 * l**3 / 3.0).sum())
+
     with _timed("cholqr", flop):
         check(lib.fs_cholqr_f64(
             n, arr, _ptr(Y), _ptr(Q), None if Rt is None else _ptr(Rt), _ptr(work), int(size.value),
@@ -736,9 +765,11 @@ def copy2d(descs, src, dst, *, row_w=None, col_w=None, col_idx=None):
     if n == 0:
         return
     tab, arr = _desc_table(_lib.Copy2dDesc, descs)
-    names = tab.dtype.names
-    per = np.where(tab[names[12]] == 0.0, 16.0, 24.0)
-    nbytes = float((per * tab[names[0]] * tab[names[1]]).sum())
+    def nbytes():
+        names = tab.dtype.names
+        per = np.where(tab[names[12]] == 0.0, 16.0, 24.0)
+        return float((per * tab[names[0]] * tab[names[1]]).sum())
+
     with _timed("copy2d", nbytes):
         check(
             lib.fs_batched_copy2d_f64(
