@@ -17,6 +17,8 @@ batched one-sided Jacobi. Host code only does bookkeeping on small integer
 metadata and the truncation decision.
 """
 
+from collections import OrderedDict
+
 import numpy as np
 import torch
 
@@ -395,7 +397,30 @@ class SymmetricTensor:
         assert type(self) is type(other)
         assert self._shape[self._nrr :] == other._shape[: other._nrr]
         assert (self._signature[self._nrr :] ^ other._signature[: other._nrr]).all()
-        # blocks present on both sides (block irreps are sorted), in irrep order
+        key = (
+            type(self), self._fortran, other._fortran,
+            self._block_irreps.tobytes(), self._bshapes.tobytes(), self._boffs.tobytes(),
+            other._block_irreps.tobytes(), other._bshapes.tobytes(), other._boffs.tobytes(),
+        )
+        layout = _MATMUL_LAYOUTS.get(key)
+        if layout is None:
+            layout = self._matmul_layout(other)
+            _MATMUL_LAYOUTS[key] = layout
+            if len(_MATMUL_LAYOUTS) > _MATMUL_LAYOUTS_MAX:
+                _MATMUL_LAYOUTS.popitem(last=False)
+        groups, shapes, offs, total, irreps = layout
+        out = _ops.empty(total)
+        _ops.grouped_gemm(groups, self._data, other._data, out, split=_precision_split())
+        signature = np.empty((self._nrr + other._ndim - other._nrr,), dtype=bool)
+        signature[: self._nrr] = self._signature[: self._nrr]
+        signature[self._nrr :] = other._signature[other._nrr :]
+        shape = self._shape[: self._nrr] + other._shape[other._nrr :]
+        return type(self)._build(self._row_reps, other._col_reps, out, shapes, offs, irreps, signature, shape=shape)
+
+    def _matmul_layout(self, other):
+        """Blocks present on both sides (block irreps are sorted), in irrep order:
+        the grouped-GEMM groups, output block shapes, offsets, size and irreps.
+        Depends only on the two block structures, so __matmul__ caches it."""
         ia, ib = self._block_irreps, other._block_irreps
         pos = np.minimum(np.searchsorted(ia, ib), max(ia.size - 1, 0))
         match = (ia[pos] == ib) if ia.size else np.zeros(ib.shape, dtype=bool)
@@ -405,19 +430,13 @@ class SymmetricTensor:
         shapes = np.stack([m, n], axis=1)
         irreps = ia[i_idx]
         offs, total = _packed_offsets(shapes)
-        out = _ops.empty(total)
         kb = other._bshapes[j_idx, 0]
         assert (kb == kk).all()
         one = np.ones_like(m)
         ars, acs = (one, m) if self._fortran else (kk, one)
         brs, bcs = (one, kb) if other._fortran else (n, one)
         groups = np.stack([m, n, kk, self._boffs[i_idx], ars, acs, other._boffs[j_idx], brs, bcs, offs, n], axis=1)
-        _ops.grouped_gemm(groups, self._data, other._data, out, split=_precision_split())
-        signature = np.empty((self._nrr + other._ndim - other._nrr,), dtype=bool)
-        signature[: self._nrr] = self._signature[: self._nrr]
-        signature[self._nrr :] = other._signature[other._nrr :]
-        shape = self._shape[: self._nrr] + other._shape[other._nrr :]
-        return type(self)._build(self._row_reps, other._col_reps, out, shapes, offs, irreps, signature, shape=shape)
+        return groups, shapes, offs, total, irreps
 
     # misc ------------------------------------------------------------------
     def match_representations(self, other):
@@ -1139,6 +1158,10 @@ class SymmetricTensor:
 
 # Precision form of the block GEMMs ("fp64" or "split"); process-wide switch.
 _PRECISION = {"gemm": "fp64"}
+
+# grouped-GEMM layouts of __matmul__ by the operands' block structures (LRU)
+_MATMUL_LAYOUTS = OrderedDict()
+_MATMUL_LAYOUTS_MAX = 4096
 
 
 def set_gemm_precision(mode):
