@@ -323,6 +323,10 @@ constexpr int kC4P = kCholInvMaxOrder / kC4Rows;  // 8 rows per thread
 constexpr int kC4Q = kCholInvMaxOrder / kC4Cols;  // 4 columns per thread
 constexpr int kC4Threads = kC4Rows * kC4Cols;     // 512
 
+// P x Q: the part of the 8 x 4 register tile that holds rows and columns
+// below the padded order (16 P and 32 Q >= l rounded up to 4; the launch picks
+// them from the batch's largest l); the rest would only carry identity padding.
+template <int P, int Q>
 __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
   const CholInvDev D = bt.d[blockIdx.x];
   const int l = D.l;
@@ -349,11 +353,11 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
   for (int w = 0; w < kC4Threads / 64; ++w) trace += red[w];
   const double shift = D.shift_rel * trace;
 
-  double a[kC4P][kC4Q];
+  double a[P][Q];
 #pragma unroll
-  for (int p = 0; p < kC4P; ++p) {
+  for (int p = 0; p < P; ++p) {
 #pragma unroll
-    for (int q = 0; q < kC4Q; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
       double v = (i == c) ? 1.0 : 0.0;  // identity padding beyond l
       if (i < l && c < l) v = (i <= c) ? g_at(i, c) + (i == c ? shift : 0.0) : 0.0;
@@ -367,10 +371,10 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     const int jp = j0 / kC4Rows, jt = j0 % kC4Rows;  // rows j0 .. j0+3: ty = jt .. jt+3, same p
     if (ty >= jt && ty < jt + kC4Blk) {
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         double v = a[0][q];
 #pragma unroll
-        for (int p = 1; p < kC4P; ++p) v = (p == jp) ? a[p][q] : v;
+        for (int p = 1; p < P; ++p) v = (p == jp) ? a[p][q] : v;
         rowbuf[buf][ty - jt][tx + kC4Cols * q] = v;
       }
     }
@@ -398,11 +402,11 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     }
     if (fail) break;
     // panel rows at own columns (Rc) and own rows (Ri): forward substitution
-    double Rc[kC4Blk][kC4Q], Ri[kC4Blk][kC4P];
+    double Rc[kC4Blk][Q], Ri[kC4Blk][P];
 #pragma unroll
     for (int r = 0; r < kC4Blk; ++r) {
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int c = tx + kC4Cols * q;
         double v = rowbuf[buf][r][c];
 #pragma unroll
@@ -410,7 +414,7 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
         Rc[r][q] = (c >= j0 + kC4Blk) ? v * uinv[r] : 0.0;
       }
 #pragma unroll
-      for (int p = 0; p < kC4P; ++p) {
+      for (int p = 0; p < P; ++p) {
         const int i = ty + kC4Rows * p;
         double v = rowbuf[buf][r][i];
 #pragma unroll
@@ -420,9 +424,9 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     }
     // rank-4 update (rows at or above the block get a zero factor)
 #pragma unroll
-    for (int p = 0; p < kC4P; ++p) {
+    for (int p = 0; p < P; ++p) {
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         double upd = a[p][q];
 #pragma unroll
         for (int r = 0; r < kC4Blk; ++r) upd = fma(-Ri[r][p], Rc[r][q], upd);
@@ -433,7 +437,7 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     // the panel row right of it, zero left of the diagonal
     if (ty >= jt && ty < jt + kC4Blk) {
       const int r = ty - jt;
-      double urow[kC4Blk], prow[kC4Q];
+      double urow[kC4Blk], prow[Q];
 #pragma unroll
       for (int t = 0; t < kC4Blk; ++t) {
         double v = U[0][t];
@@ -442,20 +446,20 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
         urow[t] = (t >= r) ? v : 0.0;
       }
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         double v = Rc[0][q];
 #pragma unroll
         for (int k = 1; k < kC4Blk; ++k) v = (k == r) ? Rc[k][q] : v;
         prow[q] = v;
       }
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int c = tx + kC4Cols * q;
         double fin = (c >= j0 + kC4Blk) ? prow[q] : 0.0;
 #pragma unroll
         for (int t = 0; t < kC4Blk; ++t) fin = (c == j0 + t) ? urow[t] : fin;
 #pragma unroll
-        for (int p = 0; p < kC4P; ++p)
+        for (int p = 0; p < P; ++p)
           if (p == jp) a[p][q] = fin;
       }
     }
@@ -473,9 +477,9 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
   if (bt.R) {
     double *R = bt.R + D.r_off;
 #pragma unroll
-    for (int p = 0; p < kC4P; ++p) {
+    for (int p = 0; p < P; ++p) {
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
         if (i < l && c < l) R[static_cast<int64_t>(i) * l + c] = (c >= i) ? a[p][q] : 0.0;
       }
@@ -486,11 +490,11 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     return;
   }
   __syncthreads();  // udiag complete, row buffers free
-  double x[kC4P][kC4Q];
+  double x[P][Q];
 #pragma unroll
-  for (int p = 0; p < kC4P; ++p) {
+  for (int p = 0; p < P; ++p) {
 #pragma unroll
-    for (int q = 0; q < kC4Q; ++q) x[p][q] = (ty + kC4Rows * p == tx + kC4Cols * q) ? 1.0 : 0.0;
+    for (int q = 0; q < Q; ++q) x[p][q] = (ty + kC4Rows * p == tx + kC4Cols * q) ? 1.0 : 0.0;
   }
   for (int i0 = lpad - kC4Blk; i0 >= 0; i0 -= kC4Blk) {
     const int buf = (i0 / kC4Blk) & 1;
@@ -498,19 +502,19 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     const int cq = i0 / kC4Cols, ct = i0 % kC4Cols;  // columns i0 .. i0+3: tx = ct .. ct+3, same q
     if (ty >= it && ty < it + kC4Blk) {  // current rows i0 .. i0+3 of B
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         double v = x[0][q];
 #pragma unroll
-        for (int p = 1; p < kC4P; ++p) v = (p == ip) ? x[p][q] : v;
+        for (int p = 1; p < P; ++p) v = (p == ip) ? x[p][q] : v;
         rowbuf[buf][ty - it][tx + kC4Cols * q] = v;
       }
     }
     if (tx >= ct && tx < ct + kC4Blk) {  // columns i0 .. i0+3 of R at rows k < i0
 #pragma unroll
-      for (int p = 0; p < kC4P; ++p) {
+      for (int p = 0; p < P; ++p) {
         double v = a[p][0];
 #pragma unroll
-        for (int q = 1; q < kC4Q; ++q) v = (q == cq) ? a[p][q] : v;
+        for (int q = 1; q < Q; ++q) v = (q == cq) ? a[p][q] : v;
         const int k = ty + kC4Rows * p;
         colbuf[buf][tx - ct][k] = (k < i0) ? v : 0.0;
       }
@@ -521,11 +525,11 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
     for (int r = 0; r < kC4Blk; ++r)
 #pragma unroll
       for (int t = 0; t < kC4Blk; ++t) U[r][t] = udiag[i0 / kC4Blk][r * kC4Blk + t];
-    double Xc[kC4Blk][kC4Q];
+    double Xc[kC4Blk][Q];
 #pragma unroll
     for (int r = kC4Blk - 1; r >= 0; --r) {
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         const int c = tx + kC4Cols * q;
         double v = rowbuf[buf][r][c];
 #pragma unroll
@@ -534,38 +538,53 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
       }
     }
 #pragma unroll
-    for (int p = 0; p < kC4P; ++p) {
+    for (int p = 0; p < P; ++p) {
       const int k = ty + kC4Rows * p;
 #pragma unroll
       for (int r = 0; r < kC4Blk; ++r) {
         const double ck = colbuf[buf][r][k];
 #pragma unroll
-        for (int q = 0; q < kC4Q; ++q) x[p][q] = fma(-ck, Xc[r][q], x[p][q]);
+        for (int q = 0; q < Q; ++q) x[p][q] = fma(-ck, Xc[r][q], x[p][q]);
       }
     }
     if (ty >= it && ty < it + kC4Blk) {  // the block's rows of X are final
       const int r = ty - it;
 #pragma unroll
-      for (int q = 0; q < kC4Q; ++q) {
+      for (int q = 0; q < Q; ++q) {
         double v = Xc[0][q];
 #pragma unroll
         for (int k = 1; k < kC4Blk; ++k) v = (k == r) ? Xc[k][q] : v;
 #pragma unroll
-        for (int p = 0; p < kC4P; ++p)
+        for (int p = 0; p < P; ++p)
           if (p == ip) x[p][q] = v;
       }
     }
   }
   double *X = bt.Rinv + D.rinv_off;
 #pragma unroll
-  for (int p = 0; p < kC4P; ++p) {
+  for (int p = 0; p < P; ++p) {
 #pragma unroll
-    for (int q = 0; q < kC4Q; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
       if (i < l && c < l) X[static_cast<int64_t>(i) * l + c] = x[p][q];
     }
   }
   if (threadIdx.x == 0) bt.status[blockIdx.x] = 0;
+}
+
+// Instance whose register tile covers `rows16` blocks of 16 rows (and
+// ceil(rows16 / 2) blocks of 32 columns).
+const void *cholinv4_for(int rows16) {
+  switch (rows16) {
+    case 1: return reinterpret_cast<const void *>(cholinv4_kernel<1, 1>);
+    case 2: return reinterpret_cast<const void *>(cholinv4_kernel<2, 1>);
+    case 3: return reinterpret_cast<const void *>(cholinv4_kernel<3, 2>);
+    case 4: return reinterpret_cast<const void *>(cholinv4_kernel<4, 2>);
+    case 5: return reinterpret_cast<const void *>(cholinv4_kernel<5, 3>);
+    case 6: return reinterpret_cast<const void *>(cholinv4_kernel<6, 3>);
+    case 7: return reinterpret_cast<const void *>(cholinv4_kernel<7, 4>);
+    default: return reinterpret_cast<const void *>(cholinv4_kernel<kC4P, kC4Q>);
+  }
 }
 
 // Inverse X = R^-1 of one upper triangular l x l matrix (l <= 128) per
@@ -758,7 +777,12 @@ int fs_batched_cholinv_f64(int nmat, const fs_cholinv_desc *descs, const double 
                            fs::as_stream(stream), bt);
         FS_LAUNCH_CHECK("cholinv_kernel");
       } else {
-        hipLaunchKernelGGL(cholinv4_kernel, dim3(nb), dim3(kC4Threads), 0, fs::as_stream(stream), bt);
+        int lmax = 1;
+        for (int b = 0; b < nb; ++b) lmax = std::max<int>(lmax, bt.d[b].l);
+        const int rows16 = (((lmax + kC4Blk - 1) / kC4Blk * kC4Blk) + kC4Rows - 1) / kC4Rows;  // P
+        void *args[] = {&bt};
+        FS_HIP_CHECK(hipLaunchKernel(cholinv4_for(rows16), dim3(nb), dim3(kC4Threads), args, 0,
+                                     fs::as_stream(stream)));
         FS_LAUNCH_CHECK("cholinv4_kernel");
       }
     }
