@@ -617,6 +617,10 @@ struct TriInvBatch {
   TriInvDev d[kMaxMat];
 };
 
+// P x Q: the part of the 8 x 4 register tile below the order padded to 16
+// (P = number of 16-row blocks, Q = ceil(P / 2); the launch picks them from
+// the batch's largest l).
+template <int P, int Q>
 __global__ void __launch_bounds__(kTiThreads) triinv_kernel(TriInvBatch bt) {
   const TriInvDev D = bt.d[blockIdx.x];
   const int l = D.l;
@@ -659,21 +663,21 @@ __global__ void __launch_bounds__(kTiThreads) triinv_kernel(TriInvBatch bt) {
 #pragma unroll
     for (int r = 0; r < kTiBlk; ++r) diag[di][r][dc] = dcol[r];
   }
-  double x[kTiP][kTiQ];
+  double x[P][Q];
 #pragma unroll
-  for (int p = 0; p < kTiP; ++p)
+  for (int p = 0; p < P; ++p)
 #pragma unroll
-    for (int q = 0; q < kTiQ; ++q) x[p][q] = (ty + kTiRows * p == tx + kTiCols * q) ? 1.0 : 0.0;
+    for (int q = 0; q < Q; ++q) x[p][q] = (ty + kTiRows * p == tx + kTiCols * q) ? 1.0 : 0.0;
 
   for (int I = nb - 1; I >= 0; --I) {
     const int buf = I & 1;
     // publish block row I of B (every thread owns one of its rows) and the
     // column block R_:I above the diagonal block
 #pragma unroll
-    for (int q = 0; q < kTiQ; ++q) {
+    for (int q = 0; q < Q; ++q) {
       double v = x[0][q];
 #pragma unroll
-      for (int p = 1; p < kTiP; ++p) v = (p == I) ? x[p][q] : v;
+      for (int p = 1; p < P; ++p) v = (p == I) ? x[p][q] : v;
       brow[buf][ty][tx + kTiCols * q] = v;
     }
     for (int e = threadIdx.x; e < I * kTiBlk * kTiBlk; e += kTiThreads) {
@@ -682,43 +686,56 @@ __global__ void __launch_bounds__(kTiThreads) triinv_kernel(TriInvBatch bt) {
     }
     __syncthreads();
     // X_I = D_I B_I at own row ty of the block and own columns
-    double xi[kTiQ];
+    double xi[Q];
 #pragma unroll
-    for (int q = 0; q < kTiQ; ++q) xi[q] = 0.0;
+    for (int q = 0; q < Q; ++q) xi[q] = 0.0;
     for (int t = ty; t < kTiBlk; ++t) {
       const double dv = diag[I][ty][t];
 #pragma unroll
-      for (int q = 0; q < kTiQ; ++q) xi[q] = fma(dv, brow[buf][t][tx + kTiCols * q], xi[q]);
+      for (int q = 0; q < Q; ++q) xi[q] = fma(dv, brow[buf][t][tx + kTiCols * q], xi[q]);
     }
 #pragma unroll
-    for (int q = 0; q < kTiQ; ++q) {
+    for (int q = 0; q < Q; ++q) {
       xrow[buf][ty][tx + kTiCols * q] = xi[q];
 #pragma unroll
-      for (int p = 0; p < kTiP; ++p)
+      for (int p = 0; p < P; ++p)
         if (p == I) x[p][q] = xi[q];
     }
     __syncthreads();
     // rows of the blocks above: B_K -= R_KI X_I
 #pragma unroll
-    for (int p = 0; p < kTiP; ++p) {
+    for (int p = 0; p < P; ++p) {
       if (p < I) {
         const int k = ty + kTiRows * p;
 #pragma unroll 4
         for (int t = 0; t < kTiBlk; ++t) {
           const double rk = rcol[buf][k][t];
 #pragma unroll
-          for (int q = 0; q < kTiQ; ++q) x[p][q] = fma(-rk, xrow[buf][t][tx + kTiCols * q], x[p][q]);
+          for (int q = 0; q < Q; ++q) x[p][q] = fma(-rk, xrow[buf][t][tx + kTiCols * q], x[p][q]);
         }
       }
     }
   }
 #pragma unroll
-  for (int p = 0; p < kTiP; ++p) {
+  for (int p = 0; p < P; ++p) {
 #pragma unroll
-    for (int q = 0; q < kTiQ; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int i = ty + kTiRows * p, c = tx + kTiCols * q;
       if (i < l && c < l) X[static_cast<int64_t>(i) * l + c] = x[p][q];
     }
+  }
+}
+
+const void *triinv_for(int rows16) {
+  switch (rows16) {
+    case 1: return reinterpret_cast<const void *>(triinv_kernel<1, 1>);
+    case 2: return reinterpret_cast<const void *>(triinv_kernel<2, 1>);
+    case 3: return reinterpret_cast<const void *>(triinv_kernel<3, 2>);
+    case 4: return reinterpret_cast<const void *>(triinv_kernel<4, 2>);
+    case 5: return reinterpret_cast<const void *>(triinv_kernel<5, 3>);
+    case 6: return reinterpret_cast<const void *>(triinv_kernel<6, 3>);
+    case 7: return reinterpret_cast<const void *>(triinv_kernel<7, 4>);
+    default: return reinterpret_cast<const void *>(triinv_kernel<kTiP, kTiQ>);
   }
 }
 
@@ -886,7 +903,11 @@ int fs_batched_triinv_f64(int nmat, const fs_triinv_desc *descs, const double *R
         d.x_off = descs[i0].x_off;
       }
       bt.nmat = nb;
-      hipLaunchKernelGGL(triinv_kernel, dim3(nb), dim3(kTiThreads), 0, fs::as_stream(stream), bt);
+      int lmax = 1;
+      for (int b = 0; b < nb; ++b) lmax = std::max<int>(lmax, bt.d[b].l);
+      void *args[] = {&bt};
+      FS_HIP_CHECK(hipLaunchKernel(triinv_for((lmax + kTiBlk - 1) / kTiBlk), dim3(nb), dim3(kTiThreads), args,
+                                   0, fs::as_stream(stream)));
       FS_LAUNCH_CHECK("triinv_kernel");
     }
     return FS_OK;

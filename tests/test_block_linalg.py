@@ -239,6 +239,46 @@ def test_triangular_inverse_blocked(backend):
     assert np.abs(x1 - x2).max() < 1e-12 * np.abs(x1).max()
 
 
+@pytest.mark.parametrize(
+    "orders",
+    [[16, 1, 3], [31, 5, 17], [40, 2, 33], [64, 9, 49], [80, 65, 17], [96, 1, 90], [112, 50, 97], [128, 7, 100]],
+)
+def test_cholesky_and_inverse_register_tiles(backend, orders):
+    """The blocked Cholesky and triangular inverse size their register tiles to
+    the batch's largest order: one batch per tile size, mixed orders in it, the
+    factor (factor-only and fused forms) and the inverse against scipy."""
+    import torch
+
+    rng = np.random.default_rng(sum(orders))
+    n = len(orders)
+    sq = np.array([l * l for l in orders])
+    offs = np.concatenate([[0], np.cumsum(sq)[:-1]])
+    Gs, Rref = [], []
+    for l in orders:
+        y = rng.standard_normal((l + 30, l))
+        g = y.T @ y
+        Gs.append(g.reshape(-1))
+        Rref.append(lg.cholesky(g, lower=False))
+    G = _ops.upload(np.concatenate(Gs))
+    desc = {"l": np.array(orders), "g_off": offs, "g_stride": sq, "n_part": 1, "r_off": offs,
+            "rinv_off": offs, "shift_rel": 0.0}
+    R1, R2, X1, X2 = (_ops.empty(int(sq.sum())) for _ in range(4))
+    st1, st2 = (_ops.zeros(n, dtype=torch.int32) for _ in range(2))
+    _ops.batched_cholinv(desc, G, R1, None, st1)  # factor only
+    _ops.batched_cholinv(desc, G, R2, X1, st2)  # fused factor + inverse
+    _ops.batched_triinv({"l": np.array(orders), "r_off": offs, "x_off": offs}, R1, X2, st1)
+    assert (_ops.download(st1) == 0).all() and (_ops.download(st2) == 0).all()
+    r1, r2, x1, x2 = (_ops.download(t) for t in (R1, R2, X1, X2))
+    for i, l in enumerate(orders):
+        sl = slice(offs[i], offs[i] + l * l)
+        ref = Rref[i]
+        inv = lg.solve_triangular(ref, np.eye(l), lower=False)
+        for got in (r1[sl], r2[sl]):
+            assert np.abs(got.reshape(l, l) - ref).max() < 1e-12 * np.abs(ref).max(), (orders, l)
+        for got in (x1[sl], x2[sl]):
+            assert np.abs(got.reshape(l, l) - inv).max() < 1e-11 * np.abs(inv).max(), (orders, l)
+
+
 def test_factor_only_cholesky(backend):
     """Factor-only batched Cholesky (the four-columns-per-barrier kernel without
     its inverse phase) against scipy: orders across block boundaries, Gram matrices
