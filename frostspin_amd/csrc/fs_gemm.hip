@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "fs_hip.h"
@@ -484,28 +485,46 @@ inline int pick_splits(int64_t tiles, int64_t kmax) {
   return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({by_cus, by_k, kMaxSplits})));
 }
 
-// Split-K workspace per (host thread, device, stream), grow-only: launches
-// that may run concurrently (other threads, or other streams of one thread)
-// never share one.
-double *splitk_workspace(int64_t n_doubles, hipStream_t stream) {
-  struct Ws {
-    int device = -1;
-    hipStream_t stream = nullptr;
-    double *ptr = nullptr;
-    int64_t size = 0;
-  };
-  static thread_local std::vector<Ws> ws;
+// Split-K workspaces: one per (device, stream), grow-only, process-wide, so
+// host threads that come and go (serving slots) reuse them instead of each
+// leaving one behind. A launch holds the pool's lock from the workspace lookup
+// until both of its kernels (partial tiles, ordered reduction) are enqueued:
+// threads sharing a stream then enqueue whole split-K pairs one after the
+// other, and launches on other streams use other workspaces. The pool lives
+// until the process ends (at most one buffer per stream that ever ran a
+// split-K launch; torch hands out streams from a fixed pool).
+struct SplitkWs {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  double *ptr = nullptr;
+  int64_t size = 0;
+};
+
+struct SplitkPool {
+  std::mutex mu;
+  std::vector<SplitkWs> ws;
+};
+
+SplitkPool &splitk_pool() {
+  static SplitkPool *pool = new SplitkPool;  // never destroyed: no HIP calls at exit
+  return *pool;
+}
+
+// Caller holds splitk_pool().mu.
+double *splitk_workspace_locked(int64_t n_doubles, hipStream_t stream) {
+  SplitkPool &pool = splitk_pool();
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
-  Ws *found = nullptr;
-  for (Ws &x : ws)
+  SplitkWs *found = nullptr;
+  for (SplitkWs &x : pool.ws)
     if (x.device == dev && x.stream == stream) found = &x;
   if (!found) {
-    ws.push_back(Ws{dev, stream, nullptr, 0});
-    found = &ws.back();
+    pool.ws.push_back(SplitkWs{dev, stream, nullptr, 0});
+    found = &pool.ws.back();
   }
-  Ws &w = *found;
+  SplitkWs &w = *found;
   if (w.size < n_doubles) {
+    // hipFree waits for the device, so work still reading the old buffer is done
     if (w.ptr) (void)hipFree(w.ptr);
     w.ptr = nullptr;
     w.size = 0;
@@ -605,6 +624,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     std::vector<fs_gemm_group> sel;
     for (size_t t = c0; t < c1; ++t) sel.push_back(groups[tiled[t]]);
     const int tsz = SPLIT ? 64 : pick_tile(static_cast<int>(sel.size()), sel.data());
+    std::unique_lock<std::mutex> ws_lock;  // held from workspace lookup to the reduction launch
     Batch bt;
     std::memset(&bt, 0, sizeof(bt));
     bt.alpha = alpha;
@@ -640,7 +660,8 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
           elems += static_cast<int64_t>(d.m) * d.n;
           bt.elem_prefix[i + 1] = elems;
         }
-        bt.ws = splitk_workspace(elems * splits, fs::as_stream(stream));
+        ws_lock = std::unique_lock<std::mutex>(splitk_pool().mu);
+        bt.ws = splitk_workspace_locked(elems * splits, fs::as_stream(stream));
         if (bt.ws == nullptr) {
           fs::set_error("grouped gemm: split-K workspace allocation failed");
           return FS_ERR_HIP;
@@ -670,6 +691,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
       FS_LAUNCH_CHECK("gemm_splitk_reduce");
       ++dispatches;
     }
+    if (ws_lock.owns_lock()) ws_lock.unlock();
   }
   last_dispatches = dispatches;
   return FS_OK;

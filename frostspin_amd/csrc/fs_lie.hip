@@ -15,6 +15,7 @@
 // lie_group_symmetric_tensor.py:96-265 (transpose-reshape + small GEMM +
 // slice) and the per-elementary-block projections of lie:856-1109.
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -350,6 +351,10 @@ struct fs_lie_weights {
   mutable fs::DeviceTables tables;  // note_stream() from the plans' launches
   int64_t nw = 0;                // length of the unpadded W it was built from
   std::vector<int64_t> off;      // per pattern task: offset of its padded copy
+  // per pattern task: (w_off, n_in, n_out_pad) the copy was packed with; a
+  // plan may use these weights only if its tasks agree on all three
+  std::vector<std::array<int64_t, 3>> layout;
+  uint64_t fingerprint = 0;  // pattern_fingerprint of the pattern it was built from
 };
 
 struct fs_lie_plan {
@@ -442,6 +447,40 @@ void repack_w(int64_t ntasks, Get get, const double *w, std::vector<double> &wde
   }
 }
 
+// FNV-1a over the pattern's shape and per-task arrays (legs, permutation,
+// irreps counts, task irreps indices and row counts): weights built from one
+// pattern are refused for a plan of another.
+bool pattern_arrays_present(const fs_lie_pattern &p) {
+  if (p.ndim < 1 || p.ntasks < 0) return false;
+  if (!p.axes || !p.nirr) return false;
+  if (p.ntasks == 0) return true;
+  for (const int64_t *a : {p.sig_idx, p.ir_in, p.ic_in, p.ir_out, p.ic_out, p.n_in, p.n_out, p.w_off,
+                           p.n_out_pad})
+    if (!a) return false;
+  return true;
+}
+
+// Caller checks pattern_arrays_present first.
+uint64_t pattern_fingerprint(const fs_lie_pattern &p) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](int64_t v) {
+    for (int b = 0; b < 8; ++b) {
+      h ^= static_cast<uint64_t>(v >> (8 * b)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  };
+  auto mix_arr = [&mix](const int64_t *a, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) mix(a[i]);
+  };
+  for (int64_t v : {p.ndim, p.nrr, p.nrr_out, p.ntasks, p.nt_in, p.nt_out}) mix(v);
+  mix_arr(p.axes, p.ndim);
+  mix_arr(p.nirr, p.ndim);
+  mix_arr(p.sig_idx, p.ntasks * p.ndim);
+  for (const int64_t *a : {p.ir_in, p.ic_in, p.ir_out, p.ic_out, p.n_in, p.n_out, p.w_off, p.n_out_pad})
+    mix_arr(a, p.ntasks);
+  return h;
+}
+
 // Plan from tables; W either uploaded with the plan (shared == nullptr) or
 // taken from a shared weights object built from the same pattern.
 int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const fs_lie_row *rows,
@@ -453,6 +492,13 @@ int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const f
     *plan = nullptr;
     if (shared) {
       FS_CHECK_ARG(static_cast<int64_t>(shared->off.size()) == ntasks, "lie plan: weights of another pattern");
+      FS_CHECK_ARG(ntasks == 0 || tasks, "lie plan: null table");
+      for (int64_t t = 0; t < ntasks; ++t) {
+        const std::array<int64_t, 3> &l = shared->layout[static_cast<size_t>(t)];
+        FS_CHECK_ARG(l[0] == tasks[t].w_off && l[1] == tasks[t].n_in && l[2] == tasks[t].n_out_pad,
+                     "lie plan: weights of another pattern (task %lld layout differs)",
+                     static_cast<long long>(t));
+      }
       nw = shared->nw;
     }
     FS_CHECK_ARG(ntasks >= 0 && nrows >= 0 && ndigits >= 0 && nw >= 0, "lie plan: negative size");
@@ -596,6 +642,7 @@ int fs_lie_weights_create(const fs_lie_pattern *pattern, int64_t nw, const doubl
   return fs::guarded([&]() -> int {
     FS_CHECK_ARG(pattern && weights, "lie weights: null argument");
     *weights = nullptr;
+    FS_CHECK_ARG(pattern_arrays_present(*pattern), "lie weights: pattern array missing");
     const int64_t ns = pattern->ntasks;
     FS_CHECK_ARG(ns >= 0 && nw >= 0 && (nw == 0 || w), "lie weights: bad W");
     for (int64_t t = 0; t < ns; ++t) {
@@ -608,6 +655,10 @@ int fs_lie_weights_create(const fs_lie_pattern *pattern, int64_t nw, const doubl
     repack_w(ns, [&](int64_t t, int64_t &wo, int64_t &ni, int64_t &nop) {
       wo = pattern->w_off[t]; ni = pattern->n_in[t]; nop = pattern->n_out_pad[t];
     }, w, wdev, q->off);
+    q->fingerprint = pattern_fingerprint(*pattern);
+    q->layout.resize(static_cast<size_t>(ns));
+    for (int64_t t = 0; t < ns; ++t)
+      q->layout[static_cast<size_t>(t)] = {pattern->w_off[t], pattern->n_in[t], pattern->n_out_pad[t]};
     FS_HIP_CHECK(hipGetDevice(&q->device));
     q->nw = nw;
     int st = q->tables.upload(wdev.data(), wdev.size() * sizeof(double));
@@ -717,6 +768,9 @@ int fs_lie_plan_build(const fs_lie_pattern *pattern, int64_t ndegen, const int64
     *plan = nullptr;
     const int64_t ns = pattern->ntasks, nd = pattern->ndim;
     FS_CHECK_ARG(ns >= 0 && nd >= 1 && nd <= FS_LIE_MAX_LEGS, "lie plan build: bad pattern");
+    FS_CHECK_ARG(pattern_arrays_present(*pattern), "lie plan build: pattern array missing");
+    FS_CHECK_ARG(pattern_fingerprint(*pattern) == weights->fingerprint,
+                 "lie plan build: weights of another pattern");
     int64_t nrows = 0;
     for (int64_t t = 0; t < ns; ++t) {
       FS_CHECK_ARG(pattern->n_in[t] >= 0 && pattern->n_out[t] >= 0, "lie plan build: negative row count");

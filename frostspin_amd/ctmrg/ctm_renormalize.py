@@ -57,12 +57,15 @@ def construct_projectors_batch(
     rtol=None,
     degen_ratio=1.0,
     rsvd_seed=0,
+    diagnostics=None,
 ):
     """construct_projectors for several independent (corner1, corner2, corner3,
     corner4, last_renormalized) problems at once: the sectors of all problems
     share every launch (operands gathered from the different corners through the
     per-group operand bases of the grouped GEMM), while the truncation is decided
-    per problem exactly as for a single call. Returns a list of (P, Pt)."""
+    per problem exactly as for a single call. Returns a list of (P, Pt).
+    diagnostics, if a dict, receives the sectors (problem, irrep, block_chi),
+    the kept counts and the singular values handed to the cut."""
     sectors = []  # (problem, shared irrep, block_chi, corner block indices)
     for pi, (corner1, corner2, corner3, corner4, last) in enumerate(problems):
         shared = sorted(
@@ -157,19 +160,62 @@ def construct_projectors_batch(
             U_of[i] = (U, j)
             VT_of[i] = (VT, j)
             s_of[i] = s_list[j]
-    for i in range(n):
-        if sparse_ref[i] or not dense[i]:
-            s_of[i] = s_of[i][: block_chi[i]]
+    # the randomized SVD's oversampled tail is not accurate: only the k_need
+    # leading values enter the cut (block_chi on the reference's ARPACK branch)
+    for i in rand_ids:
+        s_of[i] = s_of[i][: k_need[i]]
+
+    # The reference hands find_block_cuts every singular value of a sector on
+    # its dense branch and block_chi values on its ARPACK branch (:96-103); the
+    # default relative cutoff counts them all (svd_tools.py:45-48).
+    ref_len = np.where(sparse_ref, np.minimum(block_chi, mins), mins)
+    by_problem = [[i for i in range(n) if sectors[i][0] == pi] for pi in range(len(problems))]
+
+    def cut_problem(pi):
+        ids = by_problem[pi]
+        if not ids:
+            return
+        dims = [problems[pi][1].irrep_dimension(sectors[i][1]) for i in ids]
+        rt = rtol if rtol is not None else float(ref_len[ids].sum()) * np.finfo(np.float64).eps
+        cuts[ids] = find_block_cuts([s_of[i] for i in ids], chi_target, dims=dims, atol=atol, rtol=rt,
+                                    degen_ratio=degen_ratio)
 
     # multiplet-aware cut, per problem
     cuts = np.zeros((n,), dtype=np.int64)
-    for pi, prob in enumerate(problems):
-        ids = [i for i in range(n) if sectors[i][0] == pi]
-        if not ids:
-            continue
-        dims = [prob[1].irrep_dimension(sectors[i][1]) for i in ids]
-        cuts[ids] = find_block_cuts([s_of[i] for i in ids], chi_target, dims=dims, atol=atol, rtol=rtol,
-                                    degen_ratio=degen_ratio)
+    for pi in range(len(problems)):
+        cut_problem(pi)
+    # A dense-branch sector served by the randomized SVD holds only its leading
+    # values; while the cut takes all of them (a sector growing faster than
+    # block_chi allows), the decision may depend on the values not computed:
+    # such sectors are decomposed again in full (exact Jacobi up to the dense
+    # limit, a randomized SVD of twice the size above) and their problem is cut
+    # again, until every such cut stops inside the computed values.
+    while True:
+        short = [i for i in range(n)
+                 if not sparse_ref[i] and len(s_of[i]) < mins[i] and cuts[i] >= len(s_of[i])]
+        if not short:
+            break
+        exact = [i for i in short if mins[i] <= DENSE_SVD_LIMIT]
+        wider = [i for i in short if mins[i] > DENSE_SVD_LIMIT]
+        if exact:
+            U, S, s_off, VT, finish = bl.svd_async(M.select(exact))
+            s_list = finish()
+            for j, i in enumerate(exact):
+                U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j]
+        if wider:
+            ks = np.array([min(int(mins[i]), 2 * len(s_of[i])) for i in wider], dtype=np.int64)
+            U, S, s_off, VT, finish = bl.rsvd_async(M.select(wider), ks, seed=rsvd_seed)
+            _ops.wait_current()
+            s_list = finish()
+            for j, i in enumerate(wider):
+                U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j][: ks[j]]
+        for pi in sorted({sectors[i][0] for i in short}):
+            cut_problem(pi)
+    if diagnostics is not None:
+        diagnostics.update(
+            sectors=[(sec[0], sec[1], sec[2]) for sec in sectors], cuts=cuts.copy(),
+            values=[np.array(v, copy=True) for v in s_of], sparse_ref=sparse_ref.copy(), mins=mins.copy(),
+        )
     non_empty = cuts.nonzero()[0]
 
     # s^-1/2 of the kept values, one device vector for all sectors

@@ -345,7 +345,7 @@ def _orthonormal_columns(Y, passes=3, statuses=None, robust=False):
     return cholqr(Y, passes=passes, statuses=statuses)
 
 
-def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
+def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False, certify=True):
     """Randomized truncated SVD (range finder + power iterations, Halko et al.,
     SIAM Rev. 53 (2011) Alg. 4.4/5.1) of every A_i, keeping l_i = k_i + p_i
     vectors. Returns U (m x l), S, s_off, s_list, VT (l x n) like svd().
@@ -355,16 +355,29 @@ def rsvd(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
     the singular values, and any matrix whose factorisation broke down is redone
     with Householder QR (robust=True) and spliced into the results."""
     U, S, s_off, VT, finish = rsvd_async(A, ks, oversampling=oversampling, power_iters=power_iters,
-                                         seed=seed, robust=robust)
+                                         seed=seed, robust=robust, certify=certify)
     _ops.wait_current()
     return U, S, s_off, finish(), VT
 
 
-def rsvd_async(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False):
-    """rsvd() with its one host readback (singular values and CholeskyQR
-    statuses) started but not waited for: returns U, S, s_off, VT and a
-    callable giving the host singular values once the current stream has been
-    synchronized (it splices the redone matrices, if any, into U, S and VT)."""
+# A leading singular triplet (s_i, u_i, v_i) of the randomized SVD is accepted
+# when ||A v_i - s_i u_i|| <= RSVD_RESIDUAL_TOL * s_1 (u_i^T A = s_i v_i^T holds
+# by construction, so this residual bounds the error of the pair); the
+# reference's ARPACK runs with tol = 0, i.e. to machine precision.
+RSVD_RESIDUAL_TOL = 1e-10
+
+
+def rsvd_async(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False, certify=True):
+    """rsvd() with its one host readback (singular values, CholeskyQR
+    statuses and residual norms) started but not waited for: returns U, S,
+    s_off, VT and a callable giving the host singular values once the current
+    stream has been synchronized (it splices the redone matrices, if any, into
+    U, S and VT).
+
+    certify: the k_i leading triplets of every matrix are checked against
+    RSVD_RESIDUAL_TOL; a matrix that fails is decomposed again (exact Jacobi
+    SVD up to the dense limit, more power iterations above) before its values
+    are returned."""
     ks = np.asarray(ks, dtype=np.int64)
     mins = np.minimum(A.rows, A.cols)
     p = np.maximum(10, ks // 2) if oversampling is None else np.full_like(ks, oversampling)
@@ -392,16 +405,94 @@ def rsvd_async(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False)
     VT = gemm(Vrt, Q2.T())  # l x n
     s_h = _ops.download_start(S)
     st_h = _ops.download_start(torch.stack(statuses)) if statuses else None
+    kc = np.minimum(ks, ls)
+    res = _residual_norms2(A, U, S, s_off, VT, kc) if certify else None
+    r_h = _ops.download_start(res[0]) if res is not None and res[0].numel() else None
 
     def finish():
         s_list = finish_small(s_h)
+        failed = np.zeros((0,), dtype=np.int64)
         if st_h is not None:
             failed = st_h.numpy().any(axis=0).nonzero()[0]
             if failed.size:
                 _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed)
+        if r_h is not None:
+            r2 = r_h.numpy()
+            bad = [
+                j for j in range(len(A))
+                if kc[j] and j not in failed
+                and np.sqrt(max(r2[res[1][j] : res[1][j] + kc[j]].max(), 0.0)) > RSVD_RESIDUAL_TOL * s_list[j][0]
+            ]
+            if bad:
+                _refine_rsvd(A, ks, ls, np.array(bad, dtype=np.int64), U, S, s_off, s_list, VT, power_iters, seed)
         return s_list
 
     return U, S, s_off, VT, finish
+
+
+def _residual_norms2(A, U, S, s_off, VT, ks):
+    """||A v_i - s_i u_i||^2 for the ks[j] leading triplets of every matrix j,
+    on the device: (packed vector, offset of each matrix's values)."""
+    r_off = off_parts(ks)
+    out = _ops.empty(int(ks.sum()))
+    live = [j for j in range(len(A)) if ks[j] > 0]
+    if not live:
+        return out, r_off
+    Al = A.select(live)
+    VTk = Mats(VT.buf, VT.off[live], ks[live], VT.cols[live], VT.rs[live], VT.cs[live])
+    AV = gemm(Al, VTk.T())  # m x k: A v_i
+    _ops.copy2d(
+        [(A.rows[j], ks[j], U.off[j], U.rs[j], U.cs[j], AV.off[t], AV.rs[t], 1, 0, s_off[j], 0, -1.0, 1.0)
+         for t, j in enumerate(live)],
+        U.buf, AV.buf, col_w=S,
+    )
+    G = gemm(AV.T(), AV)  # k x k: diagonal = squared residual norms
+    _ops.copy2d(
+        [(ks[j], 1, G.off[t], G.cols[t] + 1, 0, r_off[j], 1, 0, 0, 0, 0, 1.0, 0.0) for t, j in enumerate(live)],
+        G.buf, out,
+    )
+    return out, r_off
+
+
+def _refine_rsvd(A, ks, ls, bad, U, S, s_off, s_list, VT, power_iters, seed):
+    """Matrices whose leading triplets missed the residual tolerance: exact
+    Jacobi SVD where min(m, n) allows it, else randomized SVD with more power
+    iterations (checked again); the leading l_i triplets replace theirs."""
+    mins = np.minimum(A.rows, A.cols)
+    exact = bad[mins[bad] <= _ops.SVD_MAX_MIN_DIM]
+    wide = bad[mins[bad] > _ops.SVD_MAX_MIN_DIM]
+    parts = []
+    if exact.size:
+        U2, S2, s_off2, s2_list, VT2 = svd(A.select(exact))
+        parts.append((exact, U2, S2, s_off2, s2_list, VT2))
+    if wide.size:
+        q = power_iters
+        todo = wide
+        while todo.size:
+            q = 3 * q + 2
+            U2, S2, s_off2, s2_list, VT2 = rsvd(A.select(todo), ks[todo], power_iters=q, seed=seed + q,
+                                                certify=False)
+            kc = np.minimum(ks[todo], ls[todo])
+            r, r_off = _residual_norms2(A.select(todo), U2, S2, s_off2, VT2, kc)
+            r2 = _ops.download(r) if r.numel() else np.zeros((0,))
+            ok = np.array([kc[t] == 0 or np.sqrt(max(r2[r_off[t] : r_off[t] + kc[t]].max(), 0.0))
+                           <= RSVD_RESIDUAL_TOL * s2_list[t][0] or q > 200 for t in range(todo.size)])
+            parts.append((todo[ok], U2, S2, s_off2, s2_list, VT2, np.nonzero(ok)[0]))
+            todo = todo[~ok]
+    for part in parts:
+        ids, U2, S2, s_off2, s2_list, VT2 = part[:6]
+        src = part[6] if len(part) > 6 else np.arange(ids.size)
+        if not ids.size:
+            continue
+        l = ls[ids]
+        _ops.copy2d([(U.rows[i], l[t], U2.off[j], U2.rs[j], U2.cs[j], U.off[i], U.rs[i], 1, 0, 0, 0, 1.0, 0.0)
+                     for t, (i, j) in enumerate(zip(ids, src, strict=True))], U2.buf, U.buf)
+        _ops.copy2d([(l[t], VT.cols[i], VT2.off[j], VT2.rs[j], VT2.cs[j], VT.off[i], VT.rs[i], 1, 0, 0, 0, 1.0, 0.0)
+                     for t, (i, j) in enumerate(zip(ids, src, strict=True))], VT2.buf, VT.buf)
+        _ops.copy2d([(l[t], 1, s_off2[j], 1, 0, s_off[i], 1, 0, 0, 0, 0, 1.0, 0.0)
+                     for t, (i, j) in enumerate(zip(ids, src, strict=True))], S2, S)
+        for t, (i, j) in enumerate(zip(ids, src, strict=True)):
+            s_list[i] = np.array(s2_list[j][: l[t]], copy=True)
 
 
 def _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed):

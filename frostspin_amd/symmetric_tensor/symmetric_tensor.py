@@ -17,6 +17,7 @@ batched one-sided Jacobi. Host code only does bookkeeping on small integer
 metadata and the truncation decision.
 """
 
+import threading
 from collections import OrderedDict
 
 import numpy as np
@@ -402,12 +403,16 @@ class SymmetricTensor:
             self._block_irreps.tobytes(), self._bshapes.tobytes(), self._boffs.tobytes(),
             other._block_irreps.tobytes(), other._bshapes.tobytes(), other._boffs.tobytes(),
         )
-        layout = _MATMUL_LAYOUTS.get(key)
+        with _MATMUL_LAYOUTS_LOCK:
+            layout = _MATMUL_LAYOUTS.get(key)
+            if layout is not None:
+                _MATMUL_LAYOUTS.move_to_end(key)
         if layout is None:
             layout = self._matmul_layout(other)
-            _MATMUL_LAYOUTS[key] = layout
-            if len(_MATMUL_LAYOUTS) > _MATMUL_LAYOUTS_MAX:
-                _MATMUL_LAYOUTS.popitem(last=False)
+            with _MATMUL_LAYOUTS_LOCK:
+                _MATMUL_LAYOUTS[key] = layout
+                if len(_MATMUL_LAYOUTS) > _MATMUL_LAYOUTS_MAX:
+                    _MATMUL_LAYOUTS.popitem(last=False)
         groups, shapes, offs, total, irreps = layout
         out = _ops.empty(total)
         _ops.grouped_gemm(groups, self._data, other._data, out, split=_precision_split())
@@ -1159,9 +1164,11 @@ class SymmetricTensor:
 # Precision form of the block GEMMs ("fp64" or "split"); process-wide switch.
 _PRECISION = {"gemm": "fp64"}
 
-# grouped-GEMM layouts of __matmul__ by the operands' block structures (LRU)
+# grouped-GEMM layouts of __matmul__ by the operands' block structures (LRU;
+# serving slots call __matmul__ from several host threads)
 _MATMUL_LAYOUTS = OrderedDict()
 _MATMUL_LAYOUTS_MAX = 4096
+_MATMUL_LAYOUTS_LOCK = threading.Lock()
 
 
 def set_gemm_precision(mode):

@@ -238,3 +238,49 @@ def test_native_plan_tables_match_numpy_restatement(emulated_device):
                 t._present_map(st.in_targets), t._boffs, rs_b, cs_b, nrows=int(st.n_in.sum() + st.n_out.sum()))
             for g, e in zip(got, ref, strict=True):
                 assert g.shape == e.shape and np.array_equal(g, e)
+
+
+@pytest.mark.gpu
+def test_plan_rejects_weights_of_another_pattern(gpu):
+    """fs_lie_plan_build with the shared weights of a different permute pattern
+    that has the same task count and W layout must fail with FS_ERR_INVALID
+    (weights carry a fingerprint of their pattern and the W layout of every
+    task), not apply the other pattern's matrices."""
+    import ctypes
+
+    from frostspin_amd import SU2SymmetricTensor, _lib
+    from frostspin_amd._lib import lib
+
+    irreps = ((1, 3), (2, 4), (1, 3), (2,))
+    sig = (False, True, False, True)
+    # same tensor, two permutes: equal task counts and W layouts, other matrices
+    st1 = SU2SymmetricTensor._permute_structure(irreps, sig, 2, (0, 1, 2, 3), 1)
+    st2 = SU2SymmetricTensor._permute_structure(irreps, sig, 2, (2, 0, 3, 1), 3)
+    assert len(st1.n_in) == len(st2.n_in)
+    t = SU2SymmetricTensor.random(
+        tuple(np.array([[1] * len(ir), list(ir)]) for ir in irreps[:2]),
+        tuple(np.array([[1] * len(ir), list(ir)]) for ir in irreps[2:]),
+        signature=np.array(sig), rng=np.random.default_rng(0),
+    )
+    degen = np.ones((sum(len(ir) for ir in irreps),), dtype=np.int64)
+    in_block = np.ascontiguousarray(t._present_map(st2.in_targets), dtype=np.int64)
+    rs_b, cs_b = (np.ascontiguousarray(x, dtype=np.int64) for x in t._block_strides())
+    boffs = np.ascontiguousarray(t._boffs, dtype=np.int64)
+    shapes = np.empty((len(st2.out_targets), 2), dtype=np.int64)
+    handle = ctypes.c_void_p()
+    status = lib.fs_lie_plan_build(
+        ctypes.byref(st2.pattern), degen.size, _lib.i64p(degen), _lib.i64p(in_block), boffs.size,
+        _lib.i64p(boffs), _lib.i64p(rs_b), _lib.i64p(cs_b), st1.weights.handle(), t._data.numel(),
+        _lib.i64p(shapes), ctypes.byref(handle),
+    )
+    assert status == 1  # FS_ERR_INVALID
+    assert handle.value is None
+    assert b"weights of another pattern" in lib.fs_last_error()
+    # the matching weights build a plan
+    status = lib.fs_lie_plan_build(
+        ctypes.byref(st2.pattern), degen.size, _lib.i64p(degen), _lib.i64p(in_block), boffs.size,
+        _lib.i64p(boffs), _lib.i64p(rs_b), _lib.i64p(cs_b), st2.weights.handle(), t._data.numel(),
+        _lib.i64p(shapes), ctypes.byref(handle),
+    )
+    assert status == 0 and handle.value is not None
+    lib.fs_lie_plan_destroy(handle.value)

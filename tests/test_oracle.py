@@ -147,3 +147,19 @@ def test_ising_oracle_onsager():
     ro = ising.run(beta=0.5, chi=16, n_iter=80, symmetry="trivial")
     assert abs(abs(ro["magnetization"]) - ising.exact_magn(0.5)) < 1e-6
     assert abs(ro["energy"] - ising.exact_energy(0.5)) < 1e-8
+
+
+def test_ising_oracle_correlation_length():
+    """Edge transfer-matrix correlation length against Onsager's exact axis
+    value xi^-1 = -ln tanh(beta) - 2 beta (disordered phase): at beta = 0.3 it
+    approaches the exact 1.5789 from below as chi grows (1.5324 at chi = 8,
+    1.5565 at chi = 16). Near the critical point it grows with chi and stays
+    below the exact value (about 364 at beta = 0.44)."""
+    exact = ising.exact_xi(0.3)
+    xi8 = ising.run(beta=0.3, chi=8, n_iter=60, symmetry="Z2")["xi"]
+    xi16 = ising.run(beta=0.3, chi=16, n_iter=60, symmetry="Z2")["xi"]
+    assert xi8 < xi16 < exact
+    assert abs(xi16 - exact) < 0.02 * exact
+    xi_c16 = ising.run(beta=0.44, chi=16, n_iter=200, symmetry="Z2")["xi"]
+    xi_c32 = ising.run(beta=0.44, chi=32, n_iter=200, symmetry="Z2")["xi"]
+    assert 10.0 < xi_c16 < xi_c32 < ising.exact_xi(0.44)
