@@ -295,6 +295,7 @@ def run_ctmrg(args):
 
     from frostspin_amd import _ops
     from frostspin_amd.ctmrg import models
+    from frostspin_amd.misc_tools import block_linalg as bl
 
     if args.symmetry == "SU2":
         h = models.heisenberg_bond("SU2")
@@ -364,6 +365,7 @@ def run_ctmrg(args):
             "energy_per_bond": energy,
             "simple_update_s": t_su,
             "parallelism": "single GPU",
+            "rsvd_certificates": dict(bl.RSVD_STATS),
         },
         "roofline": {
             "bound": "mfma",

@@ -187,16 +187,18 @@ def construct_projectors_batch(
     # A dense-branch sector served by the randomized SVD holds only its leading
     # values; while the cut takes all of them (a sector growing faster than
     # block_chi allows), the decision may depend on the values not computed:
-    # such sectors are decomposed again in full (exact Jacobi up to the dense
-    # limit, a randomized SVD of twice the size above) and their problem is cut
-    # again, until every such cut stops inside the computed values.
+    # such sectors are decomposed again (exact Jacobi for small sectors, a
+    # randomized SVD of twice as many values above) and their problem is cut
+    # again, until every such cut stops inside the computed values (the values
+    # beyond the cut do not change the decision; the default cutoff already
+    # counts every value of the sector).
     while True:
         short = [i for i in range(n)
                  if not sparse_ref[i] and len(s_of[i]) < mins[i] and cuts[i] >= len(s_of[i])]
         if not short:
             break
-        exact = [i for i in short if mins[i] <= DENSE_SVD_LIMIT]
-        wider = [i for i in short if mins[i] > DENSE_SVD_LIMIT]
+        exact = [i for i in short if mins[i] <= bl.EXACT_REDO_MAX]
+        wider = [i for i in short if mins[i] > bl.EXACT_REDO_MAX]
         if exact:
             U, S, s_off, VT, finish = bl.svd_async(M.select(exact))
             s_list = finish()
