@@ -220,6 +220,10 @@ _SIGNATURES = {
         [c_int, ctypes.POINTER(GemmGroup), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),
          c_dbl, c_dbl, c_vp],
     ),
+    "fs_grouped_gemm_f64_noalloc": (
+        c_int,
+        [c_int, ctypes.POINTER(GemmGroup), c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp],
+    ),
     "fs_gemm_last_dispatches": (c_int, []),
     "fs_grouped_gemm_f64_split": (
         c_int,

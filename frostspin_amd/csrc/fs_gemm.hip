@@ -535,6 +535,51 @@ double *splitk_workspace_locked(int64_t n_doubles, hipStream_t stream) {
   return w.ptr;
 }
 
+// Allocation-free entry point (fs_grouped_gemm_f64_noalloc): the split-K
+// partial tiles live in this file-scope device array instead of a pool
+// buffer. Its size bounds every split-K launch of the default tile rules
+// (at most ~2 * 256 partial 64 x 64 tiles); a launch that would need more is
+// refused. Each device holds its own copy at its own address, looked up once
+// per device with hipGetSymbolAddress; host code uses only that address.
+constexpr int64_t kScratchDoubles = int64_t(1) << 21;  // 16 MiB per device
+static __device__ __align__(256) double fs_gemm_scratch[kScratchDoubles];
+
+struct ScratchTable {
+  std::mutex mu;                       // guards lookup and serializes split-K pairs
+  std::vector<double *> ptr;           // per device: address of fs_gemm_scratch, or null
+  std::vector<unsigned char> looked;   // per device: lookup done
+};
+
+ScratchTable &scratch_table() {
+  static ScratchTable *t = new ScratchTable;  // never destroyed: no HIP calls at exit
+  return *t;
+}
+
+// Caller holds scratch_table().mu. Address of the current device's scratch
+// array (looked up on the first call on that device, with it current).
+int scratch_ptr_locked(double **out) {
+  ScratchTable &t = scratch_table();
+  int dev = 0;
+  FS_HIP_CHECK(hipGetDevice(&dev));
+  FS_CHECK_ARG(dev >= 0, "grouped gemm: no current device");
+  if (static_cast<size_t>(dev) >= t.ptr.size()) {
+    t.ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+    t.looked.resize(static_cast<size_t>(dev) + 1, 0);
+  }
+  if (!t.looked[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(fs_gemm_scratch));
+    if (e != hipSuccess || p == nullptr) {
+      fs::set_error("grouped gemm: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return FS_ERR_HIP;
+    }
+    t.ptr[dev] = static_cast<double *>(p);
+    t.looked[dev] = 1;
+  }
+  *out = t.ptr[dev];
+  return FS_OK;
+}
+
 inline int pick_tile(int ng, const fs_gemm_group *g) {
   static const int forced = [] {
     const char *e = std::getenv("FS_GEMM_TILE");
@@ -587,7 +632,7 @@ int fill_group(GroupDev &d, const fs_gemm_group &q, const double *a_base, const 
 template <bool SPLIT>
 int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const *A,
                    const double *const *B, double *const *C, int base_stride, double alpha,
-                   double beta, void *stream) {
+                   double beta, void *stream, bool noalloc = false) {
   FS_CHECK_ARG(ngroups >= 0, "grouped gemm: negative group count");
   FS_CHECK_ARG(ngroups == 0 || groups, "grouped gemm: null groups");
   int dispatches = 0;
@@ -660,11 +705,21 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
           elems += static_cast<int64_t>(d.m) * d.n;
           bt.elem_prefix[i + 1] = elems;
         }
-        ws_lock = std::unique_lock<std::mutex>(splitk_pool().mu);
-        bt.ws = splitk_workspace_locked(elems * splits, fs::as_stream(stream));
-        if (bt.ws == nullptr) {
-          fs::set_error("grouped gemm: split-K workspace allocation failed");
-          return FS_ERR_HIP;
+        if (noalloc) {
+          FS_CHECK_ARG(elems * splits <= kScratchDoubles,
+                       "grouped gemm (no-alloc): split-K needs %lld doubles, beyond the static "
+                       "scratch of %lld",
+                       static_cast<long long>(elems * splits), static_cast<long long>(kScratchDoubles));
+          ws_lock = std::unique_lock<std::mutex>(scratch_table().mu);
+          const int st = scratch_ptr_locked(&bt.ws);
+          if (st != FS_OK) return st;
+        } else {
+          ws_lock = std::unique_lock<std::mutex>(splitk_pool().mu);
+          bt.ws = splitk_workspace_locked(elems * splits, fs::as_stream(stream));
+          if (bt.ws == nullptr) {
+            fs::set_error("grouped gemm: split-K workspace allocation failed");
+            return FS_ERR_HIP;
+          }
         }
         bt.ws_stride = elems;
       }
@@ -714,6 +769,14 @@ int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups, const do
   return fs::guarded([&]() -> int {
     FS_CHECK_ARG(ngroups == 0 || (A && B && C), "grouped gemm: null base array");
     return launch_grouped<false>(ngroups, groups, A, B, C, 1, alpha, beta, stream);
+  });
+}
+
+int fs_grouped_gemm_f64_noalloc(int ngroups, const fs_gemm_group *groups, const double *A,
+                                const double *B, double *C, double alpha, double beta,
+                                void *stream) {
+  return fs::guarded([&]() -> int {
+    return launch_grouped<false>(ngroups, groups, &A, &B, &C, 0, alpha, beta, stream, true);
   });
 }
 

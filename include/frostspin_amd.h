@@ -324,6 +324,23 @@ int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups,
                               void *stream);
 
 /*
+ * fs_grouped_gemm_f64 without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Same kernels and
+ * results as fs_grouped_gemm_f64; the split-K partial tiles go to a static
+ * per-device scratch array of 2^21 doubles (16 MiB, held by each device the
+ * library runs on until the process ends) instead of a pooled buffer. A call
+ * whose split-K partials would exceed it fails with FS_ERR_INVALID (the
+ * default tile rules need at most ~13 MiB). Every partial is written before
+ * it is read, so nothing needs clearing between calls. Calls that overlap on
+ * one device (other streams, other threads' streams) share the array and
+ * race; serialize them.
+ */
+int fs_grouped_gemm_f64_noalloc(int ngroups, const fs_gemm_group *groups,
+                                const double *A, const double *B, double *C,
+                                double alpha, double beta, void *stream);
+
+/*
  * Kernel dispatches issued by the calling thread's last grouped GEMM call
  * (tile kernel per 32 groups, skinny-row kernel, split-K reduction); for
  * per-dispatch timing. No counterpart in the reference (instrumentation).
