@@ -23,52 +23,57 @@ from .ctm_environment import CTMEnvironment
 from .ctm_renormalize import construct_projectors_batch
 
 
+# CTMRG parameters: default and validity test (abstract_ctmrg.py:17-120); they
+# are saved with the environment as _CTM_<name>
+_PARAMS = {
+    "block_chi_ratio": (1.1, lambda v: v >= 1.0, "block_chi_ratio must be at least 1.0"),
+    "ncv_ratio": (2.0, lambda v: v >= 2.0, "ncv_ratio must be at least 2.0"),
+    "cutoff": (1e-11, lambda v: 0.0 <= v < 1.0, "cutoff must lie in [0, 1)"),
+    "degen_ratio": (0.9999, lambda v: 0.0 < v <= 1.0, "degen_ratio must lie in (0, 1]"),
+}
+# read-only views of the environment
+_ENV_PROPERTIES = ("Dmax", "Dmin", "Lx", "Ly", "cell", "chi_values", "elementary_tensors", "n_sites")
+_ENV_GETTERS = ("get_A", "get_C1", "get_C2", "get_C3", "get_C4", "get_T1", "get_T2", "get_T3", "get_T4",
+                "get_corner_representations")
+
+
 class AbstractCTMRG:
     """
     Corner Transfer Matrix Renormalization Group for bilayer (thermal or pure)
     tensor networks on the square lattice with arbitrary unit cells.
 
-    Leg ordering of site tensors: (physical, ancilla, up, right, down, left);
-    environment tensors named clockwise from the upper left corner:
+    Site tensors have legs (physical, ancilla, up, right, down, left). Around a
+    window of sites the environment reads, clockwise from the upper left corner,
 
-        C1-T1-T1-C2
-        |  |  |   |
-        T4-a--a--T2
-        |  |  |   |
-        C4-T3-T3-C3
+        C1 T1 .. T1 C2
+        T4 A  .. A  T2
+        ..          ..
+        C4 T3 .. T3 C3
+
+    Every rdm, norm and transfer-matrix contraction takes the tensors of such a
+    window in row-major order (_window).
     """
 
     def __init__(self, env, chi_target, *, verbosity=0, **ctm_params):
-        self.verbosity = verbosity
-        if self.verbosity > 0:
-            print(f"initalize CTMRG with verbosity = {self.verbosity}")
         if type(self) is AbstractCTMRG:
-            raise NotImplementedError("Cannot instantiate AbstractCTMRG")
+            raise NotImplementedError("AbstractCTMRG is abstract: use a CTMRG schedule class")
+        self.verbosity = verbosity
         self._env = env
         self._site_coords = env.site_coords
         self.chi_target = int(chi_target)
-        self.block_chi_ratio = float(ctm_params.get("block_chi_ratio", 1.1))
-        self.ncv_ratio = float(ctm_params.get("ncv_ratio", 2.0))
-        self.cutoff = float(ctm_params.get("cutoff", 1e-11))
-        self.degen_ratio = float(ctm_params.get("degen_ratio", 0.9999))
-        if self.verbosity > 0:
-            print(self)
         if self.chi_target < 2:
-            raise ValueError("chi must be larger than 2")
-        if self.block_chi_ratio < 1.0:
-            raise ValueError("block_chi_ratio must be larger than 1.0")
-        if self.ncv_ratio < 2.0:
-            raise ValueError("ncv_ratio must be larger than 2.0")
-        if not (0.0 <= self.cutoff < 1.0):
-            raise ValueError("cutoff must me obey 0.0 <= self.cutoff < 1.0")
-        if not (0.0 < self.degen_ratio <= 1.0):
-            raise ValueError("degen_ratio must obey 0.0 <= self.degen_ratio <= 1.0")
+            raise ValueError("chi_target must be at least 2")
+        for name, (default, valid, msg) in _PARAMS.items():
+            value = float(ctm_params.get(name, default))
+            if not valid(value):
+                raise ValueError(msg)
+            setattr(self, name, value)
+        if self.verbosity > 0:
+            print(f"CTMRG ready (verbosity {self.verbosity})\n{self}")
 
     @classmethod
     def from_elementary_tensors(cls, tiling, tensors, chi_target, *, verbosity=0, **ctm_params):
-        """Start from a dummy environment built from the unit-cell site tensors."""
-        if verbosity > 0:
-            print("Start CTMRG from dummy environment")
+        """Start from the trivial environment of the unit-cell site tensors."""
         env = CTMEnvironment.from_elementary_tensors(tiling, tensors)
         return cls(env, chi_target, verbosity=verbosity, **ctm_params)
 
@@ -77,56 +82,17 @@ class AbstractCTMRG:
 
     @classmethod
     def load_from_file(cls, filename, *, verbosity=0):
-        ctm_params = {}
+        """Restart from an npz written by save_to_file (abstract_ctmrg.py:168-215 keys)."""
         with np.load(filename) as fin:
             chi_target = int(fin["_CTM_chi_target"])
-            for k in ("block_chi_ratio", "ncv_ratio", "cutoff", "degen_ratio"):
-                ctm_params[k] = float(fin[f"_CTM_{k}"])
-        env = CTMEnvironment.load_from_file(filename)
-        return cls(env, chi_target, verbosity=verbosity, **ctm_params)
+            params = {name: float(fin[f"_CTM_{name}"]) for name in _PARAMS}
+        return cls(CTMEnvironment.load_from_file(filename), chi_target, verbosity=verbosity, **params)
 
     def save_to_file(self, filename, **additional_data):
-        data = {
-            "_CTM_chi_target": self.chi_target,
-            "_CTM_block_chi_ratio": self.block_chi_ratio,
-            "_CTM_ncv_ratio": self.ncv_ratio,
-            "_CTM_cutoff": self.cutoff,
-            "_CTM_degen_ratio": self.degen_ratio,
-        }
+        data = {"_CTM_chi_target": self.chi_target}
+        data.update({f"_CTM_{name}": getattr(self, name) for name in _PARAMS})
         data |= self._env.get_data_dic()
         np.savez_compressed(filename, **data, **additional_data)
-
-    @property
-    def Dmax(self):
-        return self._env.Dmax
-
-    @property
-    def Dmin(self):
-        return self._env.Dmin
-
-    @property
-    def Lx(self):
-        return self._env.Lx
-
-    @property
-    def Ly(self):
-        return self._env.Ly
-
-    @property
-    def cell(self):
-        return self._env.cell
-
-    @property
-    def chi_values(self):
-        return self._env.chi_values
-
-    @property
-    def elementary_tensors(self):
-        return self._env.elementary_tensors
-
-    @property
-    def n_sites(self):
-        return self._env.n_sites
 
     @property
     def site_coords(self):
@@ -134,110 +100,57 @@ class AbstractCTMRG:
 
     @property
     def tiling(self):
-        return "\n".join("".join(s) for s in self.cell)
+        return "\n".join("".join(row) for row in self.cell)
 
     def __repr__(self):
-        return f"{self.symmetry()} symmetric CTMRG with Dmax = {self.Dmax} and chi_target = {self.chi_target}"
+        return f"{self.symmetry()} CTMRG, chi_target {self.chi_target}, Dmax {self.Dmax}"
 
     def __str__(self):
-        return "\n".join(
-            (
-                repr(self),
-                f"chi values = {self.chi_values}",
-                f"block_chi_ratio = {self.block_chi_ratio}",
-                f"ncv_ratio = {self.ncv_ratio}",
-                f"cutoff = {self.cutoff}",
-                f"degen_ratio = {self.degen_ratio}",
-                f"unit cell =\n{self._env.cell}",
-            )
-        )
+        params = ", ".join(f"{name} = {getattr(self, name)}" for name in _PARAMS)
+        return f"{self!r}\nchi values {self.chi_values}; {params}\ncell:\n{self._env.cell}"
 
     def symmetry(self):
         return self._env.symmetry()
-
-    def get_A(self, x, y):
-        return self._env.get_A(x, y)
-
-    def get_C1(self, x, y):
-        return self._env.get_C1(x, y)
-
-    def get_C2(self, x, y):
-        return self._env.get_C2(x, y)
-
-    def get_C3(self, x, y):
-        return self._env.get_C3(x, y)
-
-    def get_C4(self, x, y):
-        return self._env.get_C4(x, y)
-
-    def get_T1(self, x, y):
-        return self._env.get_T1(x, y)
-
-    def get_T2(self, x, y):
-        return self._env.get_T2(x, y)
-
-    def get_T3(self, x, y):
-        return self._env.get_T3(x, y)
-
-    def get_T4(self, x, y):
-        return self._env.get_T4(x, y)
-
-    def get_corner_representations(self):
-        return self._env.get_corner_representations()
 
     def set_symmetry(self, symmetry):
         if self.symmetry() != symmetry:
             self._env.set_symmetry(symmetry)
 
-    # observables ---------------------------------------------------------
-    def compute_PEPS_norm_log(self):
-        """Log of the network norm per site (abstract_ctmrg.py:337-387)."""
+    # windows of the environment ------------------------------------------------------
+    def _window(self, x, y, w, h):
+        """Tensors around the w x h sites whose upper-left site is (x+1, y+1),
+        row by row: C1, w T1, C2 / (T4, w A, T2) x h / C4, w T3, C3."""
         e = self._env
-        logZ = 0.0
+        out = [e.get_C1(x, y), *(e.get_T1(x + 1 + i, y) for i in range(w)), e.get_C2(x + 1 + w, y)]
+        for j in range(1, h + 1):
+            out += [e.get_T4(x, y + j), *(e.get_A(x + 1 + i, y + j) for i in range(w)), e.get_T2(x + 1 + w, y + j)]
+        out += [e.get_C4(x, y + 1 + h), *(e.get_T3(x + 1 + i, y + 1 + h) for i in range(w)),
+                e.get_C3(x + 1 + w, y + 1 + h)]
+        return out
+
+    # observables ---------------------------------------------------------------------
+    def compute_PEPS_norm_log(self):
+        """Log of the network norm per site (abstract_ctmrg.py:337-387): with Z_wh
+        the contraction of the w x h window, log Z = log Z_11 + log Z_00 -
+        log Z_10 - log Z_01 averaged over the cell sites."""
+        total = 0.0
         for x, y in self._site_coords:
-            nc = contract_C1234(e.get_C1(x, y), e.get_C2(x + 1, y), e.get_C4(x, y + 1), e.get_C3(x + 1, y + 1))
-            nt13 = contract_T1T3(
-                e.get_C1(x, y), e.get_T1(x + 1, y), e.get_C2(x + 2, y), e.get_C4(x, y + 1),
-                e.get_T3(x + 1, y + 1), e.get_C3(x + 2, y + 1),
-            )
-            nt24 = contract_T2T4(
-                e.get_C1(x, y), e.get_C2(x + 1, y), e.get_T4(x, y + 1), e.get_T2(x + 1, y + 1),
-                e.get_C4(x, y + 2), e.get_C3(x + 1, y + 2),
-            )
-            n1234 = contract_norm(
-                e.get_C1(x, y), e.get_T1(x + 1, y), e.get_C2(x + 2, y), e.get_T4(x, y + 1),
-                e.get_A(x + 1, y + 1), e.get_T2(x + 2, y + 1), e.get_C4(x, y + 2),
-                e.get_T3(x + 1, y + 2), e.get_C3(x + 2, y + 2),
-            )
-            if nc * nt13 * nt24 * n1234 < 0:
+            z = (contract_norm(*self._window(x, y, 1, 1)), contract_C1234(*self._window(x, y, 0, 0)),
+                 contract_T1T3(*self._window(x, y, 1, 0)), contract_T2T4(*self._window(x, y, 0, 1)))
+            if np.prod(z) < 0:
                 print("Warning: negative PEPS norm")
                 return np.nan
-            logZ += np.log(abs(n1234)) + np.log(abs(nc)) - np.log(abs(nt13)) - np.log(abs(nt24))
-        return logZ / self.n_sites
+            total += np.log(abs(z[0])) + np.log(abs(z[1])) - np.log(abs(z[2])) - np.log(abs(z[3]))
+        return total / self.n_sites
 
     def compute_rdm1x1(self, x, y):
-        e = self._env
-        return rdm.rdm_1x1(
-            e.get_C1(x, y), e.get_T1(x + 1, y), e.get_C2(x + 2, y), e.get_T4(x, y + 1),
-            e.get_A(x + 1, y + 1), e.get_T2(x + 2, y + 1), e.get_C4(x, y + 2),
-            e.get_T3(x + 1, y + 2), e.get_C3(x + 2, y + 2),
-        )
+        return rdm.rdm_1x1(*self._window(x, y, 1, 1))
 
     def compute_rdm1x2(self, x, y):
-        e = self._env
-        return rdm.rdm_1x2(
-            e.get_C1(x, y), e.get_T1(x + 1, y), e.get_T1(x + 2, y), e.get_C2(x + 3, y),
-            e.get_T4(x, y + 1), e.get_A(x + 1, y + 1), e.get_A(x + 2, y + 1), e.get_T2(x + 3, y + 1),
-            e.get_C4(x, y + 2), e.get_T3(x + 1, y + 2), e.get_T3(x + 2, y + 2), e.get_C3(x + 3, y + 2),
-        )
+        return rdm.rdm_1x2(*self._window(x, y, 2, 1))
 
     def compute_rdm2x1(self, x, y):
-        e = self._env
-        return rdm.rdm_2x1(
-            e.get_C1(x, y), e.get_T1(x + 1, y), e.get_C2(x + 2, y), e.get_T4(x, y + 1),
-            e.get_A(x + 1, y + 1), e.get_T2(x + 2, y + 1), e.get_T4(x, y + 2), e.get_A(x + 1, y + 2),
-            e.get_T2(x + 2, y + 2), e.get_C4(x, y + 3), e.get_T3(x + 1, y + 3), e.get_C3(x + 2, y + 3),
-        )
+        return rdm.rdm_2x1(*self._window(x, y, 1, 2))
 
     def compute_rdm_diag_dr(self, x, y, *, free_memory=False):
         """Sites (x+1, y+1) and (x+2, y+2) (abstract_ctmrg.py:438-455)."""
@@ -312,16 +225,10 @@ class AbstractCTMRG:
         return -self.Ly / np.log(np.abs(v2))
 
     def print_tensor_shapes(self):
-        """(abstract_ctmrg.py:321-335)"""
-        e = self._env
-        print("tensor shapes for C1 T1 C2 // T4 A T2 // C4 T3 C4:")
+        """Shapes of the 3 x 3 window of every cell site (abstract_ctmrg.py:321-335)."""
+        print("tensor shapes, C1 T1 C2 // T4 A T2 // C4 T3 C3:")
         for x, y in self._site_coords:
-            print(
-                f"({x},{y}):",
-                e.get_C1(x, y).shape, e.get_T1(x + 1, y).shape, e.get_C2(x + 2, y).shape,
-                e.get_T4(x, y + 1).shape, e.get_A(x + 1, y + 1).shape, e.get_T2(x + 2, y + 1).shape,
-                e.get_C4(x, y + 3).shape, e.get_T3(x + 1, y + 2).shape, e.get_C3(x + 2, y + 2).shape,
-            )
+            print(f"({x},{y}):", *(t.shape for t in self._window(x, y, 1, 1)))
 
     def _projectors_batch(self, problems):
         """Projector pairs of several independent (corner1, corner2, corner3,
@@ -375,3 +282,21 @@ class AbstractCTMRG:
             lambda: contract_ur(e.get_T1(x + 2, y), e.get_C2(x + 3, y), e.get_A(x + 2, y + 1), e.get_T2(x + 3, y + 1)),
             x, y, free_memory,
         )
+
+
+def _env_property(name):
+    return property(lambda self: getattr(self._env, name), doc=f"{name} of the environment")
+
+
+def _env_getter(name):
+    def get(self, *args):
+        return getattr(self._env, name)(*args)
+
+    get.__name__ = get.__qualname__ = name
+    return get
+
+
+for _name in _ENV_PROPERTIES:
+    setattr(AbstractCTMRG, _name, _env_property(_name))
+for _name in _ENV_GETTERS:
+    setattr(AbstractCTMRG, _name, _env_getter(_name))

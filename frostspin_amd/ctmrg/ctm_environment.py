@@ -67,167 +67,48 @@ def parse_tiling(tiling):
     return np.atleast_2d(np.genfromtxt(txt, dtype="U1"))
 
 
+# tensors kept per inequivalent site, in npz order (ctm_environment.py:239-281)
+_SITE_TENSORS = ("A", "C1", "C2", "C3", "C4", "T1", "T2", "T3", "T4")
+_EXPECTED_NDIM = {"A": 6, "C1": 2, "C2": 2, "C3": 2, "C4": 2, "T1": 4, "T2": 4, "T3": 4, "T4": 4}
+# one move renormalizes a corner, the edge and the next corner on one side
+_MOVE_TARGETS = {
+    "up": ("C1", "T1", "C2"),
+    "right": ("C2", "T2", "C3"),
+    "down": ("C3", "T3", "C4"),
+    "left": ("C4", "T4", "C1"),
+}
+_ENLARGED = ("ul", "ur", "dl", "dr")
+
+
 class CTMEnvironment:
-    """Container for CTMRG environment tensors, CTMRG leg conventions."""
+    """Device-resident CTMRG environment of a periodic unit cell (interface and
+    leg conventions of ctmrg/ctm_environment.py:62-460).
+
+    Storage: one list per tensor kind (site tensor A, corners C1..C4, edges
+    T1..T4) indexed by inequivalent site, plus the per-move scratch (projectors
+    of each direction, renormalized corner/edge/corner) and the cached enlarged
+    corners. (x, y) is periodic and maps to a site through the cell."""
 
     def __init__(self, cell, As, C1s, C2s, C3s, C4s, T1s, T2s, T3s, T4s):
         self._cell = cell
         self._site_coords, self._indices = find_unique_coords(cell)
         self._n_sites = self._site_coords.shape[0]
         self._Ly, self._Lx = cell.shape
-        self._unique_As = tuple(As)
-        self._unique_C1s = list(C1s)
-        self._unique_C2s = list(C2s)
-        self._unique_C3s = list(C3s)
-        self._unique_C4s = list(C4s)
-        self._unique_T1s = list(T1s)
-        self._unique_T2s = list(T2s)
-        self._unique_T3s = list(T3s)
-        self._unique_T4s = list(T4s)
+        self._site = {"A": tuple(As)}
+        for name, ts in zip(_SITE_TENSORS[1:], (C1s, C2s, C3s, C4s, T1s, T2s, T3s, T4s), strict=True):
+            self._site[name] = list(ts)
         self._symmetry = type(As[0]).symmetry()
         self.check_consistency()
         self.reset_constructed_corners()
-        self._reset_temp_lists()
+        self._clear_move()
 
-    @property
-    def Dmax(self):
-        return max(max(A.shape[2:]) for A in self._unique_As)
-
-    @property
-    def Dmin(self):
-        return min(min(A.shape[2:]) for A in self._unique_As)
-
-    @property
-    def Lx(self):
-        return self._Lx
-
-    @property
-    def Ly(self):
-        return self._Ly
-
-    @property
-    def cell(self):
-        return self._cell
-
-    @property
-    def chi_values(self):
-        s = set()
-        for C in self._unique_C1s + self._unique_C3s:
-            s.add(C.shape[0])
-            s.add(C.shape[1])
-        return sorted(s)
-
-    @property
-    def elementary_tensors(self):
-        return self._unique_As
-
-    @property
-    def n_sites(self):
-        return self._n_sites
-
-    @property
-    def site_coords(self):
-        return self._site_coords
-
-    def check_consistency(self):
-        ST = type(self._unique_As[0])
-        groups = (
-            (self._unique_As, 6, "A"), (self._unique_C1s, 2, "C1"), (self._unique_C2s, 2, "C2"),
-            (self._unique_C3s, 2, "C3"), (self._unique_C4s, 2, "C4"), (self._unique_T1s, 4, "T1"),
-            (self._unique_T2s, 4, "T2"), (self._unique_T3s, 4, "T3"), (self._unique_T4s, 4, "T4"),
-        )
-        for tensors, ndim, name in groups:
-            if len(tensors) != self._n_sites:
-                raise ValueError(f"Invalid {name} number")
-            for t in tensors:
-                if type(t) is not ST:
-                    raise ValueError(f"Invalid {name} type")
-                if t.ndim != ndim:
-                    raise ValueError(f"Invalid {name} ndim")
-        for x, y in self._site_coords:
-            A = self.get_A(x, y)
-            Ay = self.get_A(x, y - 1)
-            if A.signature[2] == Ay.signature[4]:
-                raise ValueError(f"signature mismatch between {(x, y)} and {(x, y - 1)}")
-            if (np.asarray(A.col_reps[0]) != np.asarray(Ay.col_reps[2])).any():
-                raise ValueError(f"rep mismatch between {(x, y)} and {(x, y - 1)}")
-            Ax = self.get_A(x + 1, y)
-            if A.signature[3] == Ax.signature[5]:
-                raise ValueError(f"signature mismatch between {(x + 1, y)} and {(x, y)}")
-            if (np.asarray(A.col_reps[1]) != np.asarray(Ax.col_reps[3])).any():
-                raise ValueError(f"rep mismatch between {(x + 1, y)} and {(x, y)}")
-
-    def get_corner_representations(self):
-        seen, unique = set(), []
-        for C in self._unique_C1s + self._unique_C3s:
-            for r in (C.row_reps[0], C.col_reps[0]):
-                key = tuple(np.asarray(r).ravel())
-                if key not in seen:
-                    seen.add(key)
-                    unique.append(r)
-        ST = type(self._unique_As[0])
-        return sorted(unique, key=lambda r: (ST.representation_dimension(r), tuple(np.asarray(r).ravel())))
-
-    @classmethod
-    def from_elementary_tensors(cls, tiling, tensors):
-        cell = parse_tiling(tiling)
-        if len(tensors) != len(set(cell.flat)):
-            raise ValueError("Incompatible cell and tensor number")
-        env = [initialize_dummy_env(A) for A in tensors]
-        C1s, T1s, C2s, T2s, C3s, T3s, C4s, T4s = (list(x) for x in zip(*env, strict=True))
-        return cls(cell, tensors, C1s, C2s, C3s, C4s, T1s, T2s, T3s, T4s)
-
-    @classmethod
-    def load_from_file(cls, savefile):
-        """npz schema of ctm_environment.py:239-281."""
-        names = ("A", "C1", "C2", "C3", "C4", "T1", "T2", "T3", "T4")
-        lists = {k: [] for k in names}
-        with np.load(savefile) as data:
-            cell = data["_CTM_cell"]
-            ST = get_symmetric_tensor_type(str(data["_CTM_symmetry"][()]))
-            for i in range(len(set(cell.flat))):
-                for k in names:
-                    lists[k].append(ST.load_from_dic(data, prefix=f"_CTM_{k}_{i}"))
-        return cls(cell, *(lists[k] for k in names))
-
-    def symmetry(self):
-        return self._symmetry
-
-    def get_data_dic(self):
-        data = {"_CTM_cell": self._cell, "_CTM_symmetry": self._symmetry}
-        for i in range(self._n_sites):
-            for k, lst in (
-                ("A", self._unique_As), ("C1", self._unique_C1s), ("C2", self._unique_C2s),
-                ("C3", self._unique_C3s), ("C4", self._unique_C4s), ("T1", self._unique_T1s),
-                ("T2", self._unique_T2s), ("T3", self._unique_T3s), ("T4", self._unique_T4s),
-            ):
-                data |= lst[i].get_data_dic(prefix=f"_CTM_{k}_{i}")
-        return data
-
-    def reset_constructed_corners(self):
-        self._corners_ul = [None] * self._n_sites
-        self._corners_ur = [None] * self._n_sites
-        self._corners_dl = [None] * self._n_sites
-        self._corners_dr = [None] * self._n_sites
-
-    def set_tensors(self, tensors):
-        if len(tensors) != self._n_sites:
-            raise ValueError("Incompatible cell and tensors")
-        for i, A in enumerate(tensors):
-            if not A.match_representations(self._unique_As[i]):
-                raise ValueError("Incompatible representation for new tensor")
-        self._unique_As = tuple(tensors)
-        forget_site_permutes()
-        self.reset_constructed_corners()
-
-    def set_symmetry(self, symmetry):
-        if symmetry != self._symmetry:
-            self.reset_constructed_corners()
-            self._symmetry = symmetry
-            self._unique_As = tuple(A.cast(symmetry) for A in self._unique_As)
-            for name in ("C1", "C2", "C3", "C4", "T1", "T2", "T3", "T4"):
-                attr = f"_unique_{name}s"
-                setattr(self, attr, [t.cast(symmetry) for t in getattr(self, attr)])
+    # cell geometry ----------------------------------------------------------------
+    Lx = property(lambda self: self._Lx)
+    Ly = property(lambda self: self._Ly)
+    cell = property(lambda self: self._cell)
+    n_sites = property(lambda self: self._n_sites)
+    site_coords = property(lambda self: self._site_coords)
+    elementary_tensors = property(lambda self: self._site["A"])
 
     def _unique_index(self, x, y):
         return self._indices[x % self._Lx, y % self._Ly]
@@ -235,126 +116,173 @@ class CTMEnvironment:
     def get_tensor_type(self, x, y):
         return self._cell[x % self._Lx, y % self._Ly]
 
-    def get_A(self, x, y):
-        return self._unique_As[self._unique_index(x, y)]
+    def symmetry(self):
+        return self._symmetry
 
-    def get_C1(self, x, y):
-        return self._unique_C1s[self._unique_index(x, y)]
+    @property
+    def Dmax(self):
+        return max(d for A in self._site["A"] for d in A.shape[2:])
 
-    def get_T1(self, x, y):
-        return self._unique_T1s[self._unique_index(x, y)]
+    @property
+    def Dmin(self):
+        return min(d for A in self._site["A"] for d in A.shape[2:])
 
-    def get_C2(self, x, y):
-        return self._unique_C2s[self._unique_index(x, y)]
+    @property
+    def chi_values(self):
+        """Distinct corner dimensions over the cell."""
+        return sorted({d for C in self._site["C1"] + self._site["C3"] for d in C.shape})
 
-    def get_T2(self, x, y):
-        return self._unique_T2s[self._unique_index(x, y)]
+    def get_corner_representations(self):
+        """Distinct corner leg representations, by dimension then content."""
+        reps = {}
+        for C in self._site["C1"] + self._site["C3"]:
+            for r in (C.row_reps[0], C.col_reps[0]):
+                reps.setdefault(tuple(np.asarray(r).ravel()), r)
+        ST = type(self._site["A"][0])
+        return sorted(reps.values(), key=lambda r: (ST.representation_dimension(r), tuple(np.asarray(r).ravel())))
 
-    def get_C3(self, x, y):
-        return self._unique_C3s[self._unique_index(x, y)]
+    def check_consistency(self):
+        """Every kind has one tensor of the right type and rank per site, and
+        neighbouring site tensors share bond representations with opposite
+        signatures."""
+        ST = type(self._site["A"][0])
+        for name in _SITE_TENSORS:
+            ts = self._site[name]
+            if len(ts) != self._n_sites:
+                raise ValueError(f"{name}: {len(ts)} tensors for {self._n_sites} sites")
+            for t in ts:
+                if type(t) is not ST:
+                    raise ValueError(f"{name}: {type(t).__name__} among {ST.__name__} tensors")
+                if t.ndim != _EXPECTED_NDIM[name]:
+                    raise ValueError(f"{name}: rank {t.ndim} instead of {_EXPECTED_NDIM[name]}")
+        # (neighbour offset, leg of A, leg of the neighbour)
+        for x, y in self._site_coords:
+            A = self.get_A(x, y)
+            for (dx, dy), leg, nleg in (((0, -1), 2, 4), ((1, 0), 3, 5)):
+                B = self.get_A(x + dx, y + dy)
+                where = f"sites {(x, y)} and {(x + dx, y + dy)}"
+                if A.signature[leg] == B.signature[nleg]:
+                    raise ValueError(f"signature mismatch between {where}")
+                if not np.array_equal(np.asarray(A.col_reps[leg - 2]), np.asarray(B.col_reps[nleg - 2])):
+                    raise ValueError(f"representation mismatch between {where}")
 
-    def get_T3(self, x, y):
-        return self._unique_T3s[self._unique_index(x, y)]
+    # construction, I/O -------------------------------------------------------------
+    @classmethod
+    def from_elementary_tensors(cls, tiling, tensors):
+        cell = parse_tiling(tiling)
+        if len(tensors) != len(set(cell.flat)):
+            raise ValueError("the cell and the number of tensors do not agree")
+        per_site = [initialize_dummy_env(A) for A in tensors]  # C1, T1, C2, T2, C3, T3, C4, T4
+        C1s, T1s, C2s, T2s, C3s, T3s, C4s, T4s = (list(k) for k in zip(*per_site, strict=True))
+        return cls(cell, tensors, C1s, C2s, C3s, C4s, T1s, T2s, T3s, T4s)
 
-    def get_C4(self, x, y):
-        return self._unique_C4s[self._unique_index(x, y)]
+    @classmethod
+    def load_from_file(cls, savefile):
+        """Environment from an npz in the reference's schema (ctm_environment.py:239-281)."""
+        with np.load(savefile) as data:
+            cell = data["_CTM_cell"]
+            ST = get_symmetric_tensor_type(str(data["_CTM_symmetry"][()]))
+            n = len(set(cell.flat))
+            lists = [[ST.load_from_dic(data, prefix=f"_CTM_{k}_{i}") for i in range(n)] for k in _SITE_TENSORS]
+        return cls(cell, *lists)
 
-    def get_T4(self, x, y):
-        return self._unique_T4s[self._unique_index(x, y)]
+    def get_data_dic(self):
+        data = {"_CTM_cell": self._cell, "_CTM_symmetry": self._symmetry}
+        for k in _SITE_TENSORS:
+            for i, t in enumerate(self._site[k]):
+                data |= t.get_data_dic(prefix=f"_CTM_{k}_{i}")
+        return data
 
-    # projectors of the current move
-    def _store(self, d, x, y, P, Pt):
-        j = self._unique_index(x, y)
-        self._proj[d][0][j] = P
-        self._proj[d][1][j] = Pt
+    # changing tensors -----------------------------------------------------------------
+    def set_tensors(self, tensors):
+        if len(tensors) != self._n_sites:
+            raise ValueError("the cell and the number of tensors do not agree")
+        for new, old in zip(tensors, self._site["A"], strict=True):
+            if not new.match_representations(old):
+                raise ValueError("a new site tensor does not carry the old representations")
+        self._site["A"] = tuple(tensors)
+        forget_site_permutes()
+        self.reset_constructed_corners()
 
-    def store_up_projectors(self, x, y, P, Pt):
-        self._store("up", x, y, P, Pt)
+    def set_symmetry(self, symmetry):
+        if symmetry == self._symmetry:
+            return
+        self.reset_constructed_corners()
+        self._symmetry = symmetry
+        self._site = {k: (tuple if k == "A" else list)(t.cast(symmetry) for t in ts)
+                      for k, ts in self._site.items()}
 
-    def store_right_projectors(self, x, y, P, Pt):
-        self._store("right", x, y, P, Pt)
+    # per-move scratch: projectors, renormalized tensors, enlarged corners ------------
+    def _clear_move(self):
+        n = self._n_sites
+        self._projectors = {d: ([None] * n, [None] * n) for d in _MOVE_TARGETS}
+        self._renormalized = [[None] * n for _ in range(3)]
 
-    def store_down_projectors(self, x, y, P, Pt):
-        self._store("down", x, y, P, Pt)
+    def reset_constructed_corners(self):
+        self._enlarged = {c: [None] * self._n_sites for c in _ENLARGED}
 
-    def store_left_projectors(self, x, y, P, Pt):
-        self._store("left", x, y, P, Pt)
-
-    def get_up_P(self, x, y):
-        return self._proj["up"][0][self._unique_index(x, y)]
-
-    def get_up_Pt(self, x, y):
-        return self._proj["up"][1][self._unique_index(x, y)]
-
-    def get_right_P(self, x, y):
-        return self._proj["right"][0][self._unique_index(x, y)]
-
-    def get_right_Pt(self, x, y):
-        return self._proj["right"][1][self._unique_index(x, y)]
-
-    def get_down_P(self, x, y):
-        return self._proj["down"][0][self._unique_index(x, y)]
-
-    def get_down_Pt(self, x, y):
-        return self._proj["down"][1][self._unique_index(x, y)]
-
-    def get_left_P(self, x, y):
-        return self._proj["left"][0][self._unique_index(x, y)]
-
-    def get_left_Pt(self, x, y):
-        return self._proj["left"][1][self._unique_index(x, y)]
-
-    # cached enlarged corners
-    def get_corner_ul(self, x, y):
-        return self._corners_ul[self._unique_index(x, y)]
-
-    def get_corner_ur(self, x, y):
-        return self._corners_ur[self._unique_index(x, y)]
-
-    def get_corner_dl(self, x, y):
-        return self._corners_dl[self._unique_index(x, y)]
-
-    def get_corner_dr(self, x, y):
-        return self._corners_dr[self._unique_index(x, y)]
-
-    def set_corner_ul(self, x, y, ul):
-        self._corners_ul[self._unique_index(x, y)] = ul
-
-    def set_corner_ur(self, x, y, ur):
-        self._corners_ur[self._unique_index(x, y)] = ur
-
-    def set_corner_dl(self, x, y, dl):
-        self._corners_dl[self._unique_index(x, y)] = dl
-
-    def set_corner_dr(self, x, y, dr):
-        self._corners_dr[self._unique_index(x, y)] = dr
-
-    # renormalized tensors of the current move
     def store_renormalized_tensors(self, x, y, nCX, nT, nCY):
         j = self._unique_index(x, y)
-        self._nCX[j] = nCX
-        self._nT[j] = nT
-        self._nCY[j] = nCY
+        for lst, t in zip(self._renormalized, (nCX, nT, nCY), strict=True):
+            lst[j] = t
 
-    def _reset_temp_lists(self):
-        n = self._n_sites
-        self._proj = {d: ([None] * n, [None] * n) for d in ("up", "right", "down", "left")}
-        self._nCX = [None] * n
-        self._nT = [None] * n
-        self._nCY = [None] * n
+    def _commit_move(self, direction):
+        for name, lst in zip(_MOVE_TARGETS[direction], self._renormalized, strict=True):
+            self._site[name] = lst
+        self._clear_move()
 
-    def set_renormalized_tensors_up(self):
-        self._unique_C1s, self._unique_T1s, self._unique_C2s = self._nCX, self._nT, self._nCY
-        self._reset_temp_lists()
 
-    def set_renormalized_tensors_right(self):
-        self._unique_C2s, self._unique_T2s, self._unique_C3s = self._nCX, self._nT, self._nCY
-        self._reset_temp_lists()
+def _site_getter(name):
+    def get(self, x, y):
+        return self._site[name][self._unique_index(x, y)]
 
-    def set_renormalized_tensors_down(self):
-        self._unique_C3s, self._unique_T3s, self._unique_C4s = self._nCX, self._nT, self._nCY
-        self._reset_temp_lists()
+    get.__name__ = get.__qualname__ = f"get_{name}"
+    return get
 
-    def set_renormalized_tensors_left(self):
-        self._unique_C4s, self._unique_T4s, self._unique_C1s = self._nCX, self._nT, self._nCY
-        self._reset_temp_lists()
+
+def _projector_accessors(direction):
+    def store(self, x, y, P, Pt):
+        j = self._unique_index(x, y)
+        self._projectors[direction][0][j] = P
+        self._projectors[direction][1][j] = Pt
+
+    def get_P(self, x, y):
+        return self._projectors[direction][0][self._unique_index(x, y)]
+
+    def get_Pt(self, x, y):
+        return self._projectors[direction][1][self._unique_index(x, y)]
+
+    def commit(self):
+        self._commit_move(direction)
+
+    return {
+        f"store_{direction}_projectors": store,
+        f"get_{direction}_P": get_P,
+        f"get_{direction}_Pt": get_Pt,
+        f"set_renormalized_tensors_{direction}": commit,
+    }
+
+
+def _enlarged_accessors(corner):
+    def get(self, x, y):
+        return self._enlarged[corner][self._unique_index(x, y)]
+
+    def put(self, x, y, value):
+        self._enlarged[corner][self._unique_index(x, y)] = value
+
+    return {f"get_corner_{corner}": get, f"set_corner_{corner}": put}
+
+
+# the reference's accessor names (get_A, get_C1 .. get_T4, store_up_projectors,
+# get_up_P, set_renormalized_tensors_up, get_corner_ul, ...), generated from
+# the tables above
+for _name in _SITE_TENSORS:
+    setattr(CTMEnvironment, f"get_{_name}", _site_getter(_name))
+for _d in _MOVE_TARGETS:
+    for _attr, _fn in _projector_accessors(_d).items():
+        _fn.__name__ = _fn.__qualname__ = _attr
+        setattr(CTMEnvironment, _attr, _fn)
+for _c in _ENLARGED:
+    for _attr, _fn in _enlarged_accessors(_c).items():
+        _fn.__name__ = _fn.__qualname__ = _attr
+        setattr(CTMEnvironment, _attr, _fn)

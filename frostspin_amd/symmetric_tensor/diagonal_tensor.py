@@ -1,9 +1,19 @@
-"""DiagonalTensor: singular values / eigenvalues, kept on the host.
+"""DiagonalTensor: the weights (singular values, eigenvalues) of a block-diagonal
+matrix, one value vector per irrep block, each value standing for a multiplet
+of ``block_degen`` equal entries.
 
-Diagonal weights are tiny (at most chi values) and are needed on the host anyway
-for the truncation decision, so they stay numpy arrays as in the reference
-(frostspin/symmetric_tensor/diagonal_tensor.py:5-276). A device copy is made on
-demand when they scale a SymmetricTensor.
+Public interface and npz key schema follow
+frostspin/symmetric_tensor/diagonal_tensor.py:10-276 (constructor arguments,
+getters, arithmetic, ``toarray``, ``get_data_dic`` / ``load_from_dic``).
+
+Layout: all blocks live in ONE packed host vector (``_packed``) with block
+boundaries in ``_bounds``; ``diagonal_blocks`` are views into it. Weights are
+at most chi values and the truncation that produces and consumes them is a
+host decision, so the host copy is the primary one; the device copy that
+scales a SymmetricTensor (its columns or rows) is made once, on first use, and
+dropped when the values change through ``*=`` or ``/=`` (writes through the
+``diagonal_blocks`` views are not tracked). Every arithmetic operation is one
+vectorized expression over the packed vector.
 """
 
 import numpy as np
@@ -11,28 +21,70 @@ import numpy as np
 from .. import _ops
 
 
+def _same_array(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and bool((a == b).all())
+
+
+def _is_scalar(x):
+    return np.isscalar(x) and np.issubdtype(type(x), np.number)
+
+
 class DiagonalTensor:
-    """
-    Diagonal tensor as produced by SVD or eigenvalue computation.
-    Interface follows diagonal_tensor.py:10-276.
-    """
+    """Block-diagonal weights with irrep multiplicities (see module docstring)."""
 
     def __init__(self, diag_blocks, representation, block_irreps, block_degen, symmetry):
-        self._diagonal_blocks = tuple(np.asarray(db) for db in diag_blocks)
+        parts = [np.asarray(b) for b in diag_blocks]
+        if not all(p.ndim == 1 for p in parts):
+            raise ValueError("diagonal blocks must be one-dimensional")
+        sizes = np.array([p.size for p in parts], dtype=np.int64)
+        bounds = np.zeros((len(parts) + 1,), dtype=np.int64)
+        np.cumsum(sizes, out=bounds[1:])
+        dtype = np.result_type(*parts) if parts else np.float64
+        packed = np.empty((int(bounds[-1]),), dtype=dtype)
+        for p, lo, hi in zip(parts, bounds[:-1], bounds[1:], strict=True):
+            packed[lo:hi] = p
+        self._init_packed(packed, bounds, representation, block_irreps, block_degen, symmetry)
+
+    def _init_packed(self, packed, bounds, representation, block_irreps, block_degen, symmetry):
+        self._packed = packed
+        self._bounds = bounds
         self._representation = representation
         self._block_irreps = np.asarray(block_irreps)
         self._block_degen = np.asarray(block_degen)
         self._symmetry = symmetry
-        self._nblocks = len(diag_blocks)
         self._dev = None
-        assert self._block_degen.shape == (self._nblocks,)
-        assert self._block_irreps.shape[0] == self._nblocks
-        assert all(db.ndim == 1 for db in self._diagonal_blocks)
+        n = bounds.size - 1
+        if self._block_degen.shape != (n,) or self._block_irreps.shape[0] != n:
+            raise ValueError("one irrep and one degeneracy per diagonal block are required")
 
-    # getters -------------------------------------------------------------
+    def _like(self, packed):
+        """Same structure, other values (a new packed vector)."""
+        out = DiagonalTensor.__new__(type(self))
+        out._init_packed(packed, self._bounds, self._representation, self._block_irreps, self._block_degen,
+                         self._symmetry)
+        return out
+
+    def _multiplicities(self):
+        """Multiplicity of every packed value (its block's degeneracy)."""
+        return np.repeat(self._block_degen, np.diff(self._bounds))
+
+    def _check_same_structure(self, other):
+        if not _same_array(self._representation, other.representation):
+            raise ValueError("DiagonalTensor representations differ")
+
+    def _device_values(self):
+        """All values as one float64 device vector and each block's start offset
+        (made on first use; in-place changes drop it)."""
+        if self._dev is None:
+            self._dev = (_ops.upload(np.ascontiguousarray(self._packed, dtype=np.float64)), self._bounds)
+        return self._dev
+
+    # getters ------------------------------------------------------------------
     @property
     def diagonal_blocks(self):
-        return self._diagonal_blocks
+        b = self._bounds
+        return tuple(self._packed[b[i] : b[i + 1]] for i in range(b.size - 1))
 
     @property
     def block_degen(self):
@@ -44,7 +96,7 @@ class DiagonalTensor:
 
     @property
     def nblocks(self):
-        return self._nblocks
+        return self._bounds.size - 1
 
     @property
     def ndim(self):
@@ -64,57 +116,35 @@ class DiagonalTensor:
 
     @property
     def dtype(self):
-        return self._diagonal_blocks[0].dtype
+        return self._packed.dtype
 
-    def _device_values(self):
-        """Concatenated values on the device and per-block offsets (cached)."""
-        if self._dev is None:
-            offs = np.zeros((self._nblocks + 1,), dtype=np.int64)
-            offs[1:] = np.cumsum([db.size for db in self._diagonal_blocks])
-            cat = (
-                np.concatenate([np.asarray(db, dtype=np.float64) for db in self._diagonal_blocks])
-                if self._nblocks
-                else np.zeros((0,))
-            )
-            self._dev = (_ops.upload(cat), offs)
-        return self._dev
-
-    def _new(self, blocks):
-        return type(self)(
-            blocks, self._representation, self._block_irreps, self._block_degen, self._symmetry
-        )
-
-    # magic methods -------------------------------------------------------
-    def __repr__(self):
-        return f"DiagonalTensor with {self.nblocks} blocks and {self.symmetry()} symmetry"
+    def symmetry(self):
+        return self._symmetry
 
     def __len__(self):
-        return int(
-            sum(self._diagonal_blocks[i].size * self._block_degen[i] for i in range(self._nblocks))
-        )
+        return int(np.dot(np.diff(self._bounds), self._block_degen)) if self.nblocks else 0
 
+    def __repr__(self):
+        return f"DiagonalTensor ({self._symmetry}) with {self.nblocks} blocks, {len(self)} values"
+
+    # arithmetic -----------------------------------------------------------------
     def __mul__(self, x):
         if isinstance(x, DiagonalTensor):
-            if not (self._representation == x.representation).all():
-                raise ValueError("Representations do not match")
-            blocks = tuple(b1 * b2 for b1, b2 in zip(self._diagonal_blocks, x.diagonal_blocks, strict=True))
-        elif np.issubdtype(type(x), np.number):
-            blocks = tuple(x * db for db in self._diagonal_blocks)
-        else:
-            return NotImplemented
-        return self._new(blocks)
+            self._check_same_structure(x)
+            return self._like(self._packed * x._packed)
+        if _is_scalar(x):
+            return self._like(self._packed * x)
+        return NotImplemented
 
     def __rmul__(self, x):
-        if np.issubdtype(type(x), np.number):
-            return self * x
-        msg = f"Unsupported operation for type: {type(x)}"
-        raise TypeError(msg)
+        if _is_scalar(x):
+            return self._like(x * self._packed)
+        raise TypeError(f"cannot multiply a DiagonalTensor by {type(x)}")
 
     def __imul__(self, x):
-        if not np.issubdtype(type(x), np.number):
-            raise NotImplementedError("Invalid type for *=")
-        for db in self._diagonal_blocks:
-            db[:] *= x
+        if not _is_scalar(x):
+            raise NotImplementedError(f"in-place product with {type(x)}")
+        self._packed *= x
         self._dev = None
         return self
 
@@ -122,94 +152,76 @@ class DiagonalTensor:
         return self * (1.0 / x)
 
     def __rtruediv__(self, x):
-        if np.issubdtype(type(x), np.number):
-            return self._new(tuple(x / db for db in self._diagonal_blocks))
-        raise NotImplementedError("Invalid type for /")
+        if _is_scalar(x):
+            return self._like(x / self._packed)
+        raise NotImplementedError(f"cannot divide {type(x)} by a DiagonalTensor")
 
     def __itruediv__(self, x):
-        self *= 1.0 / x
-        return self
+        return self.__imul__(1.0 / x)
 
     def __pow__(self, x):
-        if not np.issubdtype(type(x), np.number):
-            raise NotImplementedError("Invalid type for **")
-        return self._new(tuple(db**x for db in self._diagonal_blocks))
+        if not _is_scalar(x):
+            raise NotImplementedError(f"power of a DiagonalTensor by {type(x)}")
+        return self._like(self._packed**x)
 
     def __neg__(self):
-        return self._new(tuple(-db for db in self._diagonal_blocks))
-
-    def __add__(self, other):
-        if not isinstance(other, DiagonalTensor):
-            raise NotImplementedError("Invalid type for +")
-        if not (self._representation == other.representation).all():
-            raise ValueError("Representations do not match")
-        return self._new(
-            tuple(b1 + b2 for b1, b2 in zip(self._diagonal_blocks, other.diagonal_blocks, strict=True))
-        )
-
-    def __sub__(self, other):
-        return self + (-other)
+        return self._like(-self._packed)
 
     def __pos__(self):
         return self
 
-    # misc ----------------------------------------------------------------
-    def copy(self):
-        return self._new(tuple(db.copy() for db in self._diagonal_blocks))
+    def __add__(self, other):
+        if not isinstance(other, DiagonalTensor):
+            raise NotImplementedError(f"cannot add {type(other)} to a DiagonalTensor")
+        self._check_same_structure(other)
+        return self._like(self._packed + other._packed)
 
-    def symmetry(self):
-        return self._symmetry
+    def __sub__(self, other):
+        return self + (-other)
+
+    # reductions and export --------------------------------------------------------
+    def copy(self):
+        return self._like(self._packed.copy())
 
     def sum(self):
-        s = 0.0
-        for i in range(self._nblocks):
-            s += self._block_degen[i] * self._diagonal_blocks[i].sum()
-        return s
+        """Sum of all diagonal entries, each value counted with its multiplicity."""
+        return self._packed @ self._multiplicities() if self._packed.size else 0.0
 
     def norm(self):
-        n2 = 0.0
-        for i in range(self._nblocks):
-            n2 += self._block_degen[i] * float(np.sum(np.abs(self._diagonal_blocks[i]) ** 2))
-        return np.sqrt(n2)
+        """Frobenius norm of the full diagonal (multiplicities included)."""
+        a = np.abs(self._packed)
+        return float(np.sqrt((a * a) @ self._multiplicities())) if a.size else 0.0
 
     def toarray(self, *, sort=False):
-        """Weights grouped by irrep, or sorted by non-increasing magnitude if sort."""
-        dt = self.dtype if self._nblocks else np.float64
-        arr = np.empty(self.shape, dtype=dt)
-        k = 0
-        for i in range(self._nblocks):
-            deg = self._block_degen[i]
-            dim = self._diagonal_blocks[i].size
-            arr[k : k + deg * dim].reshape(dim, deg)[:] = self._diagonal_blocks[i][:, None]
-            k += deg * dim
-        assert k == arr.size
+        """Every diagonal entry, block by block, each value repeated block_degen
+        times in a row; sorted by decreasing magnitude if sort."""
+        if not self.nblocks:
+            return np.zeros((0,))
+        arr = np.repeat(self._packed, self._multiplicities())
         if sort:
-            arr = arr[np.abs(arr).argsort()[::-1]]
+            arr = arr[np.argsort(np.abs(arr))[::-1]]
         return arr
 
-    # I/O (npz schema of diagonal_tensor.py:238-276) -----------------------
+    # npz I/O (key schema of diagonal_tensor.py:238-276) -----------------------------
     def save_to_file(self, savefile):
         np.savez_compressed(savefile, **self.get_data_dic())
 
     def get_data_dic(self, *, prefix=""):
         data = {
-            prefix + "_block_irreps": self._block_irreps,
-            prefix + "_block_degen": self._block_degen,
-            prefix + "_representation": self._representation,
-            prefix + "_symmetry": self._symmetry,
+            f"{prefix}_symmetry": self._symmetry,
+            f"{prefix}_representation": self._representation,
+            f"{prefix}_block_irreps": self._block_irreps,
+            f"{prefix}_block_degen": self._block_degen,
         }
-        for bi, db in enumerate(self._diagonal_blocks):
-            data[f"{prefix}_diagonal_block_{bi}"] = db
+        data.update({f"{prefix}_diagonal_block_{i}": b for i, b in enumerate(self.diagonal_blocks)})
         return data
 
     @classmethod
     def load_from_dic(cls, data, *, prefix=""):
-        symmetry = str(data[prefix + "_symmetry"])
-        block_irreps = data[prefix + "_block_irreps"]
-        block_degen = data[prefix + "_block_degen"]
-        representation = data[prefix + "_representation"]
-        diag_blocks = tuple(data[f"{prefix}_diagonal_block_{bi}"] for bi in range(block_irreps.size))
-        return cls(diag_blocks, representation, block_irreps, block_degen, symmetry)
+        irreps = data[f"{prefix}_block_irreps"]
+        blocks = [data[f"{prefix}_diagonal_block_{i}"] for i in range(irreps.size)]
+        return cls(blocks, data[f"{prefix}_representation"], irreps, data[f"{prefix}_block_degen"],
+                   str(data[f"{prefix}_symmetry"]))
 
     @classmethod
     def load_from_file(cls, savefile, *, prefix=""):
