@@ -176,7 +176,7 @@ def run_contract_permute(args):
         },
         "permute": {
             "bound": "hbm",
-            "kernel": "permute_gather_kernel",
+            "kernel": "permute_box_kernel (LDS-staged box tiles)",
             "avg_ms": float(perm_ms.mean()),
             "achieved": perm_bytes / (perm_ms.mean() * 1e-3) / 1e9,
             "peak": PEAK_HBM / 1e9,

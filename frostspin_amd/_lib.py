@@ -13,6 +13,8 @@ import numpy as np
 _LIB_PATH = Path(__file__).resolve().parent / "libfrostspin_amd.so"
 
 FS_NORM_PARTIALS = 1024
+FS_PERMUTE_MAX_DIGITS = 12
+FS_PERMUTE_TILE_RECORD = 5 + 5 * FS_PERMUTE_MAX_DIGITS
 
 c_i64 = ctypes.c_int64
 c_int = ctypes.c_int
@@ -182,10 +184,7 @@ _SIGNATURES = {
     ),
     "fs_permute_plan_info": (c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
     "fs_permute_plan_blocks": (c_int, [c_vp, c_i8p, c_i64p]),
-    "fs_permute_plan_tables": (
-        c_int,
-        [c_vp, c_i64p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i32p, c_i64p, c_i64p, c_i64p],
-    ),
+    "fs_permute_plan_tiles": (c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
     "fs_permute_plan_execute_f64": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "fs_permute_plan_execute_f32": (c_int, [c_vp, c_vp, c_vp, c_vp]),
     "fs_permute_plan_destroy": (c_int, [c_vp]),

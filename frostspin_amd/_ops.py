@@ -254,7 +254,7 @@ def upload_async(arr, dtype=DTYPE):
 # permute plans
 # ---------------------------------------------------------------------------
 class PermutePlan:
-    """Owner of a native permute plan (host tables + lazily uploaded device copy)."""
+    """Owner of a native permute plan (host box tiles + lazily uploaded device copy)."""
 
     __slots__ = ("_handle", "irreps", "new_ncoeff", "old_ncoeff", "shapes")
 
@@ -271,32 +271,16 @@ class PermutePlan:
         self.new_ncoeff = nc.value
         self.old_ncoeff = oc.value
 
-    def tables(self):
-        """Host copies of the index tables (inspection and tests)."""
-        sizes = np.zeros((5,), dtype=np.int64)
-        check(lib.fs_permute_plan_tables(self._handle, _lib.i64p(sizes), *([None] * 9)))
-        nor, noc, nnr, nnc, nob = (int(x) for x in sizes)
-        t = {
-            "old_row_block": np.empty((nor,), dtype=np.int32),
-            "old_row_pos": np.empty((nor,), dtype=np.int32),
-            "old_col_block": np.empty((noc,), dtype=np.int32),
-            "old_col_pos": np.empty((noc,), dtype=np.int32),
-            "row_a": np.empty((2 * nnr,), dtype=np.int32),
-            "col_b": np.empty((2 * nnc,), dtype=np.int32),
-            "old_off": np.empty((nob,), dtype=np.int64),
-            "old_rs": np.empty((nob,), dtype=np.int64),
-            "old_cs": np.empty((nob,), dtype=np.int64),
-        }
-        names32 = ("old_row_block", "old_row_pos", "old_col_block", "old_col_pos", "row_a", "col_b")
-        check(
-            lib.fs_permute_plan_tables(
-                self._handle,
-                _lib.i64p(sizes),
-                *(_lib.i32p(t[k]) for k in names32),
-                *(_lib.i64p(t[k]) for k in ("old_off", "old_rs", "old_cs")),
-            )
-        )
-        return t
+    def tiles(self):
+        """Host copy of the plan's box tiles (inspection and tests): an
+        (ntiles, FS_PERMUTE_TILE_RECORD) int64 array laid out as documented for
+        fs_permute_plan_tiles, and the number of boxes."""
+        nt = ctypes.c_int64()
+        nb = ctypes.c_int64()
+        check(lib.fs_permute_plan_tiles(self._handle, ctypes.byref(nt), ctypes.byref(nb), None))
+        out = np.zeros((nt.value, _lib.FS_PERMUTE_TILE_RECORD), dtype=np.int64)
+        check(lib.fs_permute_plan_tiles(self._handle, ctypes.byref(nt), ctypes.byref(nb), _lib.i64p(out)))
+        return out, nb.value
 
     def execute(self, src, dst):
         with _timed("permute", 2.0 * dst.element_size() * self.new_ncoeff):

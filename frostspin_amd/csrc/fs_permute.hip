@@ -1,13 +1,15 @@
-// Abelian permute executor: one gather kernel over the packed output blocks.
+// Abelian permute executor: one workgroup per box tile (see fs_plan.h).
 //
-// Every output coefficient (new block b, row i, col j) looks up the old dense
-// (row, col) it comes from as row_a[i] + col_b[j] (two int32 pairs built by
-// the planner), then the old block / in-block position of that row and
-// column. Writes are fully coalesced (consecutive threads write consecutive
-// coefficients of one block); reads are gathers served mostly by L2.
-// Semantics follow _numba_abelian_transpose (abelian_symmetric_tensor.py:
-// 103-250) inverted into destination order; coefficients whose old block is
-// missing read as zero exactly as the reference's zero-initialised blocks.
+// A tile is a dense box of at most kPermuteTileVol coefficients whose source
+// and destination positions are affine in the legs' digits. Load phase: the
+// workgroup walks the tile in source order (fastest source digit first, so
+// neighbouring lanes read neighbouring addresses along the longest contiguous
+// source run) and drops each coefficient into LDS at its destination-order
+// offset. Store phase: it walks the LDS copy in destination order and writes
+// contiguous destination runs. Digit decoding uses one multiply-high per digit
+// (q = umulhi(f, ceil(2^32 / len)) is exact for f < 2^12). Semantics follow
+// _numba_abelian_transpose (abelian_symmetric_tensor.py:103-250); tiles of an
+// absent old block write zeros, as the reference's zero-initialised blocks.
 #include <cstring>
 #include <memory>
 
@@ -16,55 +18,53 @@
 
 namespace {
 
-struct PermDev {
-  const int32_t *old_row_block, *old_row_pos, *old_col_block, *old_col_pos;
-  const int32_t *row_a, *col_b;
-  const int64_t *old_off, *old_rs, *old_cs;
-  const int64_t *new_off, *new_rowstart, *new_colstart, *new_shapes;
-  const int64_t *tile_prefix;
-  int32_t nblocks;
-};
+constexpr int kPermThreads = 256;
 
 template <typename T>
-__global__ void __launch_bounds__(256)
-permute_gather_kernel(const T *__restrict__ src, T *__restrict__ dst, PermDev p) {
-  const int64_t tile = blockIdx.x;
-  int lo = 0, hi = p.nblocks;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (p.tile_prefix[mid] <= tile) lo = mid;
-    else hi = mid;
-  }
-  const int b = lo;
-  const int64_t nr = p.new_shapes[2 * b];
-  const uint32_t nc = static_cast<uint32_t>(p.new_shapes[2 * b + 1]);
-  const uint32_t n = static_cast<uint32_t>(nr * nc);
-  const uint32_t base = static_cast<uint32_t>((tile - p.tile_prefix[b]) * kPermuteTile);
-  const int2 *ra = reinterpret_cast<const int2 *>(p.row_a) + p.new_rowstart[b];
-  const int2 *cb = reinterpret_cast<const int2 *>(p.col_b) + p.new_colstart[b];
-  T *out = dst + p.new_off[b];
-#pragma unroll 4
-  for (int k = 0; k < static_cast<int>(kPermuteTile / 256); ++k) {
-    const uint32_t e = base + static_cast<uint32_t>(k * 256) + threadIdx.x;
-    if (e < n) {
-      const uint32_t i = e / nc;
-      const uint32_t j = e - i * nc;
-      const int2 a = ra[i];
-      const int2 c = cb[j];
-      const int32_t R = a.x + c.x;
-      const int32_t C = a.y + c.y;
-      const int32_t ob = p.old_row_block[R];
-      T v = T(0);
-      if (ob >= 0 && ob == p.old_col_block[C]) {
-        v = src[p.old_off[ob] + static_cast<int64_t>(p.old_row_pos[R]) * p.old_rs[ob] +
-                static_cast<int64_t>(p.old_col_pos[C]) * p.old_cs[ob]];
+__global__ void __launch_bounds__(kPermThreads)
+permute_box_kernel(const T *__restrict__ src, T *__restrict__ dst, const PermTile *__restrict__ tiles) {
+  __shared__ T buf[kPermuteTileVol];
+  const PermTile &t = tiles[blockIdx.x];
+  const int vol = t.vol;
+  T *out = dst + t.dst_base;
+  if (t.src_base >= 0) {
+    const T *in = src + t.src_base;
+    const int ns = t.ns;
+    for (int f = threadIdx.x; f < vol; f += kPermThreads) {
+      uint32_t rem = static_cast<uint32_t>(f);
+      int32_t so = 0, lo = 0;
+#pragma unroll
+      for (int k = 0; k < kPermuteMaxDigits; ++k) {
+        if (k < ns) {
+          const uint32_t q = __umulhi(rem, t.smag[k]);
+          const int32_t r = static_cast<int32_t>(rem - q * static_cast<uint32_t>(t.slen[k]));
+          so += r * t.sstr[k];
+          lo += r * t.lstr[k];
+          rem = q;
+        }
       }
-      out[e] = v;
+      buf[lo] = in[so];
     }
+  } else {
+    for (int f = threadIdx.x; f < vol; f += kPermThreads) buf[f] = T(0);
+  }
+  __syncthreads();
+  const int nd = t.nd;
+  for (int f = threadIdx.x; f < vol; f += kPermThreads) {
+    uint32_t rem = static_cast<uint32_t>(f);
+    int32_t doff = 0;
+#pragma unroll
+    for (int k = 0; k < kPermuteMaxDigits; ++k) {
+      if (k < nd) {
+        const uint32_t q = __umulhi(rem, t.dmag[k]);
+        const int32_t r = static_cast<int32_t>(rem - q * static_cast<uint32_t>(t.dlen[k]));
+        doff += r * t.dstr[k];
+        rem = q;
+      }
+    }
+    out[doff] = buf[f];
   }
 }
-
-template <typename V> size_t bytes_of(const std::vector<V> &v) { return v.size() * sizeof(V); }
 
 int upload(fs_permute_plan *p) {
   int dev = 0;
@@ -75,37 +75,10 @@ int upload(fs_permute_plan *p) {
     fs::set_error("permute plan was uploaded on device %d, current device is %d", p->device, dev);
     return FS_ERR_INVALID;
   }
-  // one allocation, 256-byte aligned sub-buffers
-  struct Part { const void *src; size_t bytes; const void **dst; };
-  std::vector<Part> parts = {
-      {p->old_row_block.data(), bytes_of(p->old_row_block), (const void **)&p->d_old_row_block},
-      {p->old_row_pos.data(), bytes_of(p->old_row_pos), (const void **)&p->d_old_row_pos},
-      {p->old_col_block.data(), bytes_of(p->old_col_block), (const void **)&p->d_old_col_block},
-      {p->old_col_pos.data(), bytes_of(p->old_col_pos), (const void **)&p->d_old_col_pos},
-      {p->row_a.data(), bytes_of(p->row_a), (const void **)&p->d_row_a},
-      {p->col_b.data(), bytes_of(p->col_b), (const void **)&p->d_col_b},
-      {p->old_off.data(), bytes_of(p->old_off), (const void **)&p->d_old_off},
-      {p->old_rs.data(), bytes_of(p->old_rs), (const void **)&p->d_old_rs},
-      {p->old_cs.data(), bytes_of(p->old_cs), (const void **)&p->d_old_cs},
-      {p->new_off.data(), bytes_of(p->new_off), (const void **)&p->d_new_off},
-      {p->new_rowstart.data(), bytes_of(p->new_rowstart), (const void **)&p->d_new_rowstart},
-      {p->new_colstart.data(), bytes_of(p->new_colstart), (const void **)&p->d_new_colstart},
-      {p->new_shapes.data(), bytes_of(p->new_shapes), (const void **)&p->d_new_shapes},
-      {p->tile_prefix.data(), bytes_of(p->tile_prefix), (const void **)&p->d_tile_prefix},
-  };
-  size_t total = 0;
-  std::vector<size_t> offs;
-  for (auto &q : parts) {
-    offs.push_back(total);
-    total += (q.bytes + 255) / 256 * 256 + 256;
-  }
-  std::vector<char> host(total, 0);
-  for (size_t i = 0; i < parts.size(); ++i)
-    if (parts[i].bytes) std::memcpy(host.data() + offs[i], parts[i].src, parts[i].bytes);
   auto tables = std::make_unique<fs::DeviceTables>();
-  int st = tables->upload(host.data(), total);
+  int st = tables->upload(p->tiles.data(), p->tiles.size() * sizeof(PermTile));
   if (st != FS_OK) return st;
-  for (size_t i = 0; i < parts.size(); ++i) *parts[i].dst = tables->data() + offs[i];
+  p->d_tiles = reinterpret_cast<const PermTile *>(tables->data());
   p->d_tables = tables.release();
   p->device = dev;
   return FS_OK;
@@ -114,37 +87,17 @@ int upload(fs_permute_plan *p) {
 template <typename T>
 int execute(fs_permute_plan *plan, const T *src, T *dst, void *stream) {
   FS_CHECK_ARG(plan, "permute execute: null plan");
-  const int64_t nb = static_cast<int64_t>(plan->new_irreps.size());
-  if (nb == 0 || plan->new_ncoeff == 0) return FS_OK;
+  if (plan->tiles.empty() || plan->new_ncoeff == 0) return FS_OK;
   FS_CHECK_ARG(dst, "permute execute: null destination");
   FS_CHECK_ARG(src || plan->old_ncoeff == 0, "permute execute: null source");
-  for (int64_t b = 0; b < nb; ++b)
-    FS_CHECK_ARG(plan->new_shapes[2 * b] * plan->new_shapes[2 * b + 1] < (int64_t(1) << 32),
-                 "permute execute: block too large for 32-bit in-block indexing");
+  const int64_t ntiles = static_cast<int64_t>(plan->tiles.size());
+  FS_CHECK_ARG(ntiles < (int64_t(1) << 31), "permute execute: too many tiles");
   int st = upload(plan);
   if (st != FS_OK) return st;
-  PermDev p;
-  p.old_row_block = plan->d_old_row_block;
-  p.old_row_pos = plan->d_old_row_pos;
-  p.old_col_block = plan->d_old_col_block;
-  p.old_col_pos = plan->d_old_col_pos;
-  p.row_a = plan->d_row_a;
-  p.col_b = plan->d_col_b;
-  p.old_off = plan->d_old_off;
-  p.old_rs = plan->d_old_rs;
-  p.old_cs = plan->d_old_cs;
-  p.new_off = plan->d_new_off;
-  p.new_rowstart = plan->d_new_rowstart;
-  p.new_colstart = plan->d_new_colstart;
-  p.new_shapes = plan->d_new_shapes;
-  p.tile_prefix = plan->d_tile_prefix;
-  p.nblocks = static_cast<int32_t>(nb);
-  const int64_t ntiles = plan->tile_prefix.back();
-  FS_CHECK_ARG(ntiles < (int64_t(1) << 31), "permute execute: too many tiles");
   static_cast<fs::DeviceTables *>(plan->d_tables)->note_stream(fs::as_stream(stream));
-  hipLaunchKernelGGL(permute_gather_kernel<T>, dim3(static_cast<unsigned>(ntiles)), dim3(256), 0,
-                     fs::as_stream(stream), src, dst, p);
-  FS_LAUNCH_CHECK("permute_gather_kernel");
+  hipLaunchKernelGGL(permute_box_kernel<T>, dim3(static_cast<unsigned>(ntiles)), dim3(kPermThreads), 0,
+                     fs::as_stream(stream), src, dst, plan->d_tiles);
+  FS_LAUNCH_CHECK("permute_box_kernel");
   return FS_OK;
 }
 

@@ -1,5 +1,12 @@
-// Permute plan: host tables built by the planner, mirrored to device memory
-// on first execution.
+// Abelian permute plan: a list of box tiles built by the planner on the host,
+// mirrored to device memory on first execution.
+//
+// A box is the set of elements whose index on every leg lies in one run of
+// equal charges of that leg (a "run tuple"). Inside a box the position in the
+// old block and in the new block are both affine in the legs' digits, so the
+// permute of a box is a dense strided copy: a tile (a box, or a slice of a
+// large box) is staged through LDS, read in source order and written in
+// destination order.
 #pragma once
 
 #include <cstdint>
@@ -8,39 +15,40 @@
 
 #include "frostspin_amd.h"
 
-// Elements handled by one workgroup of the permute gather kernel.
-constexpr int64_t kPermuteTile = 2048;
+// Elements of one tile (LDS staging buffer of the permute kernel).
+constexpr int64_t kPermuteTileVol = 4096;
+// Digits (legs with more than one state in the box) of one tile.
+constexpr int kPermuteMaxDigits = FS_PERMUTE_MAX_DIGITS;
+
+// One tile: the source is read digit by digit from the fastest source digit,
+// the destination written digit by digit from the fastest destination digit.
+// src_base < 0: the old block is absent, the tile is written with zeros.
+// smag / dmag: ceil(2^32 / len), so that q = umulhi(f, mag) = f / len for the
+// flat indices f < kPermuteTileVol of a tile.
+struct PermTile {
+  int64_t src_base, dst_base;
+  int32_t vol, ns, nd, pad;
+  uint32_t smag[kPermuteMaxDigits];
+  int32_t slen[kPermuteMaxDigits], sstr[kPermuteMaxDigits], lstr[kPermuteMaxDigits];
+  uint32_t dmag[kPermuteMaxDigits];
+  int32_t dlen[kPermuteMaxDigits], dstr[kPermuteMaxDigits];
+};
 
 struct fs_permute_plan {
-  // old side (per dense row / column of the old matrix view)
-  std::vector<int32_t> old_row_block, old_row_pos, old_col_block, old_col_pos;
-  // old side (per old block)
-  std::vector<int64_t> old_off, old_rs, old_cs;
-  int64_t old_ncoeff = 0;
   // new side (per new block)
   std::vector<int8_t> new_irreps;
   std::vector<int64_t> new_shapes;  // 2 per block
-  std::vector<int64_t> new_off, new_rowstart, new_colstart;
-  // new side (per in-block row / column): old (row, col) index contributions
-  std::vector<int32_t> row_a, col_b;
   int64_t new_ncoeff = 0;
-  int64_t n_old_rows = 0, n_old_cols = 0, n_new_rows_total = 0, n_new_cols_total = 0;
-  std::vector<int64_t> tile_prefix;
+  int64_t old_ncoeff = 0;   // elements of the source buffer the plan may read
+  int64_t nboxes = 0;
+  std::vector<PermTile> tiles;
 
   // device mirror (one fs::DeviceTables, owned), filled lazily by the
   // executor under upload_mu
   void *d_tables = nullptr;
   int device = -1;
   std::mutex upload_mu;
-  const int32_t *d_old_row_block = nullptr, *d_old_row_pos = nullptr;
-  const int32_t *d_old_col_block = nullptr, *d_old_col_pos = nullptr;
-  const int32_t *d_row_a = nullptr, *d_col_b = nullptr;
-  const int64_t *d_old_off = nullptr, *d_old_rs = nullptr, *d_old_cs = nullptr;
-  const int64_t *d_new_off = nullptr, *d_new_rowstart = nullptr;
-  const int64_t *d_new_colstart = nullptr, *d_new_shapes = nullptr;
-  const int64_t *d_tile_prefix = nullptr;
-
-  void finalize_tiles();
+  const PermTile *d_tiles = nullptr;
 };
 
 namespace fs {

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <stdexcept>
 #include <vector>
 
@@ -52,32 +53,237 @@ std::vector<int8_t> fuse_charges(const std::vector<const int8_t *> &legs,
   return acc;
 }
 
-// Kronecker sum of index contributions: out[j*n+k] = acc[j] + v_l[k].
-static std::vector<int64_t>
-kron_sum_i64(const std::vector<std::vector<int64_t>> &parts) {
-  std::vector<int64_t> acc(1, 0);
-  for (const auto &p : parts) {
-    std::vector<int64_t> next(acc.size() * p.size());
-    for (size_t j = 0; j < acc.size(); ++j)
-      for (size_t k = 0; k < p.size(); ++k) next[j * p.size() + k] = acc[j] + p[k];
-    acc.swap(next);
+// ---- abelian permute as box tiles --------------------------------------------
+//
+// The reference permutes coefficient by coefficient (_numba_abelian_transpose,
+// abelian_symmetric_tensor.py:103-250): old (block, row, col) -> dense index ->
+// new (block, row, col), where the rows of a block are the dense row indices of
+// its charge in increasing order (_numpy_get_indices, :78-100). Here the same
+// map is cut into boxes: a box fixes one run of equal charges on every leg.
+// For a row multi-index (i_0 .. i_{k-1}) of charge Q, its rank among the rows
+// of charge Q is
+//     sum_p sum_{t < i_p} S_{p+1}(Q - P_p - q_p(t)),
+// S_p(x) the number of index tuples of legs p..k-1 with total charge x and P_p
+// the charge of the prefix. Within a box every q_p(t) for t in the run is
+// constant, so the rank is base + sum_p (i_p - start_p) * S_{p+1}(Q - P_p - q_p):
+// affine in the digits, on the old and on the new side alike.
+
+namespace {
+
+struct Run {
+  int64_t start, len;
+  uint8_t q;  // raw int8 charge bits
+};
+
+std::vector<Run> leg_runs(const int8_t *c, int64_t n) {
+  std::vector<Run> out;
+  for (int64_t t = 0; t < n; ++t) {
+    const uint8_t q = static_cast<uint8_t>(c[t]);
+    if (!out.empty() && out.back().q == q) ++out.back().len;
+    else out.push_back({t, 1, q});
   }
-  return acc;
+  return out;
 }
 
-// For every entry: its rank among entries of the same charge (stable order),
-// plus per-charge counts. Equivalent to _numpy_get_indices
-// (abelian_symmetric_tensor.py:78-100) whose mergesort keeps index order.
-static void charge_ranks(const std::vector<int8_t> &ch, std::vector<int32_t> &rank,
-                         std::array<int64_t, 256> &count) {
-  count.fill(0);
-  rank.resize(ch.size());
-  for (size_t i = 0; i < ch.size(); ++i) {
-    int c = static_cast<int>(ch[i]) + 128;
-    rank[i] = static_cast<int32_t>(count[c]);
-    count[c] += 1;
+inline uint8_t signed_charge(uint8_t q, bool flip) {
+  return flip ? static_cast<uint8_t>(0u - q) : q;  // int8 negation wraps
+}
+
+// All run tuples of one side of a matrix view (legs in order, row-major over
+// the legs' run indices, last leg fastest).
+struct Side {
+  std::vector<int> legs;
+  std::vector<int64_t> nruns;
+  std::array<int64_t, 256> count{};  // index tuples per fused charge
+  int64_t ntuples = 1;
+  std::vector<uint8_t> q;       // per tuple: fused signed charge
+  std::vector<int64_t> base;    // per tuple: in-block position (dense: dense rank) of its first element
+  std::vector<int64_t> stride;  // per tuple and leg: position step per digit
+  std::vector<int64_t> runidx;  // per tuple and leg: run index
+};
+
+Side make_side(const std::vector<int> &legs, const std::vector<uint8_t> &flip,
+               const std::vector<std::vector<Run>> &runs, const int64_t *shape, bool dense) {
+  Side s;
+  s.legs = legs;
+  const size_t k = legs.size();
+  for (int l : legs) s.nruns.push_back(static_cast<int64_t>(runs[l].size()));
+  for (int64_t n : s.nruns) s.ntuples *= n;
+  // suffix counts S_p, p = k .. 0
+  std::vector<std::array<int64_t, 256>> S(k + 1);
+  S[k].fill(0);
+  S[k][0] = 1;
+  for (size_t p = k; p-- > 0;) {
+    std::array<int64_t, 256> M{};
+    for (const Run &r : runs[legs[p]]) M[signed_charge(r.q, flip[p])] += r.len;
+    S[p].fill(0);
+    for (int a = 0; a < 256; ++a)
+      if (M[a])
+        for (int x = 0; x < 256; ++x)
+          if (S[p + 1][x]) S[p][(a + x) & 255] += M[a] * S[p + 1][x];
+  }
+  s.count = S[0];
+  // per leg and run: signed charge counts of the states before the run
+  std::vector<std::vector<std::vector<std::pair<uint8_t, int64_t>>>> before(k);
+  for (size_t p = 0; p < k; ++p) {
+    std::array<int64_t, 256> seen{};
+    for (const Run &r : runs[legs[p]]) {
+      std::vector<std::pair<uint8_t, int64_t>> snap;
+      for (int a = 0; a < 256; ++a)
+        if (seen[a]) snap.push_back({static_cast<uint8_t>(a), seen[a]});
+      before[p].push_back(std::move(snap));
+      seen[signed_charge(r.q, flip[p])] += r.len;
+    }
+  }
+  std::vector<int64_t> dstride(k, 1);
+  for (size_t p = k; p-- > 1;) dstride[p - 1] = dstride[p] * shape[legs[p]];
+  s.q.resize(s.ntuples);
+  s.base.resize(s.ntuples);
+  s.stride.resize(s.ntuples * k);
+  s.runidx.resize(s.ntuples * k);
+  std::vector<int64_t> ri(k, 0);
+  for (int64_t t = 0; t < s.ntuples; ++t) {
+    int64_t rem = t;
+    for (size_t p = k; p-- > 0;) {
+      ri[p] = rem % s.nruns[p];
+      rem /= s.nruns[p];
+    }
+    unsigned Q = 0;
+    for (size_t p = 0; p < k; ++p) Q += signed_charge(runs[legs[p]][ri[p]].q, flip[p]);
+    s.q[t] = static_cast<uint8_t>(Q);
+    int64_t base = 0;
+    unsigned P = 0;
+    for (size_t p = 0; p < k; ++p) {
+      const Run &r = runs[legs[p]][ri[p]];
+      s.runidx[t * k + p] = ri[p];
+      if (dense) {
+        base += r.start * dstride[p];
+        s.stride[t * k + p] = dstride[p];
+        continue;
+      }
+      const uint8_t qp = signed_charge(r.q, flip[p]);
+      for (const auto &[qq, cnt] : before[p][ri[p]]) base += cnt * S[p + 1][(Q - P - qq) & 255];
+      s.stride[t * k + p] = S[p + 1][(Q - P - qp) & 255];
+      P += qp;
+    }
+    s.base[t] = base;
+  }
+  return s;
+}
+
+struct Digit {
+  int64_t len, s, d, l;  // length, source stride, destination stride, staging stride
+};
+
+uint32_t magic_of(int64_t len) {
+  return static_cast<uint32_t>(((uint64_t(1) << 32) + static_cast<uint64_t>(len) - 1) /
+                               static_cast<uint64_t>(len));
+}
+
+// Tiles of one box: digits given in destination (new leg) order. Slices the
+// slowest destination digits until a tile holds at most kPermuteTileVol
+// elements, then builds each tile's source-order and destination-order digit
+// lists with contiguous neighbours merged.
+void box_tiles(const std::vector<Digit> &dig0, int64_t src_base, int64_t dst_base, int64_t src_limit,
+               int64_t dst_limit, std::vector<PermTile> &out) {
+  std::vector<Digit> dig;
+  for (const Digit &g : dig0)
+    if (g.len > 1) dig.push_back(g);
+  int64_t vol = 1;
+  for (const Digit &g : dig) vol *= g.len;
+  // chunk of each digit along which tiles are cut (slowest destination digits first)
+  std::vector<int64_t> chunk(dig.size());
+  int64_t rest = vol;
+  bool done = rest <= kPermuteTileVol;
+  for (size_t i = 0; i < dig.size(); ++i) {
+    if (done) {
+      chunk[i] = dig[i].len;
+      continue;
+    }
+    rest /= dig[i].len;
+    if (rest <= kPermuteTileVol) {
+      chunk[i] = std::max<int64_t>(1, kPermuteTileVol / rest);
+      done = true;
+    } else {
+      chunk[i] = 1;
+    }
+  }
+  std::vector<int64_t> nchunks(dig.size()), at(dig.size(), 0);
+  int64_t ntiles = 1;
+  for (size_t i = 0; i < dig.size(); ++i) {
+    nchunks[i] = (dig[i].len + chunk[i] - 1) / chunk[i];
+    ntiles *= nchunks[i];
+  }
+  for (int64_t t = 0; t < ntiles; ++t) {
+    int64_t rem = t;
+    for (size_t i = dig.size(); i-- > 0;) {
+      at[i] = rem % nchunks[i];
+      rem /= nchunks[i];
+    }
+    std::vector<Digit> td = dig;
+    int64_t sb = src_base, db = dst_base, tvol = 1, slast = 0, dlast = 0;
+    for (size_t i = 0; i < td.size(); ++i) {
+      const int64_t st = at[i] * chunk[i];
+      td[i].len = std::min(chunk[i], dig[i].len - st);
+      sb += st * dig[i].s;
+      db += st * dig[i].d;
+      tvol *= td[i].len;
+      slast += (td[i].len - 1) * dig[i].s;
+      dlast += (td[i].len - 1) * dig[i].d;
+    }
+    if (src_base >= 0 && (sb < 0 || sb + slast >= src_limit))
+      throw std::invalid_argument("permute plan: tile reads outside the source");
+    if (db < 0 || db + dlast >= dst_limit)
+      throw std::invalid_argument("permute plan: tile writes outside the destination");
+    if (slast >= (int64_t(1) << 31) || dlast >= (int64_t(1) << 31))
+      throw std::invalid_argument("permute plan: block too large for 32-bit in-tile offsets");
+    // staging offsets: the tile in destination order, row-major
+    int64_t lrun = 1;
+    for (size_t i = td.size(); i-- > 0;) {
+      td[i].l = lrun;
+      lrun *= td[i].len;
+    }
+    std::vector<Digit> dd;  // destination digits, fastest first, contiguous ones merged
+    for (size_t i = td.size(); i-- > 0;) {
+      if (td[i].len == 1) continue;
+      if (!dd.empty() && td[i].d == dd.back().len * dd.back().d) dd.back().len *= td[i].len;
+      else dd.push_back(td[i]);
+    }
+    std::vector<Digit> sd;  // source digits by increasing source stride, merged in both orders
+    for (const Digit &g : td)
+      if (g.len > 1) sd.push_back(g);
+    std::stable_sort(sd.begin(), sd.end(), [](const Digit &a, const Digit &b) { return a.s < b.s; });
+    std::vector<Digit> sm;
+    for (const Digit &g : sd) {
+      if (!sm.empty() && g.s == sm.back().len * sm.back().s && g.l == sm.back().len * sm.back().l)
+        sm.back().len *= g.len;
+      else sm.push_back(g);
+    }
+    if (sm.size() > static_cast<size_t>(kPermuteMaxDigits) || dd.size() > static_cast<size_t>(kPermuteMaxDigits))
+      throw std::invalid_argument("permute plan: too many digits in a tile");
+    PermTile pt;
+    std::memset(&pt, 0, sizeof(pt));
+    pt.src_base = src_base >= 0 ? sb : -1;
+    pt.dst_base = db;
+    pt.vol = static_cast<int32_t>(tvol);
+    pt.ns = static_cast<int32_t>(sm.size());
+    pt.nd = static_cast<int32_t>(dd.size());
+    for (size_t k = 0; k < sm.size(); ++k) {
+      pt.slen[k] = static_cast<int32_t>(sm[k].len);
+      pt.smag[k] = magic_of(sm[k].len);
+      pt.sstr[k] = static_cast<int32_t>(sm[k].s);
+      pt.lstr[k] = static_cast<int32_t>(sm[k].l);
+    }
+    for (size_t k = 0; k < dd.size(); ++k) {
+      pt.dlen[k] = static_cast<int32_t>(dd[k].len);
+      pt.dmag[k] = magic_of(dd[k].len);
+      pt.dstr[k] = static_cast<int32_t>(dd[k].d);
+    }
+    out.push_back(pt);
   }
 }
+
+}  // namespace
 
 fs_permute_plan *build_abelian_permute_plan(
     int ndim, const int64_t *shape, const int8_t *const *leg_charges,
@@ -85,215 +291,159 @@ fs_permute_plan *build_abelian_permute_plan(
     const int8_t *old_block_irreps, const int64_t *old_block_offsets, int old_layout_f,
     int old_dense, const int32_t *axes, int new_nrr, int new_dense) {
   if (ndim < 2) throw std::invalid_argument("ndim must be >= 2");
-  if (old_nrr < 1 || old_nrr >= ndim)
-    throw std::invalid_argument("invalid old number of row legs");
-  if (new_nrr < 1 || new_nrr >= ndim)
-    throw std::invalid_argument("invalid new number of row legs");
+  if (old_nrr < 1 || old_nrr >= ndim) throw std::invalid_argument("invalid old number of row legs");
+  if (new_nrr < 1 || new_nrr >= ndim) throw std::invalid_argument("invalid new number of row legs");
   {
     std::vector<int> seen(ndim, 0);
     for (int i = 0; i < ndim; ++i) {
-      if (axes[i] < 0 || axes[i] >= ndim || seen[axes[i]])
-        throw std::invalid_argument("axes do not match tensor");
+      if (axes[i] < 0 || axes[i] >= ndim || seen[axes[i]]) throw std::invalid_argument("axes do not match tensor");
       seen[axes[i]] = 1;
     }
   }
   for (int l = 0; l < ndim; ++l)
     if (shape[l] < 1) throw std::invalid_argument("leg dimension must be >= 1");
 
-  auto plan = new fs_permute_plan();
-  try {
-    // ---- old side ------------------------------------------------------
-    std::vector<const int8_t *> rl(leg_charges, leg_charges + old_nrr);
-    std::vector<const int8_t *> cl(leg_charges + old_nrr, leg_charges + ndim);
-    std::vector<int64_t> rs(shape, shape + old_nrr), cs(shape + old_nrr, shape + ndim);
-    std::vector<uint8_t> rf(signature, signature + old_nrr);
-    std::vector<uint8_t> cf(ndim - old_nrr);
-    for (int l = old_nrr; l < ndim; ++l) cf[l - old_nrr] = signature[l] ? 0 : 1;
+  auto plan = std::make_unique<fs_permute_plan>();
+  std::vector<std::vector<Run>> runs(ndim);
+  for (int l = 0; l < ndim; ++l) runs[l] = leg_runs(leg_charges[l], shape[l]);
 
-    int64_t n_old_rows = 1, n_old_cols = 1;
-    for (auto s : rs) n_old_rows *= s;
-    for (auto s : cs) n_old_cols *= s;
-    if (n_old_rows > std::numeric_limits<int32_t>::max() ||
-        n_old_cols > std::numeric_limits<int32_t>::max())
-      throw std::invalid_argument("matrix dimension exceeds int32 index range");
-
-    plan->old_row_block.assign(n_old_rows, -1);
-    plan->old_row_pos.assign(n_old_rows, 0);
-    plan->old_col_block.assign(n_old_cols, -1);
-    plan->old_col_pos.assign(n_old_cols, 0);
-
-    if (old_dense) {
-      for (int64_t r = 0; r < n_old_rows; ++r) {
-        plan->old_row_block[r] = 0;
-        plan->old_row_pos[r] = static_cast<int32_t>(r);
-      }
-      for (int64_t c = 0; c < n_old_cols; ++c) {
-        plan->old_col_block[c] = 0;
-        plan->old_col_pos[c] = static_cast<int32_t>(c);
-      }
-      plan->old_off = {0};
-      plan->old_rs = {n_old_cols};
-      plan->old_cs = {1};
-      plan->old_ncoeff = n_old_rows * n_old_cols;
+  // sides: rows fuse with the signature, columns with its opposite
+  std::vector<int> orl, ocl, nrl, ncl;
+  std::vector<uint8_t> orf, ocf, nrf, ncf;
+  for (int l = 0; l < ndim; ++l) {
+    if (l < old_nrr) {
+      orl.push_back(l);
+      orf.push_back(signature[l] ? 1 : 0);
     } else {
-      auto rch = fuse_charges(rl, rs, rf);
-      auto cch = fuse_charges(cl, cs, cf);
-      std::array<int64_t, 256> rcount, ccount;
-      charge_ranks(rch, plan->old_row_pos, rcount);
-      charge_ranks(cch, plan->old_col_pos, ccount);
-      std::array<int32_t, 256> block_of;
-      block_of.fill(-1);
-      int prev = -1000;
-      int64_t off = 0;
-      for (int b = 0; b < old_nblocks; ++b) {
-        int c = static_cast<int>(old_block_irreps[b]);
-        if (c <= prev) throw std::invalid_argument("old block irreps must be sorted and unique");
-        prev = c;
-        int64_t nr = rcount[c + 128], nc = ccount[c + 128];
-        if (nr == 0 || nc == 0)
-          throw std::invalid_argument("old block irrep absent from row or column representation");
-        block_of[c + 128] = b;
-        plan->old_off.push_back(old_block_offsets ? old_block_offsets[b] : off);
-        if (old_layout_f) {
-          plan->old_rs.push_back(1);
-          plan->old_cs.push_back(nr);
-        } else {
-          plan->old_rs.push_back(nc);
-          plan->old_cs.push_back(1);
-        }
-        off += nr * nc;
-      }
-      plan->old_ncoeff = off;
-      for (int64_t r = 0; r < n_old_rows; ++r)
-        plan->old_row_block[r] = block_of[static_cast<int>(rch[r]) + 128];
-      for (int64_t c = 0; c < n_old_cols; ++c)
-        plan->old_col_block[c] = block_of[static_cast<int>(cch[c]) + 128];
+      ocl.push_back(l);
+      ocf.push_back(signature[l] ? 0 : 1);
     }
-
-    // old dense strides of each leg: row legs within the row index, column
-    // legs within the column index (row-major as in abelian:175-183)
-    std::vector<int64_t> stride(ndim, 1);
-    for (int l = old_nrr - 2; l >= 0; --l) stride[l] = stride[l + 1] * shape[l + 1];
-    for (int l = ndim - 2; l >= old_nrr; --l) stride[l] = stride[l + 1] * shape[l + 1];
-
-    // ---- new side ------------------------------------------------------
-    std::vector<const int8_t *> nrl, ncl;
-    std::vector<int64_t> nrs, ncs;
-    std::vector<uint8_t> nrf, ncf;
-    // contributions of each new leg to (old row index, old col index)
-    std::vector<std::vector<int64_t>> row_partR, row_partC, col_partR, col_partC;
-    for (int i = 0; i < ndim; ++i) {
-      int L = axes[i];
-      std::vector<int64_t> pR(shape[L], 0), pC(shape[L], 0);
-      for (int64_t t = 0; t < shape[L]; ++t) {
-        if (L < old_nrr) pR[t] = t * stride[L];
-        else pC[t] = t * stride[L];
-      }
-      if (i < new_nrr) {
-        nrl.push_back(leg_charges[L]);
-        nrs.push_back(shape[L]);
-        nrf.push_back(signature[L] ? 1 : 0);
-        row_partR.push_back(std::move(pR));
-        row_partC.push_back(std::move(pC));
-      } else {
-        ncl.push_back(leg_charges[L]);
-        ncs.push_back(shape[L]);
-        ncf.push_back(signature[L] ? 0 : 1);
-        col_partR.push_back(std::move(pR));
-        col_partC.push_back(std::move(pC));
-      }
-    }
-    auto aR = kron_sum_i64(row_partR);
-    auto aC = kron_sum_i64(row_partC);
-    auto bR = kron_sum_i64(col_partR);
-    auto bC = kron_sum_i64(col_partC);
-    const int64_t n_new_rows = static_cast<int64_t>(aR.size());
-    const int64_t n_new_cols = static_cast<int64_t>(bR.size());
-    if (n_new_rows > std::numeric_limits<int32_t>::max() ||
-        n_new_cols > std::numeric_limits<int32_t>::max())
-      throw std::invalid_argument("matrix dimension exceeds int32 index range");
-
-    std::vector<std::vector<int64_t>> rows_of, cols_of;  // per new block
-    if (new_dense) {
-      plan->new_irreps = {0};
-      rows_of.resize(1);
-      cols_of.resize(1);
-      rows_of[0].resize(n_new_rows);
-      cols_of[0].resize(n_new_cols);
-      for (int64_t r = 0; r < n_new_rows; ++r) rows_of[0][r] = r;
-      for (int64_t c = 0; c < n_new_cols; ++c) cols_of[0][c] = c;
-    } else {
-      auto rch = fuse_charges(nrl, nrs, nrf);
-      auto cch = fuse_charges(ncl, ncs, ncf);
-      std::array<int64_t, 256> rcount{}, ccount{};
-      for (auto c : rch) rcount[static_cast<int>(c) + 128]++;
-      for (auto c : cch) ccount[static_cast<int>(c) + 128]++;
-      std::array<int32_t, 256> block_of;
-      block_of.fill(-1);
-      for (int c = 0; c < 256; ++c) {
-        if (rcount[c] > 0 && ccount[c] > 0) {
-          block_of[c] = static_cast<int32_t>(plan->new_irreps.size());
-          plan->new_irreps.push_back(static_cast<int8_t>(c - 128));
-        }
-      }
-      rows_of.resize(plan->new_irreps.size());
-      cols_of.resize(plan->new_irreps.size());
-      for (int64_t r = 0; r < n_new_rows; ++r) {
-        int b = block_of[static_cast<int>(rch[r]) + 128];
-        if (b >= 0) rows_of[b].push_back(r);
-      }
-      for (int64_t c = 0; c < n_new_cols; ++c) {
-        int b = block_of[static_cast<int>(cch[c]) + 128];
-        if (b >= 0) cols_of[b].push_back(c);
-      }
-    }
-
-    const size_t nb = plan->new_irreps.size();
-    int64_t off = 0, rstart = 0, cstart = 0;
-    for (size_t b = 0; b < nb; ++b) {
-      const int64_t nr = static_cast<int64_t>(rows_of[b].size());
-      const int64_t nc = static_cast<int64_t>(cols_of[b].size());
-      plan->new_shapes.push_back(nr);
-      plan->new_shapes.push_back(nc);
-      plan->new_off.push_back(off);
-      plan->new_rowstart.push_back(rstart);
-      plan->new_colstart.push_back(cstart);
-      for (auto r : rows_of[b]) {
-        plan->row_a.push_back(static_cast<int32_t>(aR[r]));
-        plan->row_a.push_back(static_cast<int32_t>(aC[r]));
-      }
-      for (auto c : cols_of[b]) {
-        plan->col_b.push_back(static_cast<int32_t>(bR[c]));
-        plan->col_b.push_back(static_cast<int32_t>(bC[c]));
-      }
-      off += nr * nc;
-      rstart += nr;
-      cstart += nc;
-    }
-    plan->new_ncoeff = off;
-    plan->n_old_rows = n_old_rows;
-    plan->n_old_cols = n_old_cols;
-    plan->n_new_rows_total = rstart;
-    plan->n_new_cols_total = cstart;
-    plan->finalize_tiles();
-  } catch (...) {
-    delete plan;
-    throw;
   }
-  return plan;
+  for (int i = 0; i < ndim; ++i) {
+    const int L = axes[i];
+    if (i < new_nrr) {
+      nrl.push_back(L);
+      nrf.push_back(signature[L] ? 1 : 0);
+    } else {
+      ncl.push_back(L);
+      ncf.push_back(signature[L] ? 0 : 1);
+    }
+  }
+  const Side orow = make_side(orl, orf, runs, shape, old_dense != 0);
+  const Side ocol = make_side(ocl, ocf, runs, shape, old_dense != 0);
+  const Side nrow = make_side(nrl, nrf, runs, shape, new_dense != 0);
+  const Side ncol = make_side(ncl, ncf, runs, shape, new_dense != 0);
+
+  // old blocks: offset and (row, column) strides per charge
+  std::array<int64_t, 256> o_off{}, o_rs{}, o_cs{};
+  std::array<bool, 256> o_present{};
+  int64_t src_limit = 0;
+  if (old_dense) {
+    int64_t nr = 1, nc = 1;
+    for (int l : orl) nr *= shape[l];
+    for (int l : ocl) nc *= shape[l];
+    o_present.fill(true);
+    o_off.fill(0);
+    o_rs.fill(nc);
+    o_cs.fill(1);
+    src_limit = nr * nc;
+    plan->old_ncoeff = nr * nc;
+  } else {
+    int prev = -1000;
+    int64_t off = 0;
+    for (int b = 0; b < old_nblocks; ++b) {
+      const int c = static_cast<int>(old_block_irreps[b]);
+      if (c <= prev) throw std::invalid_argument("old block irreps must be sorted and unique");
+      prev = c;
+      const uint8_t u = static_cast<uint8_t>(old_block_irreps[b]);
+      const int64_t nr = orow.count[u], nc = ocol.count[u];
+      if (nr == 0 || nc == 0) throw std::invalid_argument("old block irrep absent from row or column representation");
+      o_present[u] = true;
+      o_off[u] = old_block_offsets ? old_block_offsets[b] : off;
+      if (o_off[u] < 0) throw std::invalid_argument("negative old block offset");
+      o_rs[u] = old_layout_f ? 1 : nc;
+      o_cs[u] = old_layout_f ? nr : 1;
+      src_limit = std::max(src_limit, o_off[u] + nr * nc);
+      off += nr * nc;
+    }
+    plan->old_ncoeff = old_block_offsets ? src_limit : off;
+  }
+
+  // new blocks: charges present on both new sides, ascending as int8
+  std::array<int64_t, 256> n_off{}, n_cols{};
+  std::array<bool, 256> n_present{};
+  int64_t noff = 0;
+  if (new_dense) {
+    int64_t nr = 1, nc = 1;
+    for (int l : nrl) nr *= shape[l];
+    for (int l : ncl) nc *= shape[l];
+    plan->new_irreps = {0};
+    plan->new_shapes = {nr, nc};
+    n_present.fill(true);
+    n_cols.fill(nc);
+    noff = nr * nc;
+  } else {
+    for (int c = -128; c < 128; ++c) {
+      const uint8_t u = static_cast<uint8_t>(c);
+      if (nrow.count[u] > 0 && ncol.count[u] > 0) {
+        plan->new_irreps.push_back(static_cast<int8_t>(c));
+        plan->new_shapes.push_back(nrow.count[u]);
+        plan->new_shapes.push_back(ncol.count[u]);
+        n_present[u] = true;
+        n_off[u] = noff;
+        n_cols[u] = ncol.count[u];
+        noff += nrow.count[u] * ncol.count[u];
+      }
+    }
+  }
+  plan->new_ncoeff = noff;
+
+  // old column tuples by charge
+  std::vector<std::vector<int64_t>> by_q(256);
+  for (int64_t b = 0; b < ocol.ntuples; ++b) by_q[ocol.q[b]].push_back(b);
+  std::vector<int64_t> all_cols(static_cast<size_t>(ocol.ntuples));
+  for (int64_t b = 0; b < ocol.ntuples; ++b) all_cols[b] = b;
+  const bool all_pairs = old_dense || new_dense;
+
+  std::vector<int64_t> run_of(ndim), pos_old(ndim);
+  for (int l = 0; l < ndim; ++l) pos_old[l] = l < old_nrr ? l : l - old_nrr;
+  std::vector<Digit> dig(ndim);
+  const size_t kr = orl.size(), kc = ocl.size(), knr = nrl.size(), knc = ncl.size();
+  for (int64_t a = 0; a < orow.ntuples; ++a) {
+    const std::vector<int64_t> &cols = all_pairs ? all_cols : by_q[orow.q[a]];
+    for (int l = 0; l < static_cast<int>(kr); ++l) run_of[l] = orow.runidx[a * kr + l];
+    for (int64_t b : cols) {
+      for (size_t l = 0; l < kc; ++l) run_of[old_nrr + l] = ocol.runidx[b * kc + l];
+      int64_t ir = 0, ic = 0;
+      for (size_t i = 0; i < knr; ++i) ir = ir * nrow.nruns[i] + run_of[nrl[i]];
+      for (size_t i = 0; i < knc; ++i) ic = ic * ncol.nruns[i] + run_of[ncl[i]];
+      const uint8_t qn = nrow.q[ir];
+      if (!new_dense && (ncol.q[ic] != qn || !n_present[qn])) continue;  // not stored
+      const uint8_t qo = orow.q[a];
+      const bool readable = old_dense || (orow.q[a] == ocol.q[b] && o_present[qo]);
+      int64_t src_base = -1;
+      if (readable) src_base = o_off[qo] + orow.base[a] * o_rs[qo] + ocol.base[b] * o_cs[qo];
+      const uint8_t qd = new_dense ? 0 : qn;
+      const int64_t dst_base = n_off[qd] + nrow.base[ir] * n_cols[qd] + ncol.base[ic];
+      for (int i = 0; i < ndim; ++i) {
+        const int L = axes[i];
+        Digit &g = dig[i];
+        g.len = runs[L][run_of[L]].len;
+        if (!readable) g.s = 0;  // zero tile: nothing is read
+        else if (L < old_nrr) g.s = orow.stride[a * kr + pos_old[L]] * o_rs[qo];
+        else g.s = ocol.stride[b * kc + pos_old[L]] * o_cs[qo];
+        if (i < new_nrr) g.d = nrow.stride[ir * knr + i] * n_cols[qd];
+        else g.d = ncol.stride[ic * knc + (i - new_nrr)];
+        g.l = 0;
+      }
+      box_tiles(dig, src_base, dst_base, src_limit, std::max<int64_t>(noff, 1), plan->tiles);
+      ++plan->nboxes;
+    }
+  }
+  return plan.release();
 }
 
 } // namespace fs
-
-void fs_permute_plan::finalize_tiles() {
-  tile_prefix.assign(1, 0);
-  const size_t nb = new_irreps.size();
-  for (size_t b = 0; b < nb; ++b) {
-    int64_t n = new_shapes[2 * b] * new_shapes[2 * b + 1];
-    tile_prefix.push_back(tile_prefix.back() + (n + kPermuteTile - 1) / kPermuteTile);
-  }
-}
-
 
 namespace {
 
@@ -431,35 +581,34 @@ int fs_permute_plan_blocks(const fs_permute_plan *plan, int8_t *irreps,
   });
 }
 
-int fs_permute_plan_tables(const fs_permute_plan *plan, int64_t *sizes,
-                           int32_t *old_row_block, int32_t *old_row_pos,
-                           int32_t *old_col_block, int32_t *old_col_pos,
-                           int32_t *row_a, int32_t *col_b, int64_t *old_off,
-                           int64_t *old_rs, int64_t *old_cs) {
+int fs_permute_plan_tiles(const fs_permute_plan *plan, int64_t *ntiles, int64_t *nboxes,
+                          int64_t *out) {
   return fs::guarded([&]() -> int {
-    FS_CHECK_ARG(plan, "fs_permute_plan_tables: null plan");
-    if (sizes) {
-      sizes[0] = plan->n_old_rows;
-      sizes[1] = plan->n_old_cols;
-      sizes[2] = plan->n_new_rows_total;
-      sizes[3] = plan->n_new_cols_total;
-      sizes[4] = static_cast<int64_t>(plan->old_off.size());
+    FS_CHECK_ARG(plan && ntiles && nboxes, "fs_permute_plan_tiles: null argument");
+    *ntiles = static_cast<int64_t>(plan->tiles.size());
+    *nboxes = plan->nboxes;
+    if (!out) return FS_OK;
+    constexpr int K = kPermuteMaxDigits;
+    static_assert(FS_PERMUTE_TILE_RECORD == 5 + 5 * K, "tile record layout");
+    for (const PermTile &t : plan->tiles) {
+      int64_t *r = out;
+      std::memset(r, 0, FS_PERMUTE_TILE_RECORD * sizeof(int64_t));
+      r[0] = t.src_base;
+      r[1] = t.dst_base;
+      r[2] = t.vol;
+      r[3] = t.ns;
+      r[4] = t.nd;
+      for (int k = 0; k < t.ns; ++k) {
+        r[5 + 3 * k] = t.slen[k];
+        r[6 + 3 * k] = t.sstr[k];
+        r[7 + 3 * k] = t.lstr[k];
+      }
+      for (int k = 0; k < t.nd; ++k) {
+        r[5 + 3 * K + 2 * k] = t.dlen[k];
+        r[6 + 3 * K + 2 * k] = t.dstr[k];
+      }
+      out += FS_PERMUTE_TILE_RECORD;
     }
-    auto cp = [](int32_t *dst, const std::vector<int32_t> &v) {
-      if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(int32_t));
-    };
-    auto cp64 = [](int64_t *dst, const std::vector<int64_t> &v) {
-      if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(int64_t));
-    };
-    cp(old_row_block, plan->old_row_block);
-    cp(old_row_pos, plan->old_row_pos);
-    cp(old_col_block, plan->old_col_block);
-    cp(old_col_pos, plan->old_col_pos);
-    cp(row_a, plan->row_a);
-    cp(col_b, plan->col_b);
-    cp64(old_off, plan->old_off);
-    cp64(old_rs, plan->old_rs);
-    cp64(old_cs, plan->old_cs);
     return FS_OK;
   });
 }

@@ -110,19 +110,24 @@ int fs_permute_plan_blocks(const fs_permute_plan *plan, int8_t *irreps,
                            int64_t *shapes);
 
 /*
- * Host copies of the planner tables (test/inspection): sizes first, then data.
- * sizes[0..4] = n_old_rows, n_old_cols, n_new_rows_total, n_new_cols_total,
- * n_old_blocks. Any data pointer may be NULL. old_row_block/old_row_pos:
- * n_old_rows; old_col_block/old_col_pos: n_old_cols; row_a (2 per new
- * in-block row: old-row and old-col contributions), col_b (2 per new in-block
- * col); old_off/old_rs/old_cs: start, row stride and column stride of each
- * old block in the source buffer.
+ * The plan is a list of box tiles. A box holds the elements whose index on
+ * every leg lies in one run of equal charges of that leg; inside it the old
+ * and new in-block positions are affine in the legs' digits, so a tile (a box
+ * or a slice of one, at most 4096 elements) is a dense strided copy staged
+ * through LDS. Host copy of the tiles (test/inspection): *ntiles and *nboxes
+ * always; if out is not NULL it receives ntiles records of
+ * FS_PERMUTE_TILE_RECORD int64 each:
+ *   src_base (-1: absent old block, zeros), dst_base, vol, ns, nd,
+ *   ns x (len, src stride, staging stride) padded to FS_PERMUTE_MAX_DIGITS,
+ *   nd x (len, dst stride) padded to FS_PERMUTE_MAX_DIGITS;
+ * source digits fastest first (flat tile index f in source order: digit k is
+ * (f / prod of the faster lengths) mod len), staging offsets in destination
+ * order, destination digits fastest first likewise.
  */
-int fs_permute_plan_tables(const fs_permute_plan *plan, int64_t *sizes,
-                           int32_t *old_row_block, int32_t *old_row_pos,
-                           int32_t *old_col_block, int32_t *old_col_pos,
-                           int32_t *row_a, int32_t *col_b, int64_t *old_off,
-                           int64_t *old_rs, int64_t *old_cs);
+#define FS_PERMUTE_MAX_DIGITS 12
+#define FS_PERMUTE_TILE_RECORD (5 + 5 * FS_PERMUTE_MAX_DIGITS)
+int fs_permute_plan_tiles(const fs_permute_plan *plan, int64_t *ntiles, int64_t *nboxes,
+                          int64_t *out);
 
 /* Gather old blocks (packed, in block order) into the new packed blocks. */
 int fs_permute_plan_execute_f64(fs_permute_plan *plan, const double *src,

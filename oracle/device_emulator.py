@@ -24,34 +24,54 @@ def _np(t):
 
 
 def _permute_execute(plan, src, dst):
-    t = plan.tables()
+    """Box tiles of the plan (fs_permute_plan_tiles layout): every tile element f
+    is read at src_base + sum_k r_k sstr_k with r_k the source-order digits of f
+    and staged at sum_k r_k lstr_k; staged element g is written at dst_base +
+    sum_k r'_k dstr_k with r'_k the destination-order digits of g. Vectorized over
+    all tiles; every address is checked against the buffers first."""
+    K = _ops._lib.FS_PERMUTE_MAX_DIGITS
+    tiles, _ = plan.tiles()
     s = _np(src) if src is not None and src.numel() else np.zeros((0,))
     d = _np(dst)
-    shapes = plan.shapes
-    row_a = t["row_a"].reshape(-1, 2).astype(np.int64)
-    col_b = t["col_b"].reshape(-1, 2).astype(np.int64)
-    rstart = cstart = off = 0
-    for b in range(shapes.shape[0]):
-        nr, nc = (int(x) for x in shapes[b])
-        ra = row_a[rstart : rstart + nr]
-        cb = col_b[cstart : cstart + nc]
-        R = ra[:, 0][:, None] + cb[:, 0][None, :]
-        C = ra[:, 1][:, None] + cb[:, 1][None, :]
-        ob = t["old_row_block"][R]
-        ok = (ob >= 0) & (ob == t["old_col_block"][C])
-        obc = np.where(ok, ob, 0)
-        src_idx = (
-            t["old_off"][obc]
-            + t["old_row_pos"][R].astype(np.int64) * t["old_rs"][obc]
-            + t["old_col_pos"][C].astype(np.int64) * t["old_cs"][obc]
-        )
-        vals = np.zeros((nr, nc))
-        if s.size:
-            vals[ok] = s[src_idx[ok]]
-        d[off : off + nr * nc] = vals.reshape(-1)
-        rstart += nr
-        cstart += nc
-        off += nr * nc
+    if not tiles.shape[0]:
+        return
+    vol = tiles[:, 2]
+    tid = np.repeat(np.arange(tiles.shape[0]), vol)
+    start = np.zeros_like(vol)
+    start[1:] = np.cumsum(vol)[:-1]
+    f = np.arange(tid.size, dtype=np.int64) - start[tid]
+
+    def decode(f, lens, strides_list):
+        rem = f.copy()
+        offs = [np.zeros_like(f) for _ in strides_list]
+        for k in range(lens.shape[1]):
+            ln = np.maximum(lens[tid, k], 1)
+            active = lens[tid, k] > 0
+            r = np.where(active, rem % ln, 0)
+            rem = np.where(active, rem // ln, rem)
+            for o, st in zip(offs, strides_list, strict=True):
+                o += r * st[tid, k]
+        return offs
+
+    slen = tiles[:, 5 : 5 + 3 * K : 3]
+    sstr = tiles[:, 6 : 6 + 3 * K : 3]
+    lstr = tiles[:, 7 : 7 + 3 * K : 3]
+    dlen = tiles[:, 5 + 3 * K : 5 + 5 * K : 2]
+    dstr = tiles[:, 6 + 3 * K : 6 + 5 * K : 2]
+    so, lo = decode(f, slen, [sstr, lstr])
+    (do,) = decode(f, dlen, [dstr])
+    src_base = tiles[:, 0][tid]
+    dst_addr = tiles[:, 1][tid] + do
+    stage_addr = start[tid] + lo
+    assert (lo < vol[tid]).all() and (lo >= 0).all()
+    assert dst_addr.min() >= 0 and dst_addr.max() < d.size, "tile writes outside the destination"
+    staged = np.zeros((tid.size,))
+    live = src_base >= 0
+    src_addr = src_base[live] + so[live]
+    if src_addr.size:
+        assert src_addr.min() >= 0 and src_addr.max() < s.size, "tile reads outside the source"
+        staged[stage_addr[live]] = s[src_addr]
+    d[dst_addr] = staged[start[tid] + f]
 
 
 def _view(buf, off, rows, cols, rs, cs):
