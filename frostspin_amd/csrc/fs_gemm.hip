@@ -69,12 +69,12 @@ __device__ inline int xcd_logical_tile(int bid, int nblocks) {
 
 // Global -> register tile loads: the index walk follows whichever dimension of
 // the operand is contiguous so every wavefront load is coalesced.
-template <int BM, int BK, int PER>
+template <int BM, int BK, int PER, int NT = kThreads>
 __device__ inline void load_a(double (&r)[PER], const double *__restrict__ A, const GroupDev &G,
                               int m0, int k0, int kend, bool kcontig) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int idx = threadIdx.x + kThreads * q;
+    const int idx = threadIdx.x + NT * q;
     const int i = kcontig ? idx / BK : idx % BM;
     const int kk = kcontig ? idx % BK : idx / BM;
     const int gi = m0 + i, gk = k0 + kk;
@@ -84,12 +84,12 @@ __device__ inline void load_a(double (&r)[PER], const double *__restrict__ A, co
   }
 }
 
-template <int BN, int BK, int PER>
+template <int BN, int BK, int PER, int NT = kThreads>
 __device__ inline void load_b(double (&r)[PER], const double *__restrict__ B, const GroupDev &G,
                               int n0, int k0, int kend, bool ncontig) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
-    const int idx = threadIdx.x + kThreads * q;
+    const int idx = threadIdx.x + NT * q;
     const int j = ncontig ? idx % BN : idx / BK;
     const int kk = ncontig ? idx / BN : idx % BK;
     const int gj = n0 + j, gk = k0 + kk;
@@ -118,14 +118,17 @@ __device__ inline void load_b(double (&r)[PER], const double *__restrict__ B, co
 // (3 otherwise): 35.7 vs 32.5 TFLOP/s on the projector products of a SU(2)
 // chi=256 move, 51 vs 45 on 4096^3. 32-deep K panels (2 workgroups per CU)
 // measured slower. The other shapes keep the compiler's choice.
-template <int BM, int BN, int BK>
+// A 128 x 128 tile with 8 waves (2 x 4, 16 instead of 8 flop per byte of
+// L2 traffic) measured slower on every shape (45 vs 51 TFLOP/s at 4096^3).
+template <int BM, int BN, int BK, int WGM, int WGN>
 constexpr int gemm_waves_per_eu() {
-  return (BM == 64 && BN == 64 && BK == 16) ? 4 : 1;
+  return (BM == 64 && BN == 64 && BK == 16 && WGM * WGN == 4) ? 4 : 1;
 }
 
-template <int BM, int BN, int BK>
-__global__ void __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK>()))) gemm_f64_kernel(Batch bt) {
+template <int BM, int BN, int BK, int WGM = 2, int WGN = 2>
+__global__ void __launch_bounds__(64 * WGM * WGN)
+__attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK, WGM, WGN>()))) gemm_f64_kernel(Batch bt) {
+  constexpr int NT = 64 * WGM * WGN;
   constexpr int PAD = 16;
   constexpr int A_SZ = (BK * (BM + PAD) > BM * (BK + 1)) ? BK * (BM + PAD) : BM * (BK + 1);
   constexpr int B_SZ = (BK * (BN + PAD) > BN * (BK + 1)) ? BK * (BN + PAD) : BN * (BK + 1);
@@ -145,9 +148,9 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK>()))) gemm_f64_k
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int TI = WM / 16, TJ = WN / 16;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WGN, wn = wave % WGN;
 
   d4 acc[TI][TJ];
 #pragma unroll
@@ -157,8 +160,8 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK>()))) gemm_f64_k
 
   const bool a_kcontig = (G.a_cs == 1);
   const bool b_ncontig = (G.b_cs == 1);
-  constexpr int A_PER = BM * BK / kThreads;
-  constexpr int B_PER = BN * BK / kThreads;
+  constexpr int A_PER = BM * BK / NT;
+  constexpr int B_PER = BN * BK / NT;
   double ra[A_PER], rb[B_PER];
   // LDS element strides (k, i) of A and (k, j) of B for the chosen layouts
   const int a_ks = a_kcontig ? 1 : BM + PAD, a_is = a_kcontig ? BK + 1 : 1;
@@ -167,14 +170,14 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK>()))) gemm_f64_k
   auto store = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
-      const int idx = threadIdx.x + kThreads * q;
+      const int idx = threadIdx.x + NT * q;
       const int i = a_kcontig ? idx / BK : idx % BM;
       const int kk = a_kcontig ? idx % BK : idx / BM;
       As[buf][kk * a_ks + i * a_is] = ra[q];
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
-      const int idx = threadIdx.x + kThreads * q;
+      const int idx = threadIdx.x + NT * q;
       const int j = b_ncontig ? idx % BN : idx / BK;
       const int kk = b_ncontig ? idx / BN : idx % BK;
       Bs[buf][kk * b_ks + j * b_js] = rb[q];
@@ -184,16 +187,16 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK>()))) gemm_f64_k
   // split-K: blockIdx.y selects the K range [kb, ke) of this workgroup
   const int kb = static_cast<int>(blockIdx.y) * G.kc;
   const int ke = min(G.k, kb + G.kc);
-  load_a<BM, BK, A_PER>(ra, A, G, m0, kb, ke, a_kcontig);
-  load_b<BN, BK, B_PER>(rb, B, G, n0, kb, ke, b_ncontig);
+  load_a<BM, BK, A_PER, NT>(ra, A, G, m0, kb, ke, a_kcontig);
+  load_b<BN, BK, B_PER, NT>(rb, B, G, n0, kb, ke, b_ncontig);
   store(0);
   __syncthreads();
   int cur = 0;
   for (int k0 = kb; k0 < ke; k0 += BK) {
     const bool next = k0 + BK < ke;
     if (next) {
-      load_a<BM, BK, A_PER>(ra, A, G, m0, k0 + BK, ke, a_kcontig);
-      load_b<BN, BK, B_PER>(rb, B, G, n0, k0 + BK, ke, b_ncontig);
+      load_a<BM, BK, A_PER, NT>(ra, A, G, m0, k0 + BK, ke, a_kcontig);
+      load_b<BN, BK, B_PER, NT>(rb, B, G, n0, k0 + BK, ke, b_ncontig);
     }
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {

@@ -18,7 +18,11 @@
 
 namespace {
 
-constexpr int kPermThreads = 256;
+constexpr int kPermThreads = 512;
+
+// Elements per thread handled together: their loads are issued back to back
+// before any of them is stored, so each lane keeps several reads in flight.
+constexpr int kPermBatch = 4;
 
 template <typename T>
 __global__ void __launch_bounds__(kPermThreads)
@@ -30,20 +34,33 @@ permute_box_kernel(const T *__restrict__ src, T *__restrict__ dst, const PermTil
   if (t.src_base >= 0) {
     const T *in = src + t.src_base;
     const int ns = t.ns;
-    for (int f = threadIdx.x; f < vol; f += kPermThreads) {
-      uint32_t rem = static_cast<uint32_t>(f);
-      int32_t so = 0, lo = 0;
+    for (int f0 = threadIdx.x; f0 < vol; f0 += kPermThreads * kPermBatch) {
+      int32_t so[kPermBatch], lo[kPermBatch];
+      T v[kPermBatch];
 #pragma unroll
-      for (int k = 0; k < kPermuteMaxDigits; ++k) {
-        if (k < ns) {
-          const uint32_t q = __umulhi(rem, t.smag[k]);
-          const int32_t r = static_cast<int32_t>(rem - q * static_cast<uint32_t>(t.slen[k]));
-          so += r * t.sstr[k];
-          lo += r * t.lstr[k];
-          rem = q;
+      for (int u = 0; u < kPermBatch; ++u) {
+        const int f = f0 + u * kPermThreads;
+        uint32_t rem = static_cast<uint32_t>(f < vol ? f : 0);
+        int32_t a = 0, b = 0;
+#pragma unroll
+        for (int k = 0; k < kPermuteMaxDigits; ++k) {
+          if (k < ns) {
+            const uint32_t q = __umulhi(rem, t.smag[k]);
+            const int32_t r = static_cast<int32_t>(rem - q * static_cast<uint32_t>(t.slen[k]));
+            a += r * t.sstr[k];
+            b += r * t.lstr[k];
+            rem = q;
+          }
         }
+        so[u] = a;
+        lo[u] = b;
       }
-      buf[lo] = in[so];
+#pragma unroll
+      for (int u = 0; u < kPermBatch; ++u)
+        if (f0 + u * kPermThreads < vol) v[u] = in[so[u]];
+#pragma unroll
+      for (int u = 0; u < kPermBatch; ++u)
+        if (f0 + u * kPermThreads < vol) buf[lo[u]] = v[u];
     }
   } else {
     for (int f = threadIdx.x; f < vol; f += kPermThreads) buf[f] = T(0);

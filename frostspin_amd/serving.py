@@ -21,7 +21,10 @@ couplings and chi are served together:
   (first come, first served among the requests that fit); requests that
   arrive while others are on the GPU join as soon as that holds;
 * every request records its arrival, start and finish times, so latency
-  percentiles and throughput come out of the same run.
+  percentiles and throughput come out of the same run;
+* in process mode the workers share the irreps-only permute structures they
+  build through one directory (plan_store): a pattern computed by one slot is
+  loaded by the others.
 
 Each request's launches are ordered on its own stream and share no mutable
 device data with other requests, so its result does not depend on what else
@@ -31,6 +34,8 @@ is in flight (checked by the tests: alone and in the mix agree).
 import dataclasses
 import multiprocessing as mp
 import queue
+import shutil
+import tempfile
 import threading
 import time
 
@@ -125,13 +130,17 @@ def _summary(requests, results, wall, n_slots):
     }
 
 
-def _worker_main(slot, q_in, q_out, debug_checks):
-    """Slot process: serve requests from q_in until None arrives."""
+def _worker_main(slot, q_in, q_out, debug_checks, plan_dir=None):
+    """Slot process: serve requests from q_in until None arrives. With a
+    plan_dir, the irreps-only permute structures are shared with the other
+    slots through that directory."""
     import torch  # noqa: PLC0415 - imported in the worker process
 
+    from .symmetric_tensor import plan_store
     from .symmetric_tensor.tools import config
 
     config["debug_checks"] = debug_checks
+    plan_store.set_plan_dir(plan_dir)
     try:
         total = int(torch.cuda.get_device_properties(0).total_memory)
         q_out.put(("ready", slot, total, None, 0.0, 0.0))
@@ -146,7 +155,8 @@ def _worker_main(slot, q_in, q_out, debug_checks):
         try:
             its = [] if req.record_iters else None
             energy, chis = solve(req, its)
-            q_out.put(("done", slot, req.rid, (energy, chis, its or []), start, time.monotonic()))
+            q_out.put(("done", slot, req.rid, (energy, chis, its or [], plan_store.stats()), start,
+                       time.monotonic()))
         except Exception as err:  # noqa: BLE001 - reported to the engine
             q_out.put(("failed", slot, req.rid, repr(err), start, time.monotonic()))
 
@@ -155,7 +165,8 @@ class ServingEngine:
     """S slots over one GPU; see the module docstring. Use as a context manager
     (or call close()) in process mode so the workers exit."""
 
-    def __init__(self, n_slots=4, *, hbm_budget=None, mode="thread", start_timeout=600.0):
+    def __init__(self, n_slots=4, *, hbm_budget=None, mode="thread", start_timeout=600.0, plan_dir=None,
+                 share_plans=True):
         self.n_slots = int(n_slots)
         if self.n_slots < 1:
             raise ValueError("need at least one slot")
@@ -170,6 +181,18 @@ class ServingEngine:
         self._errors = []
         self._t0 = None
         self._workers = []
+        self._plan_tmp = None
+        self.worker_plan_stats = {}  # slot -> latest plan_store.stats() of that worker
+        if mode == "process" and share_plans and plan_dir is None:
+            # one directory for this engine's workers (and removed with it),
+            # unless the caller or FROSTSPIN_AMD_PLAN_DIR names a lasting one
+            from .symmetric_tensor import plan_store
+
+            plan_dir = plan_store.plan_dir()
+            if plan_dir is None:
+                self._plan_tmp = tempfile.mkdtemp(prefix="fs_plans_")
+                plan_dir = self._plan_tmp
+        self.plan_dir = plan_dir if share_plans else None
         if mode == "process":
             # workers are started before this process touches the GPU: they
             # report the device memory once their HIP context exists
@@ -179,7 +202,8 @@ class ServingEngine:
             self._q_out = ctx.Queue()
             self._q_in = [ctx.Queue() for _ in range(self.n_slots)]
             self._workers = [
-                ctx.Process(target=_worker_main, args=(i, self._q_in[i], self._q_out, config["debug_checks"]),
+                ctx.Process(target=_worker_main,
+                            args=(i, self._q_in[i], self._q_out, config["debug_checks"], self.plan_dir),
                             daemon=True)
                 for i in range(self.n_slots)
             ]
@@ -217,6 +241,9 @@ class ServingEngine:
             if w.is_alive():
                 w.terminate()
         self._workers = []
+        if self._plan_tmp is not None:
+            shutil.rmtree(self._plan_tmp, ignore_errors=True)
+            self._plan_tmp = None
 
     def __enter__(self):
         return self
@@ -330,12 +357,23 @@ class ServingEngine:
             self._free_slots.append(slot)
             if kind != "done":
                 raise RuntimeError(f"request {rid} failed: {payload}")
-            energy, chis, its = payload
+            energy, chis, its, pstats = payload
+            self.worker_plan_stats[slot] = pstats
             self._results[rid] = Result(rid, energy, chis, by_rid[rid].arrival, start - t0, finish - t0, slot,
                                         [t - t0 for t in its])
         wall = time.monotonic() - t0
         results = [self._results[r.rid] for r in requests]
-        return results, _summary(requests, results, wall, self.n_slots)
+        summary = _summary(requests, results, wall, self.n_slots)
+        stats = list(self.worker_plan_stats.values())
+        if stats:
+            summary["plan_structures"] = {
+                "shared_dir": self.plan_dir is not None,
+                "built": sum(x["built"] for x in stats),
+                "loaded": sum(x["loaded"] for x in stats),
+                "build_s_per_worker": float(np.mean([x["build_s"] for x in stats])),
+                "load_s_per_worker": float(np.mean([x["load_s"] for x in stats])),
+            }
+        return results, summary
 
 
 def overlapped_rate(results, skip):

@@ -20,12 +20,14 @@ numpy.
 """
 
 import threading
+import time
 from collections import OrderedDict
 from functools import reduce
 
 import numpy as np
 
 from .. import _lib, _ops
+from . import plan_store
 from .abelian_symmetric_tensor import _dense_source
 from .non_abelian_symmetric_tensor import NonAbelianSymmetricTensor
 
@@ -73,6 +75,23 @@ class _PermuteStructure:
         "in_t", "in_targets", "ir_in", "ir_out", "n_in", "n_out", "n_out_pad", "out_a",
         "out_b", "out_t", "out_targets", "sig_idx", "w_off", "pattern", "weights", "_keep",
     )
+
+    # arrays that define a structure (persisted by plan_store)
+    _ARRAYS = (
+        "W", "idc_in", "idc_out", "idr_in", "idr_out", "ic_in", "ic_out", "in_a", "in_b", "in_t",
+        "in_targets", "ir_in", "ir_out", "n_in", "n_out", "n_out_pad", "out_a", "out_b", "out_t",
+        "out_targets", "sig_idx", "w_off",
+    )
+
+    def to_arrays(self):
+        return {k: np.asarray(getattr(self, k)) for k in self._ARRAYS}
+
+    @classmethod
+    def from_arrays(cls, arrays):
+        st = cls()
+        for k in cls._ARRAYS:
+            setattr(st, k, arrays[k])
+        return st
 
     def bind(self, ndim, nrr, nrr_out, axes, nirr):
         """The C pattern (fs_lie_pattern) over this structure's arrays; the
@@ -204,6 +223,15 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         st = cls._permute_structures.get(key)
         if st is not None:
             return st
+        nper = np.array([len(ir) for ir in irreps_per_leg], dtype=np.int64)
+        skey = plan_store.pattern_key(cls.symmetry(), irreps_per_leg, signature, nrr, axes, nrr_out)
+        stored = plan_store.load(skey)
+        if stored is not None:
+            st = _PermuteStructure.from_arrays(stored)
+            st.bind(len(irreps_per_leg), nrr, nrr_out, axes, nper)
+            cls._permute_structures[key] = st
+            return st
+        t_build = time.perf_counter()
         ndim = len(irreps_per_leg)
         in_targets = cls._allowed_irreps(irreps_per_leg, signature, nrr)
         new_irreps = tuple(irreps_per_leg[a] for a in axes)
@@ -272,6 +300,8 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         st.out_t, st.out_a, st.out_b = eo
         st.bind(ndim, nrr, nrr_out, axes, nper)
         cls._permute_structures[key] = st
+        plan_store.note_built(time.perf_counter() - t_build)
+        plan_store.store(skey, st.to_arrays())
         return st
 
     def _block_strides(self):

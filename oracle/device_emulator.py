@@ -23,55 +23,62 @@ def _np(t):
     return t.numpy()
 
 
+def _mixed_digits(lens, vol):
+    """(vol, len(lens)) digits of 0..vol-1 in the mixed radix lens, fastest first."""
+    f = np.arange(vol, dtype=np.int64)
+    out = np.empty((vol, len(lens)), dtype=np.float64)
+    for k, ln in enumerate(lens):
+        out[:, k] = f % ln
+        f //= ln
+    return out
+
+
 def _permute_execute(plan, src, dst):
-    """Box tiles of the plan (fs_permute_plan_tiles layout): every tile element f
-    is read at src_base + sum_k r_k sstr_k with r_k the source-order digits of f
-    and staged at sum_k r_k lstr_k; staged element g is written at dst_base +
-    sum_k r'_k dstr_k with r'_k the destination-order digits of g. Vectorized over
-    all tiles; every address is checked against the buffers first."""
+    """Box tiles of the plan (fs_permute_plan_tiles layout): element f of a
+    tile (f in source order) is read at src_base + sum_k r_k sstr_k, r_k the
+    source-order digits of f, and staged at sum_k r_k lstr_k; staged element g
+    is written at dst_base + sum_k r'_k dstr_k, r'_k the destination-order
+    digits of g. Tiles of the same digit shape are done together: their
+    offsets are one (exact, < 2^53) float64 product digits @ strides. Every
+    address is checked against the buffers first."""
     K = _ops._lib.FS_PERMUTE_MAX_DIGITS
     tiles, _ = plan.tiles()
     s = _np(src) if src is not None and src.numel() else np.zeros((0,))
     d = _np(dst)
     if not tiles.shape[0]:
         return
-    vol = tiles[:, 2]
-    tid = np.repeat(np.arange(tiles.shape[0]), vol)
-    start = np.zeros_like(vol)
-    start[1:] = np.cumsum(vol)[:-1]
-    f = np.arange(tid.size, dtype=np.int64) - start[tid]
-
-    def decode(f, lens, strides_list):
-        rem = f.copy()
-        offs = [np.zeros_like(f) for _ in strides_list]
-        for k in range(lens.shape[1]):
-            ln = np.maximum(lens[tid, k], 1)
-            active = lens[tid, k] > 0
-            r = np.where(active, rem % ln, 0)
-            rem = np.where(active, rem // ln, rem)
-            for o, st in zip(offs, strides_list, strict=True):
-                o += r * st[tid, k]
-        return offs
-
     slen = tiles[:, 5 : 5 + 3 * K : 3]
-    sstr = tiles[:, 6 : 6 + 3 * K : 3]
-    lstr = tiles[:, 7 : 7 + 3 * K : 3]
+    sstr = tiles[:, 6 : 6 + 3 * K : 3].astype(np.float64)
+    lstr = tiles[:, 7 : 7 + 3 * K : 3].astype(np.float64)
     dlen = tiles[:, 5 + 3 * K : 5 + 5 * K : 2]
-    dstr = tiles[:, 6 + 3 * K : 6 + 5 * K : 2]
-    so, lo = decode(f, slen, [sstr, lstr])
-    (do,) = decode(f, dlen, [dstr])
-    src_base = tiles[:, 0][tid]
-    dst_addr = tiles[:, 1][tid] + do
-    stage_addr = start[tid] + lo
-    assert (lo < vol[tid]).all() and (lo >= 0).all()
-    assert dst_addr.min() >= 0 and dst_addr.max() < d.size, "tile writes outside the destination"
-    staged = np.zeros((tid.size,))
-    live = src_base >= 0
-    src_addr = src_base[live] + so[live]
-    if src_addr.size:
-        assert src_addr.min() >= 0 and src_addr.max() < s.size, "tile reads outside the source"
-        staged[stage_addr[live]] = s[src_addr]
-    d[dst_addr] = staged[start[tid] + f]
+    dstr = tiles[:, 6 + 3 * K : 6 + 5 * K : 2].astype(np.float64)
+    shapes, group = np.unique(np.concatenate([slen, dlen], axis=1), axis=0, return_inverse=True)
+    group = group.reshape(-1)
+    for g in range(shapes.shape[0]):
+        ids_all = np.flatnonzero(group == g)
+        ls = [int(x) for x in shapes[g, :K] if x > 0]
+        ld = [int(x) for x in shapes[g, K:] if x > 0]
+        vol = int(np.prod(ls)) if ls else 1
+        assert vol == (int(np.prod(ld)) if ld else 1) == int(tiles[ids_all[0], 2])
+        ns, nd = len(ls), len(ld)
+        Ds, Dd = _mixed_digits(ls, vol), _mixed_digits(ld, vol)
+        for c0 in range(0, ids_all.size, 2048):
+            ids = ids_all[c0 : c0 + 2048]
+            so = np.rint(Ds @ sstr[ids, :ns].T).astype(np.int64)  # (vol, T)
+            lo = np.rint(Ds @ lstr[ids, :ns].T).astype(np.int64)
+            do = np.rint(Dd @ dstr[ids, :nd].T).astype(np.int64)
+            assert lo.min() >= 0 and lo.max() < vol
+            live = tiles[ids, 0] >= 0
+            vals = np.zeros((vol, ids.size))
+            if live.any():
+                addr = tiles[ids[live], 0][None, :] + so[:, live]
+                assert addr.min() >= 0 and addr.max() < s.size, "tile reads outside the source"
+                vals[:, live] = s[addr]
+            staged = np.empty_like(vals)
+            staged[lo, np.arange(ids.size)[None, :]] = vals
+            daddr = tiles[ids, 1][None, :] + do
+            assert daddr.min() >= 0 and daddr.max() < d.size, "tile writes outside the destination"
+            d[daddr] = staged
 
 
 def _view(buf, off, rows, cols, rs, cs):

@@ -1,0 +1,223 @@
+"""The benchmark configurations at their stated sizes, on the GPU.
+
+* leg-28 microbench: rank-6 U(1) tensors with legs of charges -3..3 and
+  multiplicities round(8 exp(-q^2/4)) (dim 28), A (3|3) @ B (3|3) over 19
+  blocks up to 3584^2, then a seeded permute. The product is checked block by
+  block against the oracle's numpy products (1e-12 relative, the GEMM's
+  tolerance everywhere), the permute of the GPU product bit for bit against the
+  oracle's permute of the same coefficients. The FP32-split GEMM's error
+  against FP64 is bounded (3e-7 relative to the block's largest entry: FP32
+  accumulation inside K chunks of hi/lo products).
+* SU(2) Heisenberg, simple update tau = 0.01 to beta = 2 at D = 8, CTMRG at
+  chi = 128: from one checkpoint, one GPU iteration against one iteration of
+  the numpy restatement of every device kernel (energy and both correlation
+  lengths, 1e-10 relative), and the FP32-split contractions' error per
+  observable against the FP64 iteration (asserted below 1e-6; measured values
+  printed).
+* U(1) XXZ (Jz = 0.5, Jperp = 1) at D = 10, beta = 4, chi = 256, cutoff 1e-12
+  (with 1e-10 the spectrum ends at 253 values; 1e-12 makes chi = 256 bind):
+  size-independent checks of the rdms at full size, and one iteration against
+  the restatement at chi = 64 (energy, 1e-10 relative).
+* serving: 64 SU(2) requests with Poisson arrivals (mean 50 ms), beta in
+  [0.5, 4], chi in {32, 64, 128, 192, 256}, D in {6, 7, 8}, on 8 worker-process
+  slots; five sampled requests, the chi = 256 ones among them, solved alone in
+  this process give the same energies (1e-12).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _leg28():
+    q = np.arange(-3, 4)
+    mult = np.round(8 * np.exp(-(q**2) / 4)).astype(int)
+    return np.repeat(q, mult).astype(np.int8)
+
+
+def test_config_leg28_contract_permute(gpu):
+    from frostspin_amd import U1SymmetricTensor, _ops
+    from frostspin_amd.symmetric_tensor.symmetric_tensor import set_gemm_precision
+    from oracle.abelian import Tensor as OracleTensor
+
+    leg = _leg28()
+    assert leg.size == 28
+    sig = np.array([False, False, False, True, True, True])
+    irreps, shapes = U1SymmetricTensor.get_block_sizes((leg,) * 3, (leg,) * 3, sig)
+    assert len(irreps) == 19 and shapes.max() == 3584
+    offs = np.zeros(len(shapes), dtype=np.int64)
+    sizes = shapes[:, 0] * shapes[:, 1]
+    offs[1:] = np.cumsum(sizes)[:-1]
+
+    def make(seed):
+        data = _ops.empty(int(sizes.sum()))
+        _ops.fill_normal_(data, seed)
+        return U1SymmetricTensor._build((leg,) * 3, (leg,) * 3, data, shapes, offs, irreps, sig)
+
+    A, B = make(42), make(43)
+    perm = np.random.default_rng(42).permutation(6)
+    axes = (tuple(int(x) for x in perm[:3]), tuple(int(x) for x in perm[3:]))
+    C = A @ B
+    P = C.permute(*axes)
+    host = lambda t: [b.detach().cpu().numpy() for b in t.blocks]  # noqa: E731
+    a_h, b_h, c_h, p_h = host(A), host(B), host(C), host(P)
+    for a, b, c in zip(a_h, b_h, c_h, strict=True):
+        ref = a @ b
+        assert np.abs(c - ref).max() <= 1e-12 * np.abs(ref).max()
+    oc = OracleTensor((leg,) * 3, (leg,) * 3, c_h, irreps, sig)
+    op = oc.permute(*axes)
+    assert np.array_equal(np.asarray(P.block_irreps), op.block_irreps)
+    for got, ref in zip(p_h, op.blocks, strict=True):
+        assert got.shape == ref.shape and np.array_equal(got, ref)
+    set_gemm_precision("split")
+    try:
+        Cs = A @ B
+        worst = max(np.abs(s - c).max() / np.abs(c).max() for s, c in zip(host(Cs), c_h, strict=True))
+    finally:
+        set_gemm_precision("fp64")
+    print(f"leg-28 FP32-split GEMM: max error {worst:.2e} relative to the block's largest entry")
+    assert worst < 3e-7
+
+
+_T0 = [None]
+
+
+def _progress(msg):
+    import time
+
+    if _T0[0] is None:
+        _T0[0] = time.monotonic()
+    print(f"[configs {time.monotonic() - _T0[0]:7.1f}s] {msg}", flush=True)
+
+
+def _observables(ctm, h, with_xi=True):
+    from frostspin_amd.ctmrg import models
+
+    e = models.bond_energy(ctm, h)
+    if not with_xi:
+        return {"energy": e}
+    return {"energy": e, "xi_h": float(ctm.compute_corr_length_h(0)), "xi_v": float(ctm.compute_corr_length_v(0))}
+
+
+def _emulated_iteration(path, make_h, with_xi):
+    """One iteration of the checkpoint with every device kernel replaced by its
+    numpy/LAPACK restatement (the same host logic); the bond Hamiltonian is
+    built under the restatement too."""
+    from frostspin_amd.ctmrg import SequentialCTMRG
+    from frostspin_amd.symmetric_tensor.lie_group_symmetric_tensor import LieGroupSymmetricTensor
+    from oracle import device_emulator
+
+    undo = device_emulator.install_global()
+    saved = dict(LieGroupSymmetricTensor._plans)
+    LieGroupSymmetricTensor._plans.clear()
+    try:
+        host = SequentialCTMRG.load_from_file(path)
+        for move in ("up_move", "right_move", "down_move", "left_move"):  # = iterate()
+            getattr(host, move)()
+            _progress(f"restated {move} done")
+        return _observables(host, make_h(), with_xi)
+    finally:
+        LieGroupSymmetricTensor._plans.clear()
+        LieGroupSymmetricTensor._plans.update(saved)
+        undo()
+
+
+def _checkpoint(tmp_path, h, D, beta, chi, warm, cutoff=1e-10, tensors=None):
+    from frostspin_amd.ctmrg import SequentialCTMRG, models
+
+    if tensors is None:
+        _, tensors = models.thermal_tensors(h, D, beta)
+        _progress(f"simple update to beta={beta} at D={D} done")
+    ctm = SequentialCTMRG.from_elementary_tensors("AB\nBA", tensors, chi, cutoff=cutoff, degen_ratio=0.99)
+    for i in range(warm):
+        ctm.iterate()
+        _progress(f"warm-up iteration {i}: chi {ctm.chi_values}")
+    path = os.fspath(tmp_path / f"env_chi{chi}.npz")
+    ctm.save_to_file(path)
+    return ctm, path
+
+
+def test_config_su2_d8_chi128(gpu, tmp_path):
+    from frostspin_amd.ctmrg import SequentialCTMRG, models
+    from frostspin_amd.symmetric_tensor.symmetric_tensor import set_gemm_precision
+
+    h = models.heisenberg_bond("SU2")
+    ctm, path = _checkpoint(tmp_path, h, 8, 2.0, 128, warm=6)
+    assert max(ctm.chi_values) > 100
+    ctm.iterate()
+    gpu_obs = _observables(ctm, h)
+    _progress(f"SU(2) D=8 chi={ctm.chi_values}: GPU iteration done, {gpu_obs}")
+    emu_obs = _emulated_iteration(path, lambda: models.heisenberg_bond("SU2"), with_xi=True)
+    _progress(f"restated iteration done, {emu_obs}")
+    for k, v in gpu_obs.items():
+        assert abs(v - emu_obs[k]) <= 1e-10 * abs(v), (k, v, emu_obs[k])
+    set_gemm_precision("split")
+    try:
+        split = SequentialCTMRG.load_from_file(path)
+        split.iterate()
+        split_obs = _observables(split, h)
+    finally:
+        set_gemm_precision("fp64")
+    rel = {k: abs(split_obs[k] - v) / abs(v) for k, v in gpu_obs.items()}
+    print("SU(2) D=8 chi=128 FP32-split vs FP64, one iteration from one checkpoint:",
+          {k: f"{x:.2e}" for k, x in rel.items()})
+    assert all(x < 1e-6 for x in rel.values()), rel
+
+
+def test_config_u1_xxz_d10_chi256(gpu, tmp_path):
+    """Full size on the GPU (chi = 256 binds) with size-independent checks of
+    every first-neighbour rdm: Hermitian, unit trace, U(1) block structure
+    (charges of the two-site states conserved), bond energy inside the
+    spectrum of the XXZ bond; then one iteration at D = 10, chi = 64 against
+    the numpy restatement (energy 1e-10 relative), the size the restatement
+    finishes in seconds (at chi = 256 one restated move takes minutes)."""
+    from frostspin_amd.ctmrg import models
+
+    h = models.xxz_bond(0.5, 1.0)
+    hd = h.toarray().reshape(4, 4)
+    lo, hi = np.linalg.eigvalsh(hd)[[0, -1]]
+    _, tensors = models.thermal_tensors(h, 10, 4.0)
+    _progress("simple update to beta=4 at D=10 done")
+    ctm, _ = _checkpoint(tmp_path, h, 10, 4.0, 256, warm=5, cutoff=1e-12, tensors=tensors)
+    assert max(ctm.chi_values) >= 256, ctm.chi_values
+    ctm.iterate()
+    r12, r21 = ctm.compute_rdm_1st_neighbor_cell()
+    sz = np.array([1, -1])  # 2 S_z of the U(1) spin-1/2 leg
+    charge = (sz[:, None] + sz[None, :]).reshape(-1)
+    for r in r12 + r21:
+        m = r.toarray().reshape(4, 4)
+        assert np.abs(m - m.T).max() <= 1e-10 * np.abs(m).max()
+        assert abs(np.trace(m) - 1.0) < 1e-10
+        assert np.abs(m[charge[:, None] != charge[None, :]]).max() == 0.0
+        e = float(np.trace(m @ hd))
+        assert lo - 1e-12 <= e <= hi + 1e-12
+    _progress(f"U(1) D=10 chi={ctm.chi_values}: rdm checks done, energy {models.bond_energy(ctm, h)}")
+    del ctm, r12, r21
+    small, path = _checkpoint(tmp_path, h, 10, 4.0, 64, warm=3, cutoff=1e-12, tensors=tensors)
+    assert max(small.chi_values) >= 64
+    small.iterate()
+    gpu_obs = _observables(small, h, with_xi=False)
+    emu_obs = _emulated_iteration(path, lambda: models.xxz_bond(0.5, 1.0), with_xi=False)
+    assert abs(gpu_obs["energy"] - emu_obs["energy"]) <= 1e-10 * abs(gpu_obs["energy"])
+
+
+def test_config_serving_64_requests(gpu):
+    from frostspin_amd import serving
+
+    reqs = serving.poisson_requests(64, seed=0)
+    assert len(reqs) == 64 and {r.chi for r in reqs} == {32, 64, 128, 192, 256}
+    with serving.ServingEngine(8, mode="process", start_timeout=300) as engine:
+        mixed, summary = engine.run(reqs)
+    _progress(f"64 requests served: p50 {summary['latency_p50_s']:.2f} s, p99 {summary['latency_p99_s']:.2f} s")
+    assert summary["requests"] == 64
+    assert np.isfinite(summary["latency_p99_s"]) and summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
+    by_rid = {r.rid: r for r in mixed}
+    big = [r for r in reqs if r.chi == 256][:2]
+    sample = big + [r for r in reqs if r.chi != 256][:3]
+    for req in sample:
+        alone, _ = serving.ServingEngine(1).run([serving.Request(**{**req.__dict__, "arrival": 0.0})])
+        ref = alone[0].energy
+        assert abs(by_rid[req.rid].energy - ref) <= 1e-12 * max(1.0, abs(ref)), (req, ref)
