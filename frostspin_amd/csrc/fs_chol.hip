@@ -337,13 +337,41 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
   __shared__ double colbuf[2][kC4Blk][kCholInvMaxOrder];
   __shared__ double udiag[kCholInvMaxOrder / kC4Blk][kC4Blk * kC4Blk];
   __shared__ double red[kC4Threads / 64];
-  auto g_at = [&](int i, int k) {
-    double v = 0.0;
-    for (int p = 0; p < D.n_part; ++p) v += G[p * D.g_stride + static_cast<int64_t>(i) * l + k];
-    return v;
-  };
-  double tr = 0.0;
-  for (int i = threadIdx.x; i < l; i += kC4Threads) tr += g_at(i, i);
+  __shared__ double gdiag[kCholInvMaxOrder];
+  // upper triangle of G summed over its partials in partial order; all loads
+  // of one partial are issued before any is used (one memory round trip per
+  // partial, not one per element and partial)
+  double a[P][Q];
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) a[p][q] = 0.0;
+  for (int part = 0; part < D.n_part; ++part) {
+    const double *Gp = G + part * D.g_stride;
+    double v[P][Q];
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
+        v[p][q] = (i < l && c < l && i <= c) ? Gp[static_cast<int64_t>(i) * l + c] : 0.0;
+      }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) a[p][q] += v[p][q];
+  }
+  // trace for the shift: thread i adds G_ii (the diagonal reaches it through
+  // LDS), then a reduction over the workgroup
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = ty + kC4Rows * p;
+      if (i == tx + kC4Cols * q && i < l) gdiag[i] = a[p][q];
+    }
+  __syncthreads();
+  double tr = threadIdx.x < l ? gdiag[threadIdx.x] : 0.0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) tr += __shfl_xor(tr, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = tr;
@@ -352,16 +380,15 @@ __global__ void __launch_bounds__(kC4Threads) cholinv4_kernel(CholInvBatch bt) {
 #pragma unroll
   for (int w = 0; w < kC4Threads / 64; ++w) trace += red[w];
   const double shift = D.shift_rel * trace;
-
-  double a[P][Q];
 #pragma unroll
   for (int p = 0; p < P; ++p) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int i = ty + kC4Rows * p, c = tx + kC4Cols * q;
-      double v = (i == c) ? 1.0 : 0.0;  // identity padding beyond l
-      if (i < l && c < l) v = (i <= c) ? g_at(i, c) + (i == c ? shift : 0.0) : 0.0;
-      a[p][q] = v;
+      if (i < l && c < l)
+        a[p][q] = (i == c) ? a[p][q] + shift : a[p][q];
+      else
+        a[p][q] = (i == c) ? 1.0 : 0.0;  // identity padding beyond l
     }
   }
   const int lpad = (l + kC4Blk - 1) / kC4Blk * kC4Blk;

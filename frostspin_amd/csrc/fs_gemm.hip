@@ -469,23 +469,31 @@ constexpr int kBK = 16;
 // through L2. FS_GEMM_TILE=32/64/128 forces a size (tuning only).
 constexpr int kCUs = 256;
 
-// Split-K: a launch whose output tiles cannot fill the chip while its K is
-// long (projected corners: chi x chi outputs over chi D^2) runs `splits`
-// workgroups per tile over disjoint K ranges of at least kSplitMinK; partial
-// tiles go to a per-thread workspace and a second kernel sums them in split
-// order. FS_GEMM_SPLITK=0 disables it (tuning only).
+// Split-K: a launch whose output tiles leave most workgroup slots of the chip
+// empty while its K is long (projected corners: chi x chi outputs over chi D^2;
+// range-finder products: l x n outputs over m) runs `splits` workgroups per
+// tile over disjoint K ranges of at least kSplitMinK; partial tiles go to a
+// per-thread workspace and a second kernel sums them in split order. Each
+// workgroup keeps one K panel in flight, so on such launches the bytes in
+// flight, and with them the bandwidth reached, grow with the number of
+// resident workgroups: the split fills the resident slots (slots_per_cu
+// workgroups per CU for the tile size, set by its LDS footprint: 40 KB for
+// 64 x 64, 24 KB for 32 x 32). FS_GEMM_SPLITK=0 disables it (tuning only).
 constexpr int kSplitMinK = 256;
 constexpr int kMaxSplits = 16;
 
-inline int pick_splits(int64_t tiles, int64_t kmax) {
+inline int slots_per_cu(int tsz) { return tsz == 32 ? 6 : (tsz == 64 ? 4 : 1); }
+
+inline int pick_splits(int64_t tiles, int64_t kmax, int tsz) {
   static const bool off = [] {
     const char *e = std::getenv("FS_GEMM_SPLITK");
     return e && std::atoi(e) == 0;
   }();
-  if (off || tiles * 2 > kCUs || kmax < 2 * kSplitMinK) return 1;
-  const int64_t by_cus = fs::ceil_div(kCUs, tiles);
+  const int64_t slots = static_cast<int64_t>(kCUs) * slots_per_cu(tsz);
+  if (off || tiles * 2 > slots || kmax < 2 * kSplitMinK) return 1;
+  const int64_t by_slots = slots / tiles;
   const int64_t by_k = kmax / kSplitMinK;
-  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({by_cus, by_k, kMaxSplits})));
+  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({by_slots, by_k, kMaxSplits})));
 }
 
 // Split-K workspaces: one per (device, stream), grow-only, process-wide, so
@@ -698,7 +706,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     if (!SPLIT) {
       int64_t kmax = 0, elems = 0;
       for (int i = 0; i < ng; ++i) kmax = std::max<int64_t>(kmax, bt.g[i].k);
-      splits = pick_splits(tiles, kmax);
+      splits = pick_splits(tiles, kmax, tsz);
       if (splits > 1) {
         bt.elem_prefix[0] = 0;
         for (int i = 0; i < ng; ++i) {

@@ -121,6 +121,7 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
   extern __shared__ double lds[];
   __shared__ double sv[kMaxQ];
   __shared__ int rotated;
+  __shared__ double fro2_red[16];
   const SvdDev D = bt.d[blockIdx.x];
   const int m = D.m, n = D.n;
   const bool tr = m < n;  // work on A^T when A is wide
@@ -128,84 +129,124 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
   const int q = tr ? m : n;
   const int64_t pq = static_cast<int64_t>(p) * q;
   const bool in_lds = pq + static_cast<int64_t>(q) * q <= bt.lds_doubles;
-  double *X = in_lds ? lds : bt.work + D.work_off;  // p x q, column-major
-  double *V = X + pq;                                // q x q, column-major
-  const double *A = bt.A + D.a_off;
-  const int nthreads = blockDim.x;
-  const int nwaves = nthreads >> 6;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int group = tid >> 4;  // 16-lane pair groups
-  const int sub = tid & 15;
-  const int ngroups = nthreads >> 4;
+  // X (p x q) and V (q x q), column-major, in LDS or in the work buffer. The
+  // body is instantiated for each: through pointers derived from the LDS array
+  // the compiler emits LDS instructions; through a pointer that may address
+  // either it would emit flat accesses, which also wait on the memory counter.
+  auto body = [&](double *X, double *V) __attribute__((always_inline)) {
+    const double *A = bt.A + D.a_off;
+    const int nthreads = blockDim.x;
+    const int nwaves = nthreads >> 6;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int group = tid >> 4;  // 16-lane pair groups
+    const int sub = tid & 15;
+    const int ngroups = nthreads >> 4;
 
-  for (int64_t e = tid; e < pq; e += nthreads) {
-    const int col = static_cast<int>(e / p);
-    const int row = static_cast<int>(e - static_cast<int64_t>(col) * p);
-    const int i = tr ? col : row;  // row index in A
-    const int j = tr ? row : col;  // col index in A
-    X[e] = A[static_cast<int64_t>(i) * D.a_rs + static_cast<int64_t>(j) * D.a_cs];
-  }
-  for (int64_t e = tid; e < static_cast<int64_t>(q) * q; e += nthreads) {
-    const int col = static_cast<int>(e / q);
-    const int row = static_cast<int>(e - static_cast<int64_t>(col) * q);
-    V[e] = (row == col) ? 1.0 : 0.0;
-  }
-  __shared__ double fro2_red[16];
-  {
-    double f = 0.0;
-    for (int64_t e = tid; e < pq; e += nthreads) f += X[e] * X[e];
-    f = wave_sum(f);
-    if (lane == 0) fro2_red[wave] = f;
-  }
-  __syncthreads();
-  double fro2 = 0.0;
-  for (int w = 0; w < nwaves; ++w) fro2 += fro2_red[w];
-  // columns whose squared norm is below (eps ||A||_F)^2 are numerical zeros of a
-  // rank-deficient matrix: rotating them against each other only reshuffles
-  // rounding noise and would keep the sweep loop from terminating
-  const double negligible = fro2 * 4.930380657631324e-32;
-
-  const int Q = (q + 1) & ~1;  // even number of players (one dummy if q odd)
-  const int Qm = Q - 1;
-  // rotate only if |cos angle(x_i, x_j)| > sqrt(p) * eps (LAPACK dgesvj criterion);
-  // a smaller threshold sits below the rounding noise of the dot products and
-  // never converges
-  const double tol = sqrt(static_cast<double>(p)) * 2.220446049250313e-16;
-  const bool large = fro2 > 1e150;  // squared column norms could overflow when squared again
-  const bool regs = R > 0 && in_lds && p <= 16 * R && Q / 2 <= ngroups;
-  for (int sweep = 0; sweep < bt.max_sweeps && q > 1; ++sweep) {
-    if (tid == 0) rotated = 0;
+    for (int64_t e = tid; e < pq; e += nthreads) {
+      const int col = static_cast<int>(e / p);
+      const int row = static_cast<int>(e - static_cast<int64_t>(col) * p);
+      const int i = tr ? col : row;  // row index in A
+      const int j = tr ? row : col;  // col index in A
+      X[e] = A[static_cast<int64_t>(i) * D.a_rs + static_cast<int64_t>(j) * D.a_cs];
+    }
+    for (int64_t e = tid; e < static_cast<int64_t>(q) * q; e += nthreads) {
+      const int col = static_cast<int>(e / q);
+      const int row = static_cast<int>(e - static_cast<int64_t>(col) * q);
+      V[e] = (row == col) ? 1.0 : 0.0;
+    }
+    {
+      double f = 0.0;
+      for (int64_t e = tid; e < pq; e += nthreads) f += X[e] * X[e];
+      f = wave_sum(f);
+      if (lane == 0) fro2_red[wave] = f;
+    }
     __syncthreads();
-    int any = 0;
-    for (int r = 0; r < Qm; ++r) {
-      if (regs) {
-        if constexpr (R > 0) {
-          int i, j;
-          tournament_pair(r, group, Qm, i, j);
-          if (group < Q / 2 && i < q && j < q) {
+    double fro2 = 0.0;
+    for (int w = 0; w < nwaves; ++w) fro2 += fro2_red[w];
+    // columns whose squared norm is below (eps ||A||_F)^2 are numerical zeros of a
+    // rank-deficient matrix: rotating them against each other only reshuffles
+    // rounding noise and would keep the sweep loop from terminating
+    const double negligible = fro2 * 4.930380657631324e-32;
+
+    const int Q = (q + 1) & ~1;  // even number of players (one dummy if q odd)
+    const int Qm = Q - 1;
+    // rotate only if |cos angle(x_i, x_j)| > sqrt(p) * eps (LAPACK dgesvj criterion);
+    // a smaller threshold sits below the rounding noise of the dot products and
+    // never converges
+    const double tol = sqrt(static_cast<double>(p)) * 2.220446049250313e-16;
+    const bool large = fro2 > 1e150;  // squared column norms could overflow when squared again
+    const bool regs = R > 0 && in_lds && p <= 16 * R && Q / 2 <= ngroups;
+    for (int sweep = 0; sweep < bt.max_sweeps && q > 1; ++sweep) {
+      if (tid == 0) rotated = 0;
+      __syncthreads();
+      int any = 0;
+      for (int r = 0; r < Qm; ++r) {
+        if (regs) {
+          if constexpr (R > 0) {
+            int i, j;
+            tournament_pair(r, group, Qm, i, j);
+            if (group < Q / 2 && i < q && j < q) {
+              double *xi = X + static_cast<int64_t>(min(i, j)) * p;
+              double *xj = X + static_cast<int64_t>(max(i, j)) * p;
+              double *vi = V + static_cast<int64_t>(min(i, j)) * q;
+              double *vj = V + static_cast<int64_t>(max(i, j)) * q;
+              double u[R], v[R], vu[R], vv[R];
+              // V is read with X (before the decision) so its LDS latency
+              // overlaps the reductions
+  #pragma unroll
+              for (int e = 0; e < R; ++e) {
+                const int t = sub + 16 * e;
+                u[e] = t < p ? xi[t] : 0.0;
+                v[e] = t < p ? xj[t] : 0.0;
+                vu[e] = t < q ? vi[t] : 0.0;
+                vv[e] = t < q ? vj[t] : 0.0;
+              }
+              double a = 0.0, b = 0.0, g = 0.0;
+  #pragma unroll
+              for (int e = 0; e < R; ++e) {
+                a += u[e] * u[e];
+                b += v[e] * v[e];
+                g += u[e] * v[e];
+              }
+              a = group_sum16(a);
+              b = group_sum16(b);
+              g = group_sum16(g);
+              if (needs_rotation(a, b, g, negligible, tol, large)) {
+                double c, s;
+                jacobi_rotation(a, b, g, large, c, s);
+  #pragma unroll
+                for (int e = 0; e < R; ++e) {
+                  const int t = sub + 16 * e;
+                  if (t < p) {
+                    xi[t] = c * u[e] - s * v[e];
+                    xj[t] = s * u[e] + c * v[e];
+                  }
+                  if (t < q) {
+                    vi[t] = c * vu[e] - s * vv[e];
+                    vj[t] = s * vu[e] + c * vv[e];
+                  }
+                }
+                any = 1;
+              }
+            }
+          }
+        } else {
+          // each 16-lane group owns one pair at a time
+          for (int k = group; k < Q / 2; k += ngroups) {
+            int i, j;
+            tournament_pair(r, k, Qm, i, j);
+            if (i >= q || j >= q) continue;
             double *xi = X + static_cast<int64_t>(min(i, j)) * p;
             double *xj = X + static_cast<int64_t>(max(i, j)) * p;
-            double *vi = V + static_cast<int64_t>(min(i, j)) * q;
-            double *vj = V + static_cast<int64_t>(max(i, j)) * q;
-            double u[R], v[R], vu[R], vv[R];
-            // V is read with X (before the decision) so its LDS latency
-            // overlaps the reductions
-#pragma unroll
-            for (int e = 0; e < R; ++e) {
-              const int t = sub + 16 * e;
-              u[e] = t < p ? xi[t] : 0.0;
-              v[e] = t < p ? xj[t] : 0.0;
-              vu[e] = t < q ? vi[t] : 0.0;
-              vv[e] = t < q ? vj[t] : 0.0;
-            }
             double a = 0.0, b = 0.0, g = 0.0;
-#pragma unroll
-            for (int e = 0; e < R; ++e) {
-              a += u[e] * u[e];
-              b += v[e] * v[e];
-              g += u[e] * v[e];
+  #pragma unroll 4
+            for (int t = sub; t < p; t += 16) {
+              const double u = xi[t], v = xj[t];
+              a += u * u;
+              b += v * v;
+              g += u * v;
             }
             a = group_sum16(a);
             b = group_sum16(b);
@@ -213,105 +254,72 @@ __global__ void __launch_bounds__(1024) jacobi_svd_kernel(SvdBatch bt) {
             if (needs_rotation(a, b, g, negligible, tol, large)) {
               double c, s;
               jacobi_rotation(a, b, g, large, c, s);
-#pragma unroll
-              for (int e = 0; e < R; ++e) {
-                const int t = sub + 16 * e;
-                if (t < p) {
-                  xi[t] = c * u[e] - s * v[e];
-                  xj[t] = s * u[e] + c * v[e];
-                }
-                if (t < q) {
-                  vi[t] = c * vu[e] - s * vv[e];
-                  vj[t] = s * vu[e] + c * vv[e];
-                }
+  #pragma unroll 4
+              for (int t = sub; t < p; t += 16) {
+                const double u = xi[t], v = xj[t];
+                xi[t] = c * u - s * v;
+                xj[t] = s * u + c * v;
+              }
+              double *vi = V + static_cast<int64_t>(min(i, j)) * q;
+              double *vj = V + static_cast<int64_t>(max(i, j)) * q;
+  #pragma unroll 4
+              for (int t = sub; t < q; t += 16) {
+                const double u = vi[t], v = vj[t];
+                vi[t] = c * u - s * v;
+                vj[t] = s * u + c * v;
               }
               any = 1;
             }
           }
         }
-      } else {
-        // each 16-lane group owns one pair at a time
-        for (int k = group; k < Q / 2; k += ngroups) {
-          int i, j;
-          tournament_pair(r, k, Qm, i, j);
-          if (i >= q || j >= q) continue;
-          double *xi = X + static_cast<int64_t>(min(i, j)) * p;
-          double *xj = X + static_cast<int64_t>(max(i, j)) * p;
-          double a = 0.0, b = 0.0, g = 0.0;
-#pragma unroll 4
-          for (int t = sub; t < p; t += 16) {
-            const double u = xi[t], v = xj[t];
-            a += u * u;
-            b += v * v;
-            g += u * v;
-          }
-          a = group_sum16(a);
-          b = group_sum16(b);
-          g = group_sum16(g);
-          if (needs_rotation(a, b, g, negligible, tol, large)) {
-            double c, s;
-            jacobi_rotation(a, b, g, large, c, s);
-#pragma unroll 4
-            for (int t = sub; t < p; t += 16) {
-              const double u = xi[t], v = xj[t];
-              xi[t] = c * u - s * v;
-              xj[t] = s * u + c * v;
-            }
-            double *vi = V + static_cast<int64_t>(min(i, j)) * q;
-            double *vj = V + static_cast<int64_t>(max(i, j)) * q;
-#pragma unroll 4
-            for (int t = sub; t < q; t += 16) {
-              const double u = vi[t], v = vj[t];
-              vi[t] = c * u - s * v;
-              vj[t] = s * u + c * v;
-            }
-            any = 1;
-          }
-        }
+        __syncthreads();
       }
+      if (any && sub == 0) atomicOr(&rotated, 1);
       __syncthreads();
+      const int done = (rotated == 0);
+      __syncthreads();
+      if (done) break;
     }
-    if (any && sub == 0) atomicOr(&rotated, 1);
-    __syncthreads();
-    const int done = (rotated == 0);
-    __syncthreads();
-    if (done) break;
-  }
 
-  // singular values and their rank (descending, ties by index)
-  for (int k = wave; k < q; k += nwaves) {
-    const double *xk = X + static_cast<int64_t>(k) * p;
-    double a = 0.0;
-    for (int t = lane; t < p; t += 64) a += xk[t] * xk[t];
-    a = wave_sum(a);
-    if (lane == 0) sv[k] = sqrt(a);
-  }
-  __syncthreads();
-  double *U = bt.U + D.u_off;
-  double *VT = bt.VT + D.vt_off;
-  for (int k = wave; k < q; k += nwaves) {
-    const double sk = sv[k];
-    int rank = 0;
-    for (int l = lane; l < q; l += 64) {
-      const double sl = sv[l];
-      rank += (sl > sk || (sl == sk && l < k)) ? 1 : 0;
+    // singular values and their rank (descending, ties by index)
+    for (int k = wave; k < q; k += nwaves) {
+      const double *xk = X + static_cast<int64_t>(k) * p;
+      double a = 0.0;
+      for (int t = lane; t < p; t += 64) a += xk[t] * xk[t];
+      a = wave_sum(a);
+      if (lane == 0) sv[k] = sqrt(a);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 64);
-    const double inv = sk > 0.0 ? 1.0 / sk : 0.0;
-    const double *xk = X + static_cast<int64_t>(k) * p;
-    const double *vk = V + static_cast<int64_t>(k) * q;
-    if (!tr) {
-      // A = U S V^T: U[:, rank] = x_k / s_k (m = p rows), VT[rank, :] = v_k
-      for (int t = lane; t < p; t += 64) U[static_cast<int64_t>(t) * q + rank] = xk[t] * inv;
-      for (int t = lane; t < q; t += 64) VT[static_cast<int64_t>(rank) * n + t] = vk[t];
-    } else {
-      // A^T = X: A = V S (x/s)^T: U[:, rank] = v_k (m = q rows), VT[rank, :] = x_k / s_k
-      for (int t = lane; t < q; t += 64) U[static_cast<int64_t>(t) * q + rank] = vk[t];
-      for (int t = lane; t < p; t += 64) VT[static_cast<int64_t>(rank) * n + t] = xk[t] * inv;
+    __syncthreads();
+    double *U = bt.U + D.u_off;
+    double *VT = bt.VT + D.vt_off;
+    for (int k = wave; k < q; k += nwaves) {
+      const double sk = sv[k];
+      int rank = 0;
+      for (int l = lane; l < q; l += 64) {
+        const double sl = sv[l];
+        rank += (sl > sk || (sl == sk && l < k)) ? 1 : 0;
+      }
+  #pragma unroll
+      for (int off = 32; off > 0; off >>= 1) rank += __shfl_xor(rank, off, 64);
+      const double inv = sk > 0.0 ? 1.0 / sk : 0.0;
+      const double *xk = X + static_cast<int64_t>(k) * p;
+      const double *vk = V + static_cast<int64_t>(k) * q;
+      if (!tr) {
+        // A = U S V^T: U[:, rank] = x_k / s_k (m = p rows), VT[rank, :] = v_k
+        for (int t = lane; t < p; t += 64) U[static_cast<int64_t>(t) * q + rank] = xk[t] * inv;
+        for (int t = lane; t < q; t += 64) VT[static_cast<int64_t>(rank) * n + t] = vk[t];
+      } else {
+        // A^T = X: A = V S (x/s)^T: U[:, rank] = v_k (m = q rows), VT[rank, :] = x_k / s_k
+        for (int t = lane; t < q; t += 64) U[static_cast<int64_t>(t) * q + rank] = vk[t];
+        for (int t = lane; t < p; t += 64) VT[static_cast<int64_t>(rank) * n + t] = xk[t] * inv;
+      }
+      if (lane == 0) bt.S[D.s_off + rank] = sk;
     }
-    if (lane == 0) bt.S[D.s_off + rank] = sk;
-  }
+  };
+  if (in_lds)
+    body(lds, lds + pq);
+  else
+    body(bt.work + D.work_off, bt.work + D.work_off + pq);
 }
 
 // Instance for the register form of length R per lane (0: strided form only).

@@ -284,3 +284,14 @@ def test_gemm_split_k(gpu):
         _ops.grouped_gemm([(60, 50, 4000, 0, 4000, 1, 0, 50, 1, 0, 50)], a, b, C)
         outs.append(_ops.download(C))
     assert np.array_equal(outs[0], outs[1])
+
+
+def test_gemm_split_k_fills_slots(gpu):
+    """Range-finder products B = Q^T M (l x n over K = m, l << n): 167 tiles of
+    32 x 32 over K up to 1444 are split to fill the resident workgroup slots
+    (one tile kernel and one reduction), and match numpy for every layout."""
+    rng = np.random.default_rng(13)
+    shapes = [(19, 660, 668), (36, 1432, 1444), (21, 1216, 1220), (4, 576, 576)]
+    layouts = [(True, False), (True, False), (False, True), (False, False)]
+    _check_groups(shapes, layouts, rng)
+    assert _ops._gemm_dispatches() == 2

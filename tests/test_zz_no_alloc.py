@@ -37,7 +37,7 @@ def _call(fn, table, A, B, C):
 
 
 CASES = [
-    # uneven tiled groups (split over K in 4) plus one skinny-row group
+    # uneven tiled groups (split over K in 11) plus one skinny-row group
     [(200, 150, 3000), (64, 500, 1024), (17, 33, 700), (8, 1024, 64)],
     # one small product with a long K (split over K in 8), smaller than the first call
     [(64, 64, 2048)],
