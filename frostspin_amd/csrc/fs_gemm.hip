@@ -68,34 +68,36 @@ __device__ inline int xcd_logical_tile(int bid, int nblocks) {
 }
 
 // Global -> register tile loads: the index walk follows whichever dimension of
-// the operand is contiguous so every wavefront load is coalesced.
+// the operand is contiguous so every wavefront load is coalesced. Loads are
+// unconditional, from indices clamped into the matrix (m, n, k >= 1): a load
+// under a branch, or a select right behind it, makes the compiler wait for
+// every load in flight, which would serialise the prefetched K panels. Rows
+// and columns past the matrix then hold copies of its last row or column and
+// only reach output elements that are never written; K indices past the
+// split's range are zeroed when the panel is stored to LDS (k_mask_a/b).
 template <int BM, int BK, int PER, int NT = kThreads>
 __device__ inline void load_a(double (&r)[PER], const double *__restrict__ A, const GroupDev &G,
-                              int m0, int k0, int kend, bool kcontig) {
+                              int m0, int k0, bool kcontig) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int idx = threadIdx.x + NT * q;
     const int i = kcontig ? idx / BK : idx % BM;
     const int kk = kcontig ? idx % BK : idx / BM;
-    const int gi = m0 + i, gk = k0 + kk;
-    r[q] = (gi < G.m && gk < kend)
-               ? A[static_cast<int64_t>(gi) * G.a_rs + static_cast<int64_t>(gk) * G.a_cs]
-               : 0.0;
+    r[q] = A[static_cast<int64_t>(min(m0 + i, G.m - 1)) * G.a_rs +
+             static_cast<int64_t>(min(k0 + kk, G.k - 1)) * G.a_cs];
   }
 }
 
 template <int BN, int BK, int PER, int NT = kThreads>
 __device__ inline void load_b(double (&r)[PER], const double *__restrict__ B, const GroupDev &G,
-                              int n0, int k0, int kend, bool ncontig) {
+                              int n0, int k0, bool ncontig) {
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int idx = threadIdx.x + NT * q;
     const int j = ncontig ? idx % BN : idx / BK;
     const int kk = ncontig ? idx / BN : idx % BK;
-    const int gj = n0 + j, gk = k0 + kk;
-    r[q] = (gj < G.n && gk < kend)
-               ? B[static_cast<int64_t>(gk) * G.b_rs + static_cast<int64_t>(gj) * G.b_cs]
-               : 0.0;
+    r[q] = B[static_cast<int64_t>(min(k0 + kk, G.k - 1)) * G.b_rs +
+             static_cast<int64_t>(min(n0 + j, G.n - 1)) * G.b_cs];
   }
 }
 
@@ -125,7 +127,12 @@ constexpr int gemm_waves_per_eu() {
   return (BM == 64 && BN == 64 && BK == 16 && WGM * WGN == 4) ? 4 : 1;
 }
 
-template <int BM, int BN, int BK, int WGM = 2, int WGN = 2>
+// PF: K panels in flight per workgroup (register prefetch depth). A launch
+// with few workgroups per CU and a long K (thin products) is bound by the
+// memory latency of its panel loads; PF panels in flight multiply its
+// bandwidth. The panel loads of a K step are issued right after the previous
+// panel went from registers to LDS, PF steps ahead of their use.
+template <int BM, int BN, int BK, int WGM = 2, int WGN = 2, int PF = 1>
 __global__ void __launch_bounds__(64 * WGM * WGN)
 __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK, WGM, WGN>()))) gemm_f64_kernel(Batch bt) {
   constexpr int NT = 64 * WGM * WGN;
@@ -162,61 +169,92 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK, WGM, WGN>()))) 
   const bool b_ncontig = (G.b_cs == 1);
   constexpr int A_PER = BM * BK / NT;
   constexpr int B_PER = BN * BK / NT;
-  double ra[A_PER], rb[B_PER];
+  double ra[PF][A_PER], rb[PF][B_PER];  // panel p lives in slot p % PF
   // LDS element strides (k, i) of A and (k, j) of B for the chosen layouts
   const int a_ks = a_kcontig ? 1 : BM + PAD, a_is = a_kcontig ? BK + 1 : 1;
   const int b_ks = b_ncontig ? BN + PAD : 1, b_js = b_ncontig ? 1 : BK + 1;
 
-  auto store = [&](int buf) {
+  // store the panel starting at K index k0, zero past the split's end ke
+  auto store = [&](int buf, const double (&sa)[A_PER], const double (&sb)[B_PER], int k0, int ke) {
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const int idx = threadIdx.x + NT * q;
       const int i = a_kcontig ? idx / BK : idx % BM;
       const int kk = a_kcontig ? idx % BK : idx / BM;
-      As[buf][kk * a_ks + i * a_is] = ra[q];
+      As[buf][kk * a_ks + i * a_is] = k0 + kk < ke ? sa[q] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < B_PER; ++q) {
       const int idx = threadIdx.x + NT * q;
       const int j = b_ncontig ? idx % BN : idx / BK;
       const int kk = b_ncontig ? idx / BN : idx % BK;
-      Bs[buf][kk * b_ks + j * b_js] = rb[q];
+      Bs[buf][kk * b_ks + j * b_js] = k0 + kk < ke ? sb[q] : 0.0;
     }
   };
 
-  // split-K: blockIdx.y selects the K range [kb, ke) of this workgroup
+  // split-K: blockIdx.y selects the K range [kb, ke) of this workgroup (empty
+  // for the last splits of a group much shorter than the launch's longest K)
   const int kb = static_cast<int>(blockIdx.y) * G.kc;
   const int ke = min(G.k, kb + G.kc);
-  load_a<BM, BK, A_PER, NT>(ra, A, G, m0, kb, ke, a_kcontig);
-  load_b<BN, BK, B_PER, NT>(rb, B, G, n0, kb, ke, b_ncontig);
-  store(0);
-  __syncthreads();
-  int cur = 0;
-  for (int k0 = kb; k0 < ke; k0 += BK) {
-    const bool next = k0 + BK < ke;
-    if (next) {
-      load_a<BM, BK, A_PER, NT>(ra, A, G, m0, k0 + BK, ke, a_kcontig);
-      load_b<BN, BK, B_PER, NT>(rb, B, G, n0, k0 + BK, ke, b_ncontig);
-    }
+  const int np = ke > kb ? (ke - kb + BK - 1) / BK : 0;  // K panels
+  auto mfma_panel = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4);
       double av[TI], bv[TJ];
 #pragma unroll
       for (int a = 0; a < TI; ++a)
-        av[a] = As[cur][kr * a_ks + (wm * WM + a * 16 + (lane & 15)) * a_is];
+        av[a] = As[buf][kr * a_ks + (wm * WM + a * 16 + (lane & 15)) * a_is];
 #pragma unroll
       for (int b = 0; b < TJ; ++b)
-        bv[b] = Bs[cur][kr * b_ks + (wn * WN + b * 16 + (lane & 15)) * b_js];
+        bv[b] = Bs[buf][kr * b_ks + (wn * WN + b * 16 + (lane & 15)) * b_js];
 #pragma unroll
       for (int a = 0; a < TI; ++a)
 #pragma unroll
         for (int b = 0; b < TJ; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    if (next) store(cur ^ 1);
+  };
+  // Panel p lives in register slot p % PF until it goes to LDS buffer p & 1.
+  // The loads stay outside any branch (a load issued on one path only makes
+  // the compiler wait for all loads at the join): the prologue loads panels
+  // 0..PF, the main loop runs whole groups of PF panels, storing panel p + 1
+  // and loading panel p + 1 + PF after each (clamped loads; panels past the
+  // range store zeros that are never read), and the tail finishes the last
+  // np % PF panels from the slots without loading.
+  int cur = 0;
+  if (G.k > 0) {  // uniform; k = 0 leaves the accumulators at zero
+    load_a<BM, BK, A_PER, NT>(ra[0], A, G, m0, kb, a_kcontig);
+    load_b<BN, BK, B_PER, NT>(rb[0], B, G, n0, kb, b_ncontig);
+    store(0, ra[0], rb[0], kb, ke);
+#pragma unroll
+    for (int q = 1; q <= PF; ++q) {
+      load_a<BM, BK, A_PER, NT>(ra[q % PF], A, G, m0, kb + q * BK, a_kcontig);
+      load_b<BN, BK, B_PER, NT>(rb[q % PF], B, G, n0, kb + q * BK, b_ncontig);
+    }
     __syncthreads();
-    cur ^= 1;
+    int p = 0;
+    for (; p + PF <= np; p += PF) {
+#pragma unroll
+      for (int s = 0; s < PF; ++s) {
+        const int slot = (s + 1) % PF;  // holds panel p + s + 1
+        mfma_panel(cur);
+        store(cur ^ 1, ra[slot], rb[slot], kb + (p + s + 1) * BK, ke);
+        load_a<BM, BK, A_PER, NT>(ra[slot], A, G, m0, kb + (p + s + 1 + PF) * BK, a_kcontig);
+        load_b<BN, BK, B_PER, NT>(rb[slot], B, G, n0, kb + (p + s + 1 + PF) * BK, b_ncontig);
+        __syncthreads();
+        cur ^= 1;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < PF - 1; ++s) {  // p is a multiple of PF: panel p + s + 1 in slot s + 1
+      if (p + s < np) {
+        mfma_panel(cur);
+        if (p + s + 1 < np) store(cur ^ 1, ra[s + 1], rb[s + 1], kb + (p + s + 1) * BK, ke);
+        __syncthreads();
+        cur ^= 1;
+      }
+    }
   }
 
   // epilogue: for v_mfma_f64_16x16x4f64 lane l holds D[l/16 + 4*r][l%16] of
@@ -464,10 +502,25 @@ constexpr int kBK = 16;
 // workgroup per CU at 398 registers) 30-39 TFLOP/s and far less on thin or
 // small shapes, so 64x64 is the default; FS_GEMM_TILE=128 forces the other.
 // 64 x 64 tiles by default; 32 x 32 when a launch would not cover the chip's
-// 256 CUs with 64 x 64 tiles (narrow or small products: range-finder panels,
-// Gram partials), which quadruples the workgroups at the same total traffic
-// through L2. FS_GEMM_TILE=32/64/128 forces a size (tuning only).
+// 256 CUs with 64 x 64 tiles (small products, Gram partials), which
+// quadruples the workgroups at the same total traffic through L2.
+// FS_GEMM_TILE=32/64/128 forces a size (tuning only). Panel tiles spanning a
+// thin side of up to 128 (64 x 128, the large operand streamed once) measured
+// 20-45% slower than 32 x 32 tiles with four K panels in flight on the
+// range-finder products (m x m times m x l, l = 20..69).
 constexpr int kCUs = 256;
+
+struct TileShape {
+  int bm, bn;
+};
+
+// Static LDS of one workgroup of gemm_f64_kernel<BM, BN, kBK> (double
+// buffered A and B panels, padded for either source layout).
+constexpr int tile_lds_bytes(int bm, int bn) {
+  const int a = (kBK * (bm + 16) > bm * (kBK + 1)) ? kBK * (bm + 16) : bm * (kBK + 1);
+  const int b = (kBK * (bn + 16) > bn * (kBK + 1)) ? kBK * (bn + 16) : bn * (kBK + 1);
+  return 2 * (a + b) * 8;
+}
 
 // Split-K: a launch whose output tiles leave most workgroup slots of the chip
 // empty while its K is long (projected corners: chi x chi outputs over chi D^2;
@@ -477,19 +530,22 @@ constexpr int kCUs = 256;
 // workgroup keeps one K panel in flight, so on such launches the bytes in
 // flight, and with them the bandwidth reached, grow with the number of
 // resident workgroups: the split fills the resident slots (slots_per_cu
-// workgroups per CU for the tile size, set by its LDS footprint: 40 KB for
-// 64 x 64, 24 KB for 32 x 32). FS_GEMM_SPLITK=0 disables it (tuning only).
+// workgroups per CU, set by the tile's LDS footprint out of 160 KB, at most 6).
+// FS_GEMM_SPLITK=0 disables it (tuning only).
 constexpr int kSplitMinK = 256;
 constexpr int kMaxSplits = 16;
 
-inline int slots_per_cu(int tsz) { return tsz == 32 ? 6 : (tsz == 64 ? 4 : 1); }
+inline int slots_per_cu(const TileShape &t) {
+  if (t.bm == 128) return 1;  // register bound
+  return std::max(1, std::min(6, (160 * 1024) / tile_lds_bytes(t.bm, t.bn)));
+}
 
-inline int pick_splits(int64_t tiles, int64_t kmax, int tsz) {
+inline int pick_splits(int64_t tiles, int64_t kmax, const TileShape &tile) {
   static const bool off = [] {
     const char *e = std::getenv("FS_GEMM_SPLITK");
     return e && std::atoi(e) == 0;
   }();
-  const int64_t slots = static_cast<int64_t>(kCUs) * slots_per_cu(tsz);
+  const int64_t slots = static_cast<int64_t>(kCUs) * slots_per_cu(tile);
   if (off || tiles * 2 > slots || kmax < 2 * kSplitMinK) return 1;
   const int64_t by_slots = slots / tiles;
   const int64_t by_k = kmax / kSplitMinK;
@@ -591,16 +647,35 @@ int scratch_ptr_locked(double **out) {
   return FS_OK;
 }
 
-inline int pick_tile(int ng, const fs_gemm_group *g) {
+inline TileShape pick_tile(int ng, const fs_gemm_group *g) {
   static const int forced = [] {
     const char *e = std::getenv("FS_GEMM_TILE");
     return e ? std::atoi(e) : 0;
   }();
-  if (forced == 32 || forced == 64 || forced == 128) return forced;
+  if (forced == 32 || forced == 64 || forced == 128) return {forced, forced};
   int64_t tiles64 = 0;
   for (int i = 0; i < ng; ++i)
     if (g[i].m > 0 && g[i].n > 0) tiles64 += fs::ceil_div(g[i].m, 64) * fs::ceil_div(g[i].n, 64);
-  return tiles64 < kCUs ? 32 : 64;
+  const int t = tiles64 < kCUs ? 32 : 64;
+  return {t, t};
+}
+
+// K panels in flight per workgroup (gemm_f64_kernel PF) for the 32 x 32
+// tiles: their launches hold few workgroups per CU over long K ranges
+// (range-finder products, Gram partials); four in flight measured 15-20%
+// faster than one on the range-finder products (two and eight in between).
+// The 64 x 64 tiles of the large square products keep one (register budget
+// of 4 waves per SIMD).
+constexpr int kSmallPrefetch = 4;
+
+// Launch one tiled batch with the kernel instance of its tile shape.
+void launch_tiles(const TileShape &t, dim3 grid, hipStream_t stream, const Batch &bt) {
+  if (t.bm == 128)
+    hipLaunchKernelGGL((gemm_f64_kernel<128, 128, kBK>), grid, dim3(kThreads), 0, stream, bt);
+  else if (t.bm == 32)
+    hipLaunchKernelGGL((gemm_f64_kernel<32, 32, kBK, 2, 2, kSmallPrefetch>), grid, dim3(kThreads), 0, stream, bt);
+  else
+    hipLaunchKernelGGL((gemm_f64_kernel<64, 64, kBK>), grid, dim3(kThreads), 0, stream, bt);
 }
 
 thread_local int last_dispatches = 0;
@@ -679,7 +754,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     const size_t c1 = std::min(tiled.size(), c0 + kMaxGroups);
     std::vector<fs_gemm_group> sel;
     for (size_t t = c0; t < c1; ++t) sel.push_back(groups[tiled[t]]);
-    const int tsz = SPLIT ? 64 : pick_tile(static_cast<int>(sel.size()), sel.data());
+    const TileShape tile = SPLIT ? TileShape{64, 64} : pick_tile(static_cast<int>(sel.size()), sel.data());
     std::unique_lock<std::mutex> ws_lock;  // held from workspace lookup to the reduction launch
     Batch bt;
     std::memset(&bt, 0, sizeof(bt));
@@ -694,8 +769,8 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
       GroupDev &d = bt.g[ng];
       int st = fill_group(d, q, A[i * base_stride], B[i * base_stride], C[i * base_stride]);
       if (st != FS_OK) return st;
-      d.tiles_n = static_cast<int32_t>(fs::ceil_div(q.n, tsz));
-      tiles += fs::ceil_div(q.m, tsz) * d.tiles_n;
+      d.tiles_n = static_cast<int32_t>(fs::ceil_div(q.n, tile.bn));
+      tiles += fs::ceil_div(q.m, tile.bm) * d.tiles_n;
       FS_CHECK_ARG(tiles < (1ll << 31), "grouped gemm: too many tiles");
       bt.tile_prefix[ng + 1] = static_cast<int32_t>(tiles);
       ++ng;
@@ -706,7 +781,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     if (!SPLIT) {
       int64_t kmax = 0, elems = 0;
       for (int i = 0; i < ng; ++i) kmax = std::max<int64_t>(kmax, bt.g[i].k);
-      splits = pick_splits(tiles, kmax, tsz);
+      splits = pick_splits(tiles, kmax, tile);
       if (splits > 1) {
         bt.elem_prefix[0] = 0;
         for (int i = 0; i < ng; ++i) {
@@ -739,15 +814,8 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     if (SPLIT)
       hipLaunchKernelGGL((gemm_split_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,
                          fs::as_stream(stream), bt);
-    else if (tsz == 128)
-      hipLaunchKernelGGL((gemm_f64_kernel<128, 128, kBK>), grid, dim3(kThreads), 0,
-                         fs::as_stream(stream), bt);
-    else if (tsz == 32)
-      hipLaunchKernelGGL((gemm_f64_kernel<32, 32, kBK>), grid, dim3(kThreads), 0,
-                         fs::as_stream(stream), bt);
     else
-      hipLaunchKernelGGL((gemm_f64_kernel<64, 64, kBK>), grid, dim3(kThreads), 0,
-                         fs::as_stream(stream), bt);
+      launch_tiles(tile, grid, fs::as_stream(stream), bt);
     FS_LAUNCH_CHECK(SPLIT ? "gemm_split_kernel" : "gemm_f64_kernel");
     ++dispatches;
     if (splits > 1) {

@@ -1,11 +1,13 @@
 """CTMRG projectors and renormalisation (ctmrg/ctm_renormalize.py:6-341).
 
 construct_projectors runs every symmetry sector in the same launches:
-R = c1 @ c2, Rt = c3 @ c4 and M = R @ Rt as three grouped FP64 MFMA GEMMs over
-all shared sectors, then one batched SVD of all M blocks (exact Jacobi for
-blocks up to 1024, randomized range finder with power iterations above, where
-the reference calls ARPACK), then the multiplet-aware cut on the host, then
-P = s^-1/2 U^T R and Pt = Rt V s^-1/2 as two more grouped GEMMs with the
+R = c1 @ c2, Rt = c3 @ c4 and M = R @ Rt either formed as three grouped FP64
+MFMA GEMMs over all shared sectors or, when a cost model says it is cheaper
+(large sectors), kept as the factor chain c1 c2 c3 c4 and applied factor by
+factor to the thin operands; then one batched SVD of all M blocks (exact
+Jacobi for blocks up to 1024, randomized range finder with power iterations
+above, where the reference calls ARPACK), then the multiplet-aware cut on the
+host, then P = s^-1/2 U^T R and Pt = Rt V s^-1/2 as grouped GEMMs with the
 s^-1/2 scaling fused into the slicing copy.
 """
 
@@ -99,13 +101,9 @@ def construct_projectors_batch(
         )
         for c in range(4)
     ]
-    R = bl.gemm(cmats[0], cmats[1], split=split)
-    Rt = bl.gemm(cmats[2], cmats[3], split=split)
-    M = bl.gemm(R, Rt, split=split)
-
     n = len(sectors)
     block_chi = np.array([sec[2] for sec in sectors], dtype=int)
-    mins = np.minimum(M.rows, M.cols)
+    mins = np.minimum(cmats[0].rows, cmats[3].cols)
     ncv = (ncv_ratio * block_chi).astype(int) + 10
     # the reference uses ARPACK (block_chi values) where min(shape) >= max(100, 4 ncv)
     sparse_ref = mins >= np.maximum(100, 4 * ncv)
@@ -119,6 +117,16 @@ def construct_projectors_batch(
     dense = (mins <= SMALL_DENSE) | ((l_need >= 0.8 * mins) & (mins <= DENSE_SVD_LIMIT))
     dense_ids = [i for i in range(n) if dense[i]]
     rand_ids = [i for i in range(n) if not dense[i]]
+    if _implicit_pays(cmats, dense_ids, rand_ids, l_need, k_need):
+        # R = c1 c2, Rt = c3 c4 and M = R Rt applied factor by factor to the
+        # thin operands; only the dense-branch sectors are formed
+        R = bl.MatProduct(cmats[:2])
+        Rt = bl.MatProduct(cmats[2:])
+        M = bl.MatProduct(cmats)
+    else:
+        R = bl.gemm(cmats[0], cmats[1], split=split)
+        Rt = bl.gemm(cmats[2], cmats[3], split=split)
+        M = bl.gemm(R, Rt, split=split)
     U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
     # every singular value the truncation needs reaches the host in one wait per
     # stream: the readbacks are started where the values are produced and read
@@ -264,6 +272,61 @@ def construct_projectors_batch(
         )
         out.append((P, Pt))
     return out
+
+
+# Cost model of a grouped FP64 GEMM launch on the device: at least
+# GEMM_LAUNCH_S (launch and tail latency of a small launch), else its flops at
+# the rate measured on MI355X for its kind: GEMM_RATE_SQUARE for the corner
+# products (chi=256 SU(2) sectors: 37 TFLOP/s), GEMM_RATE_THIN for products
+# with a range-finder panel (l = 20..69 columns: 17 TFLOP/s).
+GEMM_LAUNCH_S = 20e-6
+GEMM_RATE_SQUARE = 37e12
+GEMM_RATE_THIN = 17e12
+# None: the cost model below picks formed or factored products per call;
+# True / False forces one form (tests compare the two).
+FACTORED = None
+
+
+def _implicit_pays(cmats, dense_ids, rand_ids, l_need, k_need, power_iters=2):
+    """Whether the projector step is cheaper with M = c1 c2 c3 c4 (and R, Rt)
+    applied factor by factor than formed. Formed: three launches over every
+    sector (R, Rt, M), then one launch per product with M in the randomized SVD
+    (range finder, 2 per power iteration, the projection B = Q^T M and the
+    residual check) and one per projector. Factored: the dense-branch sectors
+    formed (three launches over them), four launches per product with M and two
+    per projector. Large sectors favour the factored form (chi=256 SU(2): the
+    three formed launches cost ~1.7 ms, the extra thin launches ~1.2 ms); small
+    ones the formed form (each extra launch costs its latency)."""
+
+    def launch(flop, rate=GEMM_RATE_THIN):
+        return max(GEMM_LAUNCH_S, flop / rate) if flop > 0 else 0.0
+
+    if FACTORED is not None:
+        return FACTORED
+    if not rand_ids:
+        return False
+    rows = [c.rows.astype(np.float64) for c in cmats]
+    cols = [c.cols.astype(np.float64) for c in cmats]
+    every = np.arange(len(rows[0]))
+    dense = np.asarray(dense_ids, dtype=np.int64)
+    rand = np.asarray(rand_ids, dtype=np.int64)
+
+    def form(ids):  # R, Rt, M over the sectors ids
+        return (launch(2.0 * (rows[0] * cols[0] * cols[1])[ids].sum(), GEMM_RATE_SQUARE)
+                + launch(2.0 * (rows[2] * cols[2] * cols[3])[ids].sum(), GEMM_RATE_SQUARE)
+                + launch(2.0 * (rows[0] * cols[1] * cols[3])[ids].sum(), GEMM_RATE_SQUARE))
+
+    n_apply = 1 + 2 * power_iters + 2
+    width = l_need.astype(np.float64)[rand]
+    thin_m = 2.0 * (rows[0] * cols[3])[rand] * width
+    thin_f = [2.0 * (rows[f] * cols[f])[rand] * width for f in range(4)]
+    kept = k_need.astype(np.float64)
+    proj_m = [2.0 * (kept * rows[0] * cols[1]).sum(), 2.0 * (kept * rows[2] * cols[3]).sum()]
+    proj_f = [2.0 * (kept * rows[f] * cols[f]).sum() for f in range(4)]
+    formed = form(every) + n_apply * launch(thin_m.sum()) + sum(launch(x) for x in proj_m)
+    factored = ((form(dense) if dense.size else 0.0) + n_apply * sum(launch(x.sum()) for x in thin_f)
+                + sum(launch(x) for x in proj_f))
+    return factored < formed
 
 
 def _scaled(parts, cuts, w_dev, w_off, *, cols):

@@ -81,8 +81,56 @@ class Mats:
         return Mats(buf, self.off[idx], self.rows[idx], self.cols[idx], self.rs[idx], self.cs[idx])
 
 
+class MatProduct:
+    """Matrices A_i = F0_i @ F1_i @ ... given by their factors (Mats of equal
+    length, conforming shapes) and never formed unless asked (materialize).
+    gemm with a product evaluates it factor by factor starting from the other
+    operand's side, so a thin operand (a range-finder panel, a block of
+    singular vectors) keeps every intermediate thin: A @ X costs
+    sum_f 2 rows(F_f) cols(F_f) cols(X) instead of the cost of forming A."""
+
+    def __init__(self, factors):
+        self.factors = list(factors)
+        assert self.factors and all(len(f) == len(self.factors[0]) for f in self.factors)
+        for f, g in zip(self.factors[:-1], self.factors[1:]):
+            assert (f.cols == g.rows).all()
+
+    def __len__(self):
+        return len(self.factors[0])
+
+    @property
+    def rows(self):
+        return self.factors[0].rows
+
+    @property
+    def cols(self):
+        return self.factors[-1].cols
+
+    def T(self):
+        return MatProduct([f.T() for f in reversed(self.factors)])
+
+    def select(self, idx):
+        idx = list(idx)
+        return MatProduct([f.select(idx) for f in self.factors])
+
+    def materialize(self, *, split=False):
+        """The products as packed matrices, pairwise: (F0 F1)(F2 F3)..."""
+        level = list(self.factors)
+        while len(level) > 1:
+            nxt = [gemm(level[i], level[i + 1], split=split) for i in range(0, len(level) - 1, 2)]
+            if len(level) % 2:
+                nxt.append(level[-1])
+            level = nxt
+        return level[0]
+
+
+def _dense(A):
+    return A.materialize() if isinstance(A, MatProduct) else A
+
+
 def gemm(A, B, *, alpha=1.0, split=False):
-    """C_i = alpha * A_i @ B_i for all i in one grouped launch; C packed row-major."""
+    """C_i = alpha * A_i @ B_i for all i in one grouped launch (one launch per
+    factor when an operand is a MatProduct); C packed row-major."""
     C = Mats.packed(A.rows, B.cols)
     gemm_into(A, B, C, alpha=alpha, split=split)
     return C
@@ -90,7 +138,22 @@ def gemm(A, B, *, alpha=1.0, split=False):
 
 def gemm_into(A, B, C, *, alpha=1.0, split=False):
     """C_i = alpha * A_i @ B_i into given row-major outputs (C_i row stride c.rs,
-    column stride 1); operands and outputs may span several buffers."""
+    column stride 1); operands and outputs may span several buffers. A product
+    operand (MatProduct, not both) is applied factor by factor from the side of
+    the other operand, the last step writing C."""
+    if isinstance(A, MatProduct):
+        assert not isinstance(B, MatProduct)
+        X = B
+        for f in reversed(A.factors[1:]):
+            X = gemm(f, X, split=split)
+        gemm_into(A.factors[0], X, C, alpha=alpha, split=split)
+        return
+    if isinstance(B, MatProduct):
+        X = A
+        for f in B.factors[:-1]:
+            X = gemm(X, f, split=split)
+        gemm_into(X, B.factors[-1], C, alpha=alpha, split=split)
+        return
     assert len(A) == len(B) == len(C)
     assert (A.cols == B.rows).all() and (C.rows == A.rows).all() and (C.cols == B.cols).all()
     assert (C.cs[C.rows * C.cols > 0] == 1).all()
@@ -133,6 +196,7 @@ def svd_async(A, *, max_sweeps=60):
     if q.size and q.max() > _ops.SVD_MAX_MIN_DIM:
         msg = "dense device SVD limited to min(m, n) <= 1024; use rsvd"
         raise NotImplementedError(msg)
+    A = _dense(A)
     U = Mats.packed(A.rows, q)
     VT = Mats.packed(q, A.cols)
     s_off = np.zeros_like(q)

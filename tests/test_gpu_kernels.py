@@ -295,3 +295,38 @@ def test_gemm_split_k_fills_slots(gpu):
     layouts = [(True, False), (True, False), (False, True), (False, False)]
     _check_groups(shapes, layouts, rng)
     assert _ops._gemm_dispatches() == 2
+
+
+@pytest.mark.parametrize(
+    "shapes",
+    [
+        # thin n: the range-finder products A X, uneven groups
+        [(1444, 69, 1444), (668, 43, 668), (200, 20, 150), (37, 5, 3)],
+        [(600, 128, 300), (129, 100, 50)],
+        [(300, 20, 100), (129, 32, 40), (7, 1, 1)],
+        # thin m: the projections X^T A
+        [(69, 1444, 1444), (43, 668, 668), (20, 200, 150), (5, 37, 3)],
+        [(128, 700, 200), (100, 600, 64)],
+    ],
+    ids=["n69", "n128", "n32", "m69", "m128"],
+)
+def test_gemm_thin_shapes(gpu, shapes):
+    """Products with one thin side over long K ranges (32 x 32 tiles, split-K,
+    four K panels in flight, K tails of every length modulo the panel depth):
+    every operand layout, alpha and beta, uneven groups, against numpy."""
+    rng = np.random.default_rng(14)
+    for layouts in ([(False, False)] * len(shapes), [(True, True)] * len(shapes),
+                    [(i % 2 == 0, i % 2 == 1) for i in range(len(shapes))],
+                    [(i % 2 == 1, i % 2 == 0) for i in range(len(shapes))]):
+        _check_groups(shapes, layouts, rng)
+
+
+@pytest.mark.parametrize("k", [0, 1, 15, 16, 17, 48, 63, 64, 65, 80, 129, 1000])
+def test_gemm_k_tails(gpu, k):
+    """Every K length modulo the panel (16) and the panel depth (4 x 16), K = 0
+    (C = beta C) included, with and without split-K: one small and one large
+    launch per K against numpy."""
+    rng = np.random.default_rng(15 + k)
+    # a (2, 2, 2) group keeps the operand buffers non-empty when k = 0
+    _check_groups([(40, 30, k), (3, 50, k), (2, 2, 2)], [(False, True), (True, False), (False, False)], rng)
+    _check_groups([(300, 260, k), (2, 2, 2)], [(False, False), (True, True)], rng)

@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 from frostspin_amd import U1SymmetricTensor
+from frostspin_amd.ctmrg import ctm_renormalize
 from frostspin_amd.ctmrg.ctm_renormalize import construct_projectors_batch
 from oracle.abelian import find_block_cuts as oracle_cuts
 
@@ -83,8 +84,15 @@ def _reference_rule(corners, last, chi, ncv_ratio, block_chi_ratio, degen_ratio,
     return np.array(shared), cuts, s_list
 
 
+@pytest.fixture(params=[False, True], ids=["formed", "factored"])
+def product_form(request, monkeypatch):
+    """M = (c1 c2)(c3 c4) formed, or applied factor by factor."""
+    monkeypatch.setattr(ctm_renormalize, "FACTORED", request.param)
+    return request.param
+
+
 @pytest.mark.parametrize("rtol", [None, 1e-10])
-def test_projector_cuts_follow_reference_branches(backend, rtol):
+def test_projector_cuts_follow_reference_branches(backend, rtol, product_form):
     rng = np.random.default_rng(3)
     sizes = {-1: 150, 0: 200, 1: 150, 2: 60}
     decays = {-1: 0.9, 0: 0.93, 1: 0.9, 2: 0.5}
@@ -107,3 +115,41 @@ def test_projector_cuts_follow_reference_branches(backend, rtol):
     mid = P.col_reps[0] if P.n_row_reps == len(P.row_reps) else None
     assert mid is not None
     assert sum(b.shape[0] for b in P.transpose().blocks) == int(ref_cuts.sum())
+
+
+def test_factored_and_formed_projectors_agree(backend, monkeypatch):
+    """The same problem with M formed and with M applied factor by factor:
+    identical kept counts, gauge-invariant products Pt P^T equal to rounding."""
+    rng = np.random.default_rng(4)
+    sizes = {-1: 150, 0: 200, 1: 150, 2: 60}
+    decays = {-1: 0.9, 0: 0.93, 1: 0.9, 2: 0.5}
+    corners = _corners(rng, sizes, decays)
+    last = U1SymmetricTensor.random((_rep({q: 4 for q in sizes}),), (_rep({q: 5 for q in sizes}),), rng=rng)
+    out = {}
+    for factored in (False, True):
+        monkeypatch.setattr(ctm_renormalize, "FACTORED", factored)
+        diag = {}
+        ((P, Pt),) = construct_projectors_batch([(*corners, last)], 80, diagnostics=diag)
+        out[factored] = (diag["cuts"].tolist(), (Pt @ P.transpose()).toarray())
+    assert out[False][0] == out[True][0]
+    ref, got = out[False][1], out[True][1]
+    assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_cost_model_factors_large_sectors():
+    """The cost model keeps M formed for small sectors (launch latency
+    dominates) and factored for the chi=256 SU(2) sector sizes."""
+    from frostspin_amd.misc_tools import block_linalg as bl
+
+    def mats(sizes):
+        s = np.array(sizes, dtype=np.int64)
+        return bl.Mats(None, np.zeros_like(s), s, s, s, np.ones_like(s))
+
+    def pays(sizes, l):
+        cm = [mats(sizes) for _ in range(4)]
+        ids = list(range(len(sizes)))
+        l = np.array(l, dtype=np.int64)
+        return ctm_renormalize._implicit_pays(cm, [], ids, l, (2 * l) // 3)
+
+    assert pays([668, 1444, 1220, 576, 148] * 2, [43, 69, 46, 24, 20] * 2)
+    assert not pays([120, 200, 150], [20, 30, 25])
