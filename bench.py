@@ -336,7 +336,7 @@ def run_ctmrg(args):
         h = models.xxz_bond(0.5, 1.0)
         model = "U(1) XXZ (Jz=0.5, Jperp=1)"
     t0 = time.perf_counter()
-    su, ctm = models.thermal_ctmrg(h, args.D, args.beta, args.chi)
+    su, ctm = models.thermal_ctmrg(h, args.D, args.beta, args.chi, cutoff=args.cutoff)
     torch.cuda.synchronize()
     t_su = time.perf_counter() - t0
     precision = None
@@ -373,7 +373,8 @@ def run_ctmrg(args):
     ms = wall * 1000.0 / args.steps
     gemm = kr.get("gemm", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
     perm = kr.get("lie" if args.symmetry == "SU2" else "permute", {"rate": 0.0, "ms": 0.0, "launches": 0, "work": 0.0})
-    default_workload = (args.symmetry, args.D, args.chi, args.beta, args.precision) == ("SU2", 8, 256, 4.0, "fp64")
+    default_workload = (args.symmetry, args.D, args.chi, args.beta, args.precision, args.cutoff) == (
+        "SU2", 8, 256, 4.0, "fp64", 1e-10)
     traffic = GEMM_TRAFFIC_SU2_D8_CHI256 if default_workload else None
     res = {
         "metric": METRIC,
@@ -392,7 +393,7 @@ def run_ctmrg(args):
         "data": "synthetic (simple-update thermal state, no checkpoint)",
         "config": {
             "workload": f"{model} finite-temperature CTMRG, AB/BA cell, D={args.D}, "
-            f"chi={args.chi}, beta={su.beta:.3g} (simple update tau=0.01), FP64",
+            f"chi={args.chi}, beta={su.beta:.3g} (simple update tau=0.01), cutoff={args.cutoff:g}, FP64",
             "chi_values": [int(c) for c in ctm.chi_values],
             "energy_per_bond": energy,
             "simple_update_s": t_su,
@@ -588,6 +589,8 @@ def main():
     ap.add_argument("--chi", type=int, default=256)
     ap.add_argument("--D", type=int, default=8)
     ap.add_argument("--beta", type=float, default=4.0)
+    ap.add_argument("--cutoff", type=float, default=1e-10,
+                    help="ctmrg: relative singular-value cutoff of the projectors (U(1) D=10 needs 1e-12 for chi=256 to bind)")
     ap.add_argument("--symmetry", default="SU2", choices=["SU2", "U1"])
     ap.add_argument("--no-cpu-baseline", dest="cpu_baseline", action="store_false")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "split"],

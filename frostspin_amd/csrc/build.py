@@ -86,7 +86,11 @@ def build(verbose=False):
         res = subprocess.run(cmd, capture_output=True, text=True, check=False)
         if res.returncode != 0:
             raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-        shutil.copy2(tmp_out, out)
+        # replace the library by a rename: a process that has the old file
+        # mapped keeps its copy (overwriting it in place would fault that process)
+        staged = out.with_name(out.name + ".tmp")
+        shutil.copy2(tmp_out, staged)
+        os.replace(staged, out)
     return out
 
 

@@ -311,52 +311,96 @@ __global__ void __launch_bounds__(kReduceThreads) gemm_splitk_reduce(Batch bt, i
   *c = (bt.beta == 0.0) ? v : v + bt.beta * (*c);
 }
 
-// Skinny products (m <= 16 rows, k <= 128): C is a few rows over a long n
-// (a site tensor applied to a whole environment tensor). One thread per output
-// column keeps its m accumulators in registers and streams its column of B;
-// A is staged once per workgroup in LDS and read as broadcasts. HBM bound:
-// B is read once, C written once, both coalesced when their columns are
-// contiguous.
-constexpr int kSkinnyM = 16;
-constexpr int kSkinnyK = 128;
-constexpr int kSkinnyThreads = 256;
+// Wide products (m <= 128 rows, k <= 128, n long): C is a few rows over a very
+// long n, e.g. a site tensor applied to a whole environment tensor. B is read
+// once and C written once, with 2k flop per output element: both the HBM
+// stream and the matrix cores are near their limits together, so the kernel
+// keeps per-element overhead low by doing the arithmetic on FP64 MFMA
+// (v_mfma_f64_16x16x4f64) with the whole of m in one workgroup. A workgroup
+// stages all of A in LDS once (k-major, K padded to a multiple of 4, row
+// stride MT*16 + 16 so the 4 k rows of one operand read fall in disjoint bank
+// halves), then walks kWideSlabs slabs of 64 consecutive columns; in a slab
+// wave w owns column tile w (16 columns) and all MT row tiles of 16 rows. Each
+// wave reads its B operand straight from global memory (16 lanes read 16
+// consecutive doubles of one K row), eight K steps' loads issued before their
+// MFMAs. K rows past k read a clamped row of B against zero rows of A. C is
+// written from the MFMA layout: each store covers four rows of 16
+// consecutive doubles.
+constexpr int kWideThreads = 256;
+constexpr int kWideMaxM = 128;  // 8 row tiles
+constexpr int kWideMaxK = 128;
+constexpr int kWideCols = 64;   // columns per slab (4 waves x 16)
+constexpr int kWideSlabs = 4;   // slabs per workgroup (A staged once for all)
+constexpr int kWideLds = 64 * 1024;
+constexpr int kWideStep = 8;    // K steps whose B loads are issued together
 
-struct SkinnyBatch {
+struct WideBatch {
   double alpha, beta;
   int32_t ngroups, pad;
   int32_t block_prefix[kMaxGroups + 1];
   GroupDev g[kMaxGroups];
 };
 
-__global__ void __launch_bounds__(kSkinnyThreads) gemm_skinny_kernel(SkinnyBatch bt) {
-  __shared__ double As[kSkinnyM * kSkinnyK];
+constexpr int wide_lds_stride(int mt) { return mt * 16 + 16; }
+
+template <int MT>
+__global__ void __launch_bounds__(kWideThreads) gemm_wide_kernel(WideBatch bt) {
+  constexpr int MR = MT * 16;
+  constexpr int LS = wide_lds_stride(MT);
+  extern __shared__ double As[];  // kpad x LS
   int gi = 0;
   while (gi + 1 < bt.ngroups && bt.block_prefix[gi + 1] <= static_cast<int>(blockIdx.x)) ++gi;
   const GroupDev &G = bt.g[gi];
   const int m = G.m, k = G.k;
-  for (int e = threadIdx.x; e < m * k; e += kSkinnyThreads) {
-    const int i = e / k, kk = e - i * k;
-    As[i * kSkinnyK + kk] = G.a[static_cast<int64_t>(i) * G.a_rs + static_cast<int64_t>(kk) * G.a_cs];
+  const int kpad = (k + 3) & ~3;
+  for (int e = threadIdx.x; e < kpad * MR; e += kWideThreads) {
+    const int kk = e / MR, i = e - kk * MR;
+    As[kk * LS + i] = (i < m && kk < k)
+                          ? G.a[static_cast<int64_t>(i) * G.a_rs + static_cast<int64_t>(kk) * G.a_cs]
+                          : 0.0;
   }
   __syncthreads();
-  const int64_t j = static_cast<int64_t>(blockIdx.x - bt.block_prefix[gi]) * kSkinnyThreads + threadIdx.x;
-  if (j >= G.n) return;
-  double acc[kSkinnyM];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nsteps = kpad / 4;
+  const double *arow0 = &As[(lane >> 4) * LS + (lane & 15)];
+  const int64_t slab0 = static_cast<int64_t>(blockIdx.x - bt.block_prefix[gi]) * kWideSlabs;
+  for (int sl = 0; sl < kWideSlabs; ++sl) {
+    const int64_t j0 = (slab0 + sl) * kWideCols;
+    if (j0 >= G.n) break;  // uniform
+    const int64_t j = j0 + wave * 16 + (lane & 15);
+    const double *__restrict__ bcol = G.b + min(j, static_cast<int64_t>(G.n) - 1) * G.b_cs;  // clamped column
+    d4 acc[MT];
 #pragma unroll
-  for (int i = 0; i < kSkinnyM; ++i) acc[i] = 0.0;
-  const double *bcol = G.b + j * G.b_cs;
-  for (int kk = 0; kk < k; ++kk) {
-    const double b = bcol[static_cast<int64_t>(kk) * G.b_rs];
+    for (int r = 0; r < MT; ++r) acc[r] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int s0 = 0; s0 < nsteps; s0 += kWideStep) {
+      double bv[kWideStep];
 #pragma unroll
-    for (int i = 0; i < kSkinnyM; ++i)
-      if (i < m) acc[i] = fma(As[i * kSkinnyK + kk], b, acc[i]);
-  }
+      for (int u = 0; u < kWideStep; ++u)
+        bv[u] = bcol[static_cast<int64_t>(min(4 * (s0 + u) + (lane >> 4), k - 1)) * G.b_rs];
 #pragma unroll
-  for (int i = 0; i < kSkinnyM; ++i) {
-    if (i < m) {
-      double *c = G.c + static_cast<int64_t>(i) * G.c_ld + j;
-      const double v = bt.alpha * acc[i];
-      *c = (bt.beta == 0.0) ? v : v + bt.beta * (*c);
+      for (int u = 0; u < kWideStep; ++u) {
+        if (s0 + u < nsteps) {  // uniform
+          const double *arow = arow0 + 4 * (s0 + u) * LS;
+#pragma unroll
+          for (int r = 0; r < MT; ++r)
+            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[16 * r], bv[u], acc[r], 0, 0, 0);
+        }
+      }
+    }
+    if (j < G.n) {
+      // lane holds D[lane / 16 + 4 q][lane % 16] of each 16 x 16 tile
+#pragma unroll
+      for (int r = 0; r < MT; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int i = 16 * r + (lane >> 4) + 4 * q;
+          if (i < m) {
+            double *c = G.c + static_cast<int64_t>(i) * G.c_ld + j;
+            const double v = bt.alpha * acc[r][q];
+            *c = (bt.beta == 0.0) ? v : v + bt.beta * (*c);
+          }
+        }
     }
   }
 }
@@ -680,12 +724,43 @@ void launch_tiles(const TileShape &t, dim3 grid, hipStream_t stream, const Batch
 
 thread_local int last_dispatches = 0;
 
-inline bool is_skinny(const fs_gemm_group &q) {
+// Wide-kernel instance of a group (row tiles MT = 1..8 as index MT - 1), -1
+// for the tile kernel: few rows and a short K over a long n, with B
+// row-major (the 16 lanes of a K row read 16 consecutive columns) and A
+// fitting the LDS. FS_GEMM_WIDE=0 disables it (tuning only).
+constexpr int kNumWide = 8;
+
+inline int wide_lds_bytes(int mt, int64_t k) { return static_cast<int>(((k + 3) & ~3) * wide_lds_stride(mt) * 8); }
+
+inline int wide_instance(const fs_gemm_group &q) {
   static const bool off = [] {
-    const char *e = std::getenv("FS_GEMM_SKINNY");
+    const char *e = std::getenv("FS_GEMM_WIDE");
     return e && std::atoi(e) == 0;
   }();
-  return !off && q.m > 0 && q.m <= kSkinnyM && q.k <= kSkinnyK && q.n >= kSkinnyThreads;
+  if (off || q.m <= 0 || q.m > kWideMaxM || q.k <= 0 || q.k > kWideMaxK || q.b_cs != 1) return -1;
+  // at least 16 slabs of columns (the tile kernel takes shorter products)
+  if (q.n < 16 * kWideCols) return -1;
+  const int mt = static_cast<int>((q.m + 15) / 16);
+  if (wide_lds_bytes(mt, q.k) > kWideLds) return -1;
+  return mt - 1;
+}
+
+template <int MT>
+void launch_wide_mt(dim3 grid, int lds, hipStream_t stream, const WideBatch &wb) {
+  hipLaunchKernelGGL((gemm_wide_kernel<MT>), grid, dim3(kWideThreads), lds, stream, wb);
+}
+
+void launch_wide(int w, dim3 grid, int lds, hipStream_t stream, const WideBatch &wb) {
+  switch (w) {
+    case 0: launch_wide_mt<1>(grid, lds, stream, wb); break;
+    case 1: launch_wide_mt<2>(grid, lds, stream, wb); break;
+    case 2: launch_wide_mt<3>(grid, lds, stream, wb); break;
+    case 3: launch_wide_mt<4>(grid, lds, stream, wb); break;
+    case 4: launch_wide_mt<5>(grid, lds, stream, wb); break;
+    case 5: launch_wide_mt<6>(grid, lds, stream, wb); break;
+    case 6: launch_wide_mt<7>(grid, lds, stream, wb); break;
+    default: launch_wide_mt<8>(grid, lds, stream, wb); break;
+  }
 }
 
 // Fill one device group descriptor (operand bases already resolved).
@@ -712,9 +787,9 @@ int fill_group(GroupDev &d, const fs_gemm_group &q, const double *a_base, const 
 }
 
 // A, B, C: per-group operand bases (stride 1 for one shared base, 0 entries
-// otherwise unused); group offsets are relative to them. Skinny groups (few
-// rows over a long n) go to the streaming kernel, the others to the MFMA tile
-// kernel; each launch takes at most kMaxGroups groups.
+// otherwise unused); group offsets are relative to them. Wide groups (few
+// rows over a long n) go to the streaming kernel of their row count, the
+// others to the MFMA tile kernel; each launch takes at most kMaxGroups groups.
 template <bool SPLIT>
 int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const *A,
                    const double *const *B, double *const *C, int base_stride, double alpha,
@@ -722,33 +797,38 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
   FS_CHECK_ARG(ngroups >= 0, "grouped gemm: negative group count");
   FS_CHECK_ARG(ngroups == 0 || groups, "grouped gemm: null groups");
   int dispatches = 0;
-  std::vector<int> tiled, skinny;
+  std::vector<int> tiled, wide[kNumWide];
   for (int i = 0; i < ngroups; ++i) {
     const fs_gemm_group &q = groups[i];
     FS_CHECK_ARG(q.m >= 0 && q.n >= 0 && q.k >= 0, "grouped gemm: negative dimension");
     if (q.m == 0 || q.n == 0) continue;
-    (!SPLIT && is_skinny(q) ? skinny : tiled).push_back(i);
+    const int w = SPLIT ? -1 : wide_instance(q);
+    (w >= 0 ? wide[w] : tiled).push_back(i);
   }
-  for (size_t c0 = 0; c0 < skinny.size(); c0 += kMaxGroups) {
-    SkinnyBatch sb;
-    std::memset(&sb, 0, sizeof(sb));
-    sb.alpha = alpha;
-    sb.beta = beta;
-    int ng = 0;
-    int64_t blocks = 0;
-    for (size_t t = c0; t < std::min(skinny.size(), c0 + kMaxGroups); ++t) {
-      const int i = skinny[t];
-      int st = fill_group(sb.g[ng], groups[i], A[i * base_stride], B[i * base_stride], C[i * base_stride]);
-      if (st != FS_OK) return st;
-      blocks += fs::ceil_div(groups[i].n, kSkinnyThreads);
-      FS_CHECK_ARG(blocks < (1ll << 31), "grouped gemm: too many blocks");
-      sb.block_prefix[++ng] = static_cast<int32_t>(blocks);
+  for (int w = 0; w < kNumWide; ++w) {
+    const int nc = kWideCols * kWideSlabs;  // columns per workgroup
+    int lds = 0;
+    for (size_t c0 = 0; c0 < wide[w].size(); c0 += kMaxGroups) {
+      WideBatch wb;
+      std::memset(&wb, 0, sizeof(wb));
+      wb.alpha = alpha;
+      wb.beta = beta;
+      int ng = 0;
+      int64_t blocks = 0;
+      for (size_t t = c0; t < std::min(wide[w].size(), c0 + kMaxGroups); ++t) {
+        const int i = wide[w][t];
+        int st = fill_group(wb.g[ng], groups[i], A[i * base_stride], B[i * base_stride], C[i * base_stride]);
+        if (st != FS_OK) return st;
+        blocks += fs::ceil_div(groups[i].n, nc);
+        FS_CHECK_ARG(blocks < (1ll << 31), "grouped gemm: too many blocks");
+        lds = std::max(lds, wide_lds_bytes(w + 1, groups[i].k));
+        wb.block_prefix[++ng] = static_cast<int32_t>(blocks);
+      }
+      wb.ngroups = ng;
+      launch_wide(w, dim3(static_cast<unsigned>(blocks)), lds, fs::as_stream(stream), wb);
+      FS_LAUNCH_CHECK("gemm_wide_kernel");
+      ++dispatches;
     }
-    sb.ngroups = ng;
-    hipLaunchKernelGGL(gemm_skinny_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kSkinnyThreads), 0,
-                       fs::as_stream(stream), sb);
-    FS_LAUNCH_CHECK("gemm_skinny_kernel");
-    ++dispatches;
   }
   for (size_t c0 = 0; c0 < tiled.size(); c0 += kMaxGroups) {
     const size_t c1 = std::min(tiled.size(), c0 + kMaxGroups);

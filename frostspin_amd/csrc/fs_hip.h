@@ -28,6 +28,7 @@
 namespace fs {
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
 // Device copy of a plan's read-only tables. Neither the upload nor the release
 // synchronises a compute stream: the memory comes from the device's

@@ -53,7 +53,7 @@ std::vector<int8_t> fuse_charges(const std::vector<const int8_t *> &legs,
   return acc;
 }
 
-// ---- abelian permute as box tiles --------------------------------------------
+// ---- abelian permute as boxes ------------------------------------------------
 //
 // The reference permutes coefficient by coefficient (_numba_abelian_transpose,
 // abelian_symmetric_tensor.py:103-250): old (block, row, col) -> dense index ->
@@ -180,109 +180,157 @@ uint32_t magic_of(int64_t len) {
                                static_cast<uint64_t>(len));
 }
 
-// Tiles of one box: digits given in destination (new leg) order. Slices the
-// slowest destination digits until a tile holds at most kPermuteTileVol
-// elements, then builds each tile's source-order and destination-order digit
-// lists with contiguous neighbours merged.
-void box_tiles(const std::vector<Digit> &dig0, int64_t src_base, int64_t dst_base, int64_t src_limit,
-               int64_t dst_limit, std::vector<PermTile> &out) {
-  std::vector<Digit> dig;
-  for (const Digit &g : dig0)
-    if (g.len > 1) dig.push_back(g);
-  int64_t vol = 1;
-  for (const Digit &g : dig) vol *= g.len;
-  // chunk of each digit along which tiles are cut (slowest destination digits first)
-  std::vector<int64_t> chunk(dig.size());
-  int64_t rest = vol;
-  bool done = rest <= kPermuteTileVol;
-  for (size_t i = 0; i < dig.size(); ++i) {
-    if (done) {
-      chunk[i] = dig[i].len;
-      continue;
-    }
-    rest /= dig[i].len;
-    if (rest <= kPermuteTileVol) {
-      chunk[i] = std::max<int64_t>(1, kPermuteTileVol / rest);
-      done = true;
-    } else {
-      chunk[i] = 1;
-    }
-  }
-  std::vector<int64_t> nchunks(dig.size()), at(dig.size(), 0);
-  int64_t ntiles = 1;
-  for (size_t i = 0; i < dig.size(); ++i) {
-    nchunks[i] = (dig[i].len + chunk[i] - 1) / chunk[i];
-    ntiles *= nchunks[i];
-  }
-  for (int64_t t = 0; t < ntiles; ++t) {
-    int64_t rem = t;
-    for (size_t i = dig.size(); i-- > 0;) {
-      at[i] = rem % nchunks[i];
-      rem /= nchunks[i];
-    }
-    std::vector<Digit> td = dig;
-    int64_t sb = src_base, db = dst_base, tvol = 1, slast = 0, dlast = 0;
-    for (size_t i = 0; i < td.size(); ++i) {
-      const int64_t st = at[i] * chunk[i];
-      td[i].len = std::min(chunk[i], dig[i].len - st);
-      sb += st * dig[i].s;
-      db += st * dig[i].d;
-      tvol *= td[i].len;
-      slast += (td[i].len - 1) * dig[i].s;
-      dlast += (td[i].len - 1) * dig[i].d;
-    }
-    if (src_base >= 0 && (sb < 0 || sb + slast >= src_limit))
-      throw std::invalid_argument("permute plan: tile reads outside the source");
-    if (db < 0 || db + dlast >= dst_limit)
-      throw std::invalid_argument("permute plan: tile writes outside the destination");
-    if (slast >= (int64_t(1) << 31) || dlast >= (int64_t(1) << 31))
-      throw std::invalid_argument("permute plan: block too large for 32-bit in-tile offsets");
-    // staging offsets: the tile in destination order, row-major
-    int64_t lrun = 1;
-    for (size_t i = td.size(); i-- > 0;) {
-      td[i].l = lrun;
-      lrun *= td[i].len;
-    }
-    std::vector<Digit> dd;  // destination digits, fastest first, contiguous ones merged
-    for (size_t i = td.size(); i-- > 0;) {
-      if (td[i].len == 1) continue;
-      if (!dd.empty() && td[i].d == dd.back().len * dd.back().d) dd.back().len *= td[i].len;
-      else dd.push_back(td[i]);
-    }
-    std::vector<Digit> sd;  // source digits by increasing source stride, merged in both orders
-    for (const Digit &g : td)
-      if (g.len > 1) sd.push_back(g);
-    std::stable_sort(sd.begin(), sd.end(), [](const Digit &a, const Digit &b) { return a.s < b.s; });
-    std::vector<Digit> sm;
-    for (const Digit &g : sd) {
-      if (!sm.empty() && g.s == sm.back().len * sm.back().s && g.l == sm.back().len * sm.back().l)
-        sm.back().len *= g.len;
-      else sm.push_back(g);
-    }
-    if (sm.size() > static_cast<size_t>(kPermuteMaxDigits) || dd.size() > static_cast<size_t>(kPermuteMaxDigits))
-      throw std::invalid_argument("permute plan: too many digits in a tile");
-    PermTile pt;
-    std::memset(&pt, 0, sizeof(pt));
-    pt.src_base = src_base >= 0 ? sb : -1;
-    pt.dst_base = db;
-    pt.vol = static_cast<int32_t>(tvol);
-    pt.ns = static_cast<int32_t>(sm.size());
-    pt.nd = static_cast<int32_t>(dd.size());
-    for (size_t k = 0; k < sm.size(); ++k) {
-      pt.slen[k] = static_cast<int32_t>(sm[k].len);
-      pt.smag[k] = magic_of(sm[k].len);
-      pt.sstr[k] = static_cast<int32_t>(sm[k].s);
-      pt.lstr[k] = static_cast<int32_t>(sm[k].l);
-    }
-    for (size_t k = 0; k < dd.size(); ++k) {
-      pt.dlen[k] = static_cast<int32_t>(dd[k].len);
-      pt.dmag[k] = magic_of(dd[k].len);
-      pt.dstr[k] = static_cast<int32_t>(dd[k].d);
-    }
-    out.push_back(pt);
-  }
-}
+// Record of one box: digits given in destination (new leg) order, slowest
+// first. The slowest digits are kept out of the tiles (one index per tile)
+// until the remaining volume fits kPermuteTileVol; the next digit is cut in
+// chunks of kPermuteTileVol / (volume of the faster digits) when that is at
+// least 2 (else it is a slow digit too). The tile's destination-order list
+// (fastest first, contiguous neighbours merged) ends with the cut digit; the
+// source-order list sorts the digits by source stride and merges neighbours
+// contiguous in both the source and the staging buffer, never onto an entry
+// holding a cut digit (its length differs in the last chunk). Checks the whole
+// box against both buffers and the 32-bit in-tile offsets.
+constexpr int kK = kPermuteMaxDigits;
 
+void make_box(const Digit *dig0, int ndig0, int64_t src_base, int64_t dst_base, int64_t src_limit,
+              int64_t dst_limit, PermBox &b) {
+  Digit dig[kK];
+  int n = 0;
+  for (int i = 0; i < ndig0; ++i)
+    if (dig0[i].len > 1) {
+      if (n == kK) throw std::invalid_argument("permute plan: too many digits in a box");
+      dig[n++] = dig0[i];
+    }
+  int64_t vol = 1, slast = 0, dlast = 0;
+  for (int i = 0; i < n; ++i) {
+    vol *= dig[i].len;
+    slast += (dig[i].len - 1) * dig[i].s;
+    dlast += (dig[i].len - 1) * dig[i].d;
+  }
+  if (src_base >= 0 && src_base + slast >= src_limit)
+    throw std::invalid_argument("permute plan: box reads outside the source");
+  if (dst_base < 0 || dst_base + dlast >= dst_limit)
+    throw std::invalid_argument("permute plan: box writes outside the destination");
+  std::memset(&b, 0, sizeof(b));
+  b.src_base = src_base >= 0 ? src_base : -1;
+  b.dst_base = dst_base;
+  b.s_cut = -1;
+  // slow digits and cut digit
+  int nslow = 0, cut = -1;
+  int64_t chunk = 0;
+  if (vol > kPermuteTileVol) {
+    int64_t rest = vol;
+    for (int i = 0; i < n; ++i) {
+      rest /= dig[i].len;
+      if (rest <= kPermuteTileVol) {
+        const int64_t c = kPermuteTileVol / rest;  // < dig[i].len
+        if (c >= 2) {
+          cut = i;
+          chunk = c;
+          nslow = i;
+        } else {
+          nslow = i + 1;
+        }
+        break;
+      }
+    }
+  }
+  b.nslow = nslow;
+  for (int k = 0; k < nslow; ++k) {  // fastest slow digit first
+    const Digit &g = dig[nslow - 1 - k];
+    b.slow_len[k] = static_cast<int32_t>(g.len);
+    b.slow_s[k] = g.s;
+    b.slow_d[k] = g.d;
+  }
+  // tile digits (destination order, slowest first), the cut one at its chunk length
+  Digit td[kK];
+  int nt = 0;
+  for (int i = nslow; i < n; ++i) {
+    td[nt] = dig[i];
+    if (i == cut) td[nt].len = chunk;
+    ++nt;
+  }
+  if (cut >= 0) {
+    b.nchunks = static_cast<int32_t>((dig[cut].len + chunk - 1) / chunk);
+    b.chunk = static_cast<int32_t>(chunk);
+    b.cut_s = dig[cut].s;
+    b.cut_d = dig[cut].d;
+  } else {
+    b.nchunks = 1;
+    b.chunk = 0;
+  }
+  const int64_t edge = cut >= 0 ? dig[cut].len - (b.nchunks - 1) * chunk : 0;  // last chunk's length
+  int64_t tvol = 1, tslast = 0, tdlast = 0;
+  for (int i = nt; i-- > 0;) {
+    td[i].l = tvol;  // staging: destination order, row-major
+    tvol *= td[i].len;
+    tslast += (td[i].len - 1) * td[i].s;
+    tdlast += (td[i].len - 1) * td[i].d;
+  }
+  if (tslast >= (int64_t(1) << 31) || tdlast >= (int64_t(1) << 31))
+    throw std::invalid_argument("permute plan: block too large for 32-bit in-tile offsets");
+  b.vol = static_cast<int32_t>(tvol);
+  b.vol_edge = cut >= 0 ? static_cast<int32_t>(tvol / chunk * edge) : b.vol;
+  // destination list, fastest first; the cut digit (td[0] when cut) is the last entry
+  int nd = 0;
+  int64_t dl[kK], ds[kK];
+  for (int i = nt; i-- > 0;) {
+    if (nd > 0 && td[i].d == dl[nd - 1] * ds[nd - 1]) {
+      dl[nd - 1] *= td[i].len;
+    } else {
+      dl[nd] = td[i].len;
+      ds[nd] = td[i].d;
+      ++nd;
+    }
+  }
+  // source list: by increasing source stride (stable), merged where contiguous
+  int order[kK];
+  for (int i = 0; i < nt; ++i) order[i] = i;
+  for (int i = 1; i < nt; ++i)  // insertion sort: stable, at most kK entries
+    for (int j = i; j > 0 && td[order[j]].s < td[order[j - 1]].s; --j) std::swap(order[j], order[j - 1]);
+  int ns = 0;
+  int64_t sl[kK], ss[kK], sls[kK], sfac[kK];
+  bool scut[kK];
+  for (int q = 0; q < nt; ++q) {
+    const Digit &g = td[order[q]];
+    const bool is_cut = cut >= 0 && order[q] == 0;
+    if (ns > 0 && !scut[ns - 1] && g.s == sl[ns - 1] * ss[ns - 1] && g.l == sl[ns - 1] * sls[ns - 1]) {
+      sl[ns - 1] *= g.len;
+      if (is_cut) scut[ns - 1] = true;
+      else sfac[ns - 1] *= g.len;
+    } else {
+      sl[ns] = g.len;
+      ss[ns] = g.s;
+      sls[ns] = g.l;
+      scut[ns] = is_cut;
+      sfac[ns] = is_cut ? 1 : g.len;
+      ++ns;
+    }
+  }
+  b.ns = ns;
+  b.nd = nd;
+  for (int k = 0; k < ns; ++k) {
+    b.slen[k] = static_cast<int32_t>(sl[k]);
+    b.smag[k] = magic_of(sl[k]);
+    b.sstr[k] = static_cast<int32_t>(ss[k]);
+    b.lstr[k] = static_cast<int32_t>(sls[k]);
+    // the last chunk changes the length of the entry holding the cut digit
+    if (scut[k] && edge != chunk) {
+      b.s_cut = k;
+      b.s_len_edge = static_cast<int32_t>(sfac[k] * edge);
+      b.s_drop_edge = b.s_len_edge == 1;
+      b.s_mag_edge = b.s_len_edge > 1 ? magic_of(b.s_len_edge) : 0u;
+    }
+  }
+  for (int k = 0; k < nd; ++k) {
+    b.dlen[k] = static_cast<int32_t>(dl[k]);
+    b.dmag[k] = magic_of(dl[k]);
+    b.dstr[k] = static_cast<int32_t>(ds[k]);
+  }
+  // the cut digit is in the last destination entry (its length is not used
+  // to decode, the tile volume bounds the walk; kept exact for the record)
+  b.d_len_edge = cut >= 0 ? static_cast<int32_t>(dl[nd - 1] / chunk * edge) : b.dlen[nd > 0 ? nd - 1 : 0];
+}
 }  // namespace
 
 fs_permute_plan *build_abelian_permute_plan(
@@ -436,10 +484,28 @@ fs_permute_plan *build_abelian_permute_plan(
         else g.d = ncol.stride[ic * knc + (i - new_nrr)];
         g.l = 0;
       }
-      box_tiles(dig, src_base, dst_base, src_limit, std::max<int64_t>(noff, 1), plan->tiles);
-      ++plan->nboxes;
+      plan->boxes.emplace_back();
+      make_box(dig.data(), ndim, src_base, dst_base, src_limit, std::max<int64_t>(noff, 1), plan->boxes.back());
     }
   }
+  // tiles: per box nchunks x (product of the slow lengths), numbered box by box
+  const size_t nb = plan->boxes.size();
+  plan->tile0.resize(nb + 1);
+  int64_t nt = 0;
+  for (size_t i = 0; i < nb; ++i) {
+    const PermBox &bx = plan->boxes[i];
+    int64_t t = bx.nchunks;
+    for (int k = 0; k < bx.nslow; ++k) t *= bx.slow_len[k];
+    plan->tile0[i] = static_cast<int32_t>(nt);
+    nt += t;
+    if (nt >= (int64_t(1) << 31)) throw std::invalid_argument("permute plan: too many tiles");
+  }
+  plan->tile0[nb] = static_cast<int32_t>(nt);
+  plan->ntiles = nt;
+  plan->tile_box.resize(static_cast<size_t>(nt));
+  for (size_t i = 0; i < nb; ++i)
+    std::fill(plan->tile_box.begin() + plan->tile0[i], plan->tile_box.begin() + plan->tile0[i + 1],
+              static_cast<int32_t>(i));
   return plan.release();
 }
 
@@ -577,38 +643,6 @@ int fs_permute_plan_blocks(const fs_permute_plan *plan, int8_t *irreps,
     if (irreps) std::memcpy(irreps, plan->new_irreps.data(), plan->new_irreps.size());
     if (shapes)
       std::memcpy(shapes, plan->new_shapes.data(), plan->new_shapes.size() * sizeof(int64_t));
-    return FS_OK;
-  });
-}
-
-int fs_permute_plan_tiles(const fs_permute_plan *plan, int64_t *ntiles, int64_t *nboxes,
-                          int64_t *out) {
-  return fs::guarded([&]() -> int {
-    FS_CHECK_ARG(plan && ntiles && nboxes, "fs_permute_plan_tiles: null argument");
-    *ntiles = static_cast<int64_t>(plan->tiles.size());
-    *nboxes = plan->nboxes;
-    if (!out) return FS_OK;
-    constexpr int K = kPermuteMaxDigits;
-    static_assert(FS_PERMUTE_TILE_RECORD == 5 + 5 * K, "tile record layout");
-    for (const PermTile &t : plan->tiles) {
-      int64_t *r = out;
-      std::memset(r, 0, FS_PERMUTE_TILE_RECORD * sizeof(int64_t));
-      r[0] = t.src_base;
-      r[1] = t.dst_base;
-      r[2] = t.vol;
-      r[3] = t.ns;
-      r[4] = t.nd;
-      for (int k = 0; k < t.ns; ++k) {
-        r[5 + 3 * k] = t.slen[k];
-        r[6 + 3 * k] = t.sstr[k];
-        r[7 + 3 * k] = t.lstr[k];
-      }
-      for (int k = 0; k < t.nd; ++k) {
-        r[5 + 3 * K + 2 * k] = t.dlen[k];
-        r[6 + 3 * K + 2 * k] = t.dstr[k];
-      }
-      out += FS_PERMUTE_TILE_RECORD;
-    }
     return FS_OK;
   });
 }

@@ -140,12 +140,24 @@ def _read_table(path):
 def load_su2_cg(cg_file=None, *, verbosity=0):
     """Load a Clebsch-Gordan table in the format of su2_clebsch_gordan.py:35-109
     (JSON sparse lists or npz 'i_/c_' arrays). Loaded tensors replace the computed
-    ones; dimensions outside the table are still computed. Without a file, the
-    computed coefficients are used alone. Returns the loaded table."""
+    ones; dimensions outside the table are still computed. cg_file None takes
+    config["SU2_CG_file"] (the --frostspin-SU2-CG-file option, as in the
+    reference); when that is unset too, the computed coefficients are used
+    alone. A file that cannot be read raises FileNotFoundError. Returns the
+    loaded table."""
+    from ..config import config
+
     _loaded_table.clear()
     if cg_file is None:
+        cg_file = config["SU2_CG_file"]
+    if cg_file is None:
         return {}
-    table = _read_table(cg_file)
+    cg_file = str(cg_file)
+    try:
+        table = _read_table(cg_file)
+    except FileNotFoundError:
+        msg = f"Error: cannot read SU(2) Clebsch-Gordan coefficient savefile {cg_file}"
+        raise FileNotFoundError(msg) from None
     for k, v in table.items():
         v.flags.writeable = False
         _loaded_table[k] = v
@@ -182,3 +194,8 @@ def save_su2_cg(savefile, max_spin_dim):
             json.dump(data, out, indent=2, sort_keys=True)
     else:
         raise RuntimeError("Invalid file extension: must be json or npz")
+
+
+# a table named on the command line (--frostspin-SU2-CG-file) is read once at
+# import, like the reference's class attribute (su2_symmetric_tensor.py:140)
+load_su2_cg()

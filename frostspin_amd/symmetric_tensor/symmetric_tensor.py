@@ -276,7 +276,24 @@ class SymmetricTensor:
 
     @property
     def blocks(self):
-        """Torch views (device) of each block with its logical shape."""
+        """Host numpy arrays of the blocks (symmetric_tensor.py:320 returns
+        numpy arrays): one device-to-host copy of the whole buffer, so writing
+        into them does not change the tensor. Device views: device_blocks."""
+        if self.nblocks == 0:
+            return ()
+        host = _ops.download(self._data)
+        if not self._data.is_cuda:
+            host = host.copy()  # a host buffer would otherwise be shared
+        out = []
+        for (m, n), off in zip(self._bshapes, self._boffs, strict=True):
+            seg = host[int(off) : int(off) + int(m * n)]
+            out.append(seg.reshape(int(n), int(m)).T if self._fortran else seg.reshape(int(m), int(n)))
+        return tuple(out)
+
+    @property
+    def device_blocks(self):
+        """Torch views (device) of each block with its logical shape; writes go
+        to the tensor's own buffer."""
         out = []
         for (m, n), off in zip(self._bshapes, self._boffs, strict=True):
             seg = self._data[int(off) : int(off) + int(m * n)]
@@ -894,31 +911,36 @@ class SymmetricTensor:
             raise ValueError("Tensor legs do not define a square matrix")
         return self.truncated_eigh(2**31, compute_vectors=compute_vectors)
 
+    # Taylor degree of expm: with the scaled matrix's norm <= 1/2, the series
+    # remainder is below e^(1/2) 0.5^17 / 17! ~ 2e-20 of the norm.
+    _EXPM_DEGREE = 16
+
     def expm(self):
-        """exp of each (real symmetric) block via its eigen decomposition."""
+        """exp of every block (symmetric_tensor.py:818-824 calls scipy's expm
+        on each block; missing blocks stay missing). Scaling and squaring of the
+        truncated Taylor series (Moler & Van Loan, "Nineteen dubious ways",
+        method 3): A is scaled by 2^-s until its norm is at most 1/2, the
+        degree-16 series is evaluated by Horner's rule, T = I + A T / k for
+        k = 16 .. 1, and squared s times. Every product is one grouped GEMM on
+        the device, so general (non-symmetric) blocks need no host round trip.
+        The tensor norm bounds every block's spectral norm."""
         if not self.is_square_matrix():
             raise ValueError("Tensor legs do not define a square matrix")
-        for b in self.blocks:
-            bh = b.detach().cpu().numpy()
-            if not np.allclose(bh, bh.T, rtol=0, atol=1e-12 * max(1.0, np.abs(bh).max(initial=0.0))):
-                raise NotImplementedError("device expm requires symmetric blocks")
-        # keep every eigenvalue, zeros included (exp(0) = 1 must survive)
-        s, u = self.truncated_eigh(2**31, atol=-1.0, rtol=0.0)
-        es = DiagonalTensor(
-            tuple(np.exp(db) for db in s.diagonal_blocks), s.representation, s.block_irreps,
-            s.block_degen, s.symmetry(),
+        if self.nblocks == 0:
+            return self.copy()
+        eye = type(self)(
+            self._row_reps, self._col_reps, [np.eye(int(m)) for m, _ in self._bshapes],
+            self._block_irreps, self._signature,
         )
-        res = (u * es) @ u.dagger()
-        # blocks whose eigenvalues were all cut (exact zero matrix) exponentiate to I
-        full_irreps, full_shapes = self.get_block_sizes(self._row_reps, self._col_reps, self._signature)
-        if res.nblocks == len(full_irreps):
-            return res
-        blocks = []
-        have = {int(r): i for i, r in enumerate(res.block_irreps)}
-        res_blocks = res.blocks
-        for r, sh in zip(full_irreps, full_shapes, strict=True):
-            blocks.append(res_blocks[have[int(r)]] if int(r) in have else torch.eye(int(sh[0]), dtype=_ops.DTYPE))
-        return type(self)(self._row_reps, self._col_reps, blocks, full_irreps, self._signature)
+        nrm = self.norm()
+        s = max(0, int(np.ceil(np.log2(nrm / 0.5)))) if nrm > 0 else 0
+        a = self._affine(2.0**-s)
+        t = eye
+        for k in range(self._EXPM_DEGREE, 0, -1):
+            t = eye + (a @ t)._affine(1.0 / k)
+        for _ in range(s):
+            t = t @ t
+        return t
 
     def solve(self, b):
         raise NotImplementedError("solve is not provided by the device engine")
@@ -1009,11 +1031,12 @@ class SymmetricTensor:
                 eyes = [np.eye(int(shapes[b, 0])) for b in full]
                 st = matmat(type(self)(self._row_reps, (rfull,), eyes, irr_full, sigv))
             present = {int(r): j for j, r in enumerate(st.block_irreps)}
+            host_blocks = st.blocks  # one readback for every dense block
             for bi in full:
                 j = present.get(int(irreps[bi]))
                 if j is None:
                     continue  # missing block: zero eigenvalues, truncated
-                b = _ops.download(st.blocks[j].contiguous())
+                b = np.ascontiguousarray(host_blocks[j])
                 if symmetric:
                     lam, vec = np.linalg.eigh(b) if compute_vectors else (np.linalg.eigvalsh(b), None)
                 else:
@@ -1139,7 +1162,7 @@ class SymmetricTensor:
         for ci, c in enumerate(self._col_reps):
             data[f"{prefix}_col_rep_{ci}"] = c
         for bi, b in enumerate(self.blocks):
-            data[f"{prefix}_block_{bi}"] = np.ascontiguousarray(b.detach().cpu().numpy())
+            data[f"{prefix}_block_{bi}"] = np.ascontiguousarray(b)
         return data
 
     @classmethod

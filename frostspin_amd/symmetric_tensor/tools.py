@@ -2,12 +2,11 @@
 
 import numpy as np
 
-ASSERT_TOL = 4e-13
-
 # Structural/norm consistency checks run inside `assert` statements like the
 # reference (disabled by python -O, README.md:46). They cost device reductions
-# and host synchronisations, so they can also be switched off at run time.
-config = {"debug_checks": __debug__}
+# and host synchronisations, so they can also be switched off at run time
+# through config["debug_checks"] (the package-wide configuration dict).
+from ..config import ASSERT_TOL, config  # noqa: F401
 
 symmetric_tensor_types = {}
 

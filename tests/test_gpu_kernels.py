@@ -50,6 +50,44 @@ def test_mfma_gemm_layouts_and_uneven_groups(gpu, sizes):
                 assert err <= 1e-12, f"group {(m, n)} a_f={a_f} b_f={b_f} rel err {err:.2e}"
 
 
+# (m, n, k) of the wide-product kernel family: few rows and a short K over a
+# long n (a site tensor applied to an environment tensor), each row-count
+# instance (16, 32, 64, 128 rows), K chunks of 32 with a ragged last chunk,
+# column counts that end inside a workgroup
+GEMM_SIZES_WIDE = [(5, 3000, 7), (16, 300, 128), (30, 5000, 12), (36, 20001, 12), (64, 9000, 34),
+                   (116, 20000, 34), (34, 20000, 116), (128, 8192, 128), (100, 2100, 40), (1, 4096, 1)]
+
+
+def test_wide_gemm_instances(gpu):
+    rng = np.random.default_rng(5)
+    for a_f in (False, True):
+        for b_f in (False, True):  # column-major B goes to the tile kernel
+            A_parts, B_parts, groups, refs = [], [], [], []
+            ao = bo = co = 0
+            for m, n, k in GEMM_SIZES_WIDE:
+                a = rng.standard_normal((m, k))
+                b = rng.standard_normal((k, n))
+                A_parts.append((a.T if a_f else a).reshape(-1))
+                B_parts.append((b.T if b_f else b).reshape(-1))
+                ars, acs = (1, m) if a_f else (k, 1)
+                brs, bcs = (1, k) if b_f else (n, 1)
+                groups.append((m, n, k, ao, ars, acs, bo, brs, bcs, co, n))
+                refs.append((co, a @ b))
+                ao += m * k
+                bo += k * n
+                co += m * n
+            A = _dev(np.concatenate(A_parts))
+            B = _dev(np.concatenate(B_parts))
+            C = _ops.upload(np.full(co, -3.0))
+            _ops.grouped_gemm(groups, A, B, C, alpha=-1.5, beta=2.0)
+            c = _ops.download(C)
+            for (off, ref), (m, n, _) in zip(refs, GEMM_SIZES_WIDE, strict=True):
+                got = c[off : off + m * n].reshape(m, n)
+                expect = -1.5 * ref - 6.0
+                err = float(np.abs(got - expect).max() / max(1.0, np.abs(expect).max()))
+                assert err <= 1e-12, f"group {(m, n)} a_f={a_f} b_f={b_f} rel err {err:.2e}"
+
+
 def test_gemm_many_groups_chunked(gpu):
     rng = np.random.default_rng(1)
     groups, refs = [], []

@@ -35,7 +35,7 @@ R5 = np.array([[2, 1, 1, 1], [-1, 0, 2, 3]])
 
 
 def hb(t):
-    return [b.detach().cpu().numpy() for b in t.blocks]
+    return list(t.blocks)
 
 
 def check_o2(to2, tu1):
@@ -64,19 +64,19 @@ def test_reference_o2_sequence(backend):
     check_o2(to2, tu1)
     # pathological cases: missing 0odd / 0even / both / everything but them
     t2 = to2.copy()
-    t2.blocks[0][:] = 0
+    t2.device_blocks[0][:] = 0
     t2u1 = t2.toU1()
     t2 = O2SymmetricTensor(t2.row_reps, t2.col_reps, t2.blocks[1:], t2.block_irreps[1:], t2.signature)
     check_o2(t2, t2u1)
     t2 = to2.copy()
-    t2.blocks[1][:] = 0
+    t2.device_blocks[1][:] = 0
     t2u1 = t2.toU1()
     t2 = O2SymmetricTensor(t2.row_reps, t2.col_reps, t2.blocks[:1] + t2.blocks[2:],
                            (-1, *t2.block_irreps[2:]), to2.signature)
     check_o2(t2, t2u1)
     t2 = to2.copy()
-    t2.blocks[0][:] = 0
-    t2.blocks[1][:] = 0
+    t2.device_blocks[0][:] = 0
+    t2.device_blocks[1][:] = 0
     t2u1 = t2.toU1()
     t2 = O2SymmetricTensor(t2.row_reps, t2.col_reps, t2.blocks[2:], t2.block_irreps[2:], t2.signature)
     check_o2(t2, t2u1)

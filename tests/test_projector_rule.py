@@ -69,7 +69,7 @@ def _reference_rule(corners, last, chi, ncv_ratio, block_chi_ratio, degen_ratio,
     c1, c2, c3, c4 = corners
     shared = sorted(set(c1.block_irreps.tolist()) & set(c2.block_irreps.tolist())
                     & set(c3.block_irreps.tolist()) & set(c4.block_irreps.tolist()))
-    bl = [dict(zip(t.block_irreps.tolist(), [b.detach().cpu().numpy() for b in t.blocks], strict=True))
+    bl = [dict(zip(t.block_irreps.tolist(), list(t.blocks), strict=True))
           for t in corners]
     last_cols = dict(zip(last.block_irreps.tolist(), [b.shape[1] for b in last.blocks], strict=True))
     s_list = []

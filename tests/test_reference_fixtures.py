@@ -68,7 +68,7 @@ def _load(name):
 
 
 def _host_blocks(t):
-    return [b.detach().cpu().numpy() for b in t.blocks]
+    return list(t.blocks)
 
 
 def _ref_blocks(data, prefix):
