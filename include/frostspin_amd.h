@@ -110,13 +110,15 @@ int fs_permute_plan_blocks(const fs_permute_plan *plan, int8_t *irreps,
                            int64_t *shapes);
 
 /*
- * The plan is a list of box tiles. A box holds the elements whose index on
- * every leg lies in one run of equal charges of that leg; inside it the old
- * and new in-block positions are affine in the legs' digits, so a tile (a box
- * or a slice of one, at most 4096 elements) is a dense strided copy staged
- * through LDS. Host copy of the tiles (test/inspection): *ntiles and *nboxes
- * always; if out is not NULL it receives ntiles records of
- * FS_PERMUTE_TILE_RECORD int64 each:
+ * The plan is a list of boxes. A box holds the elements whose index on every
+ * leg lies in one run of equal charges of that leg; inside it the old and new
+ * in-block positions are affine in the legs' digits, so a tile (a box or a
+ * slice of one along its slowest destination digits, at most 4096 elements)
+ * is a dense strided copy staged through LDS. The plan stores one record per
+ * box; the kernel derives each tile from its box. Host copy of the tiles as
+ * the kernel derives them (test/inspection): *ntiles and *nboxes always; if
+ * out is not NULL it receives ntiles records of FS_PERMUTE_TILE_RECORD int64
+ * each:
  *   src_base (-1: absent old block, zeros), dst_base, vol, ns, nd,
  *   ns x (len, src stride, staging stride) padded to FS_PERMUTE_MAX_DIGITS,
  *   nd x (len, dst stride) padded to FS_PERMUTE_MAX_DIGITS;
