@@ -483,8 +483,9 @@ def run_ctmrg(args):
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "grouped FP64 GEMM (gemm_f64_kernel tiles, gemm_wide_kernel, split-K reduction), "
-            "all launches of one iteration replayed back to back",
+            "kernel": "grouped FP64 GEMM (gemm_f64_kernel tiles, gemm_wide_kernel, split-K reduction): "
+            "every grouped_gemm call of one iteration replayed back to back (the Gram and Y R^-1 "
+            "products inside the one-call CholeskyQR are timed with it, not here)",
             "achieved": g_rate / 1e12,
             "peak": PEAK_FP64_MFMA / 1e12,
             "unit": "TFLOP/s",
