@@ -25,8 +25,9 @@ constexpr int kPermThreads = 512;
 constexpr int K = kPermuteMaxDigits;
 
 // Elements per thread handled together: their loads are issued back to back
-// before any of them is stored, so each lane keeps several reads in flight.
-constexpr int kPermBatch = 4;
+// before any of them is stored, so each lane keeps several reads in flight
+// (8: a full 4096-element tile in flight per 512-thread workgroup).
+constexpr int kPermBatch = 8;
 
 // One tile of a box, as the kernel walks it (and as fs_permute_plan_tiles
 // reports it).
@@ -89,14 +90,25 @@ __host__ __device__ inline void tile_of_box(const PermBox &b, int32_t local, Til
   }
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
+// Consecutive tiles (neighbouring boxes of one source block, sharing the
+// cache lines at their edges) go to the same XCD: workgroup b runs tile
+// (b mod 8) * (n / 8) + b / 8 (the last n mod 8 tiles in order).
+__device__ inline int32_t xcd_tile(int32_t b, int32_t n) {
+  constexpr int32_t kXcds = 8;
+  const int32_t per = n / kXcds;
+  return b >= per * kXcds ? b : (b % kXcds) * per + b / kXcds;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kPermThreads)
 permute_box_kernel(const T *__restrict__ src, T *__restrict__ dst, const PermBox *__restrict__ boxes,
                    const int32_t *__restrict__ tile0, const int32_t *__restrict__ tile_box) {
   __shared__ T buf[kPermuteTileVol];
-  const int32_t bi = tile_box[blockIdx.x];
+  const int32_t tile = xcd_tile(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+  const int32_t bi = tile_box[tile];
   TileView t;
-  tile_of_box(boxes[bi], static_cast<int32_t>(blockIdx.x) - tile0[bi], t);
+  tile_of_box(boxes[bi], tile - tile0[bi], t);
   const int vol = t.vol;
   T *out = dst + t.dst_base;
   if (t.src_base >= 0) {
