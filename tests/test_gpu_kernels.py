@@ -17,9 +17,19 @@ def _dev(a):
 GEMM_SIZES_SMALL = [(1, 1, 1), (3, 70, 5), (64, 64, 16), (65, 63, 17), (130, 7, 200), (17, 129, 33)]
 # one group with >= 256 64x64 tiles switches the launch to the 64x64 kernel
 GEMM_SIZES_LARGE = [(1100, 1050, 70), (33, 47, 65), (70, 1, 130)]
+# thin launches (every group at most 96 wide on one side, long K: the
+# range-finder and projector products), ragged groups, split over K
+GEMM_SIZES_THIN_N = [(1444, 46, 1444), (700, 40, 900), (150, 20, 300), (9, 3, 260)]
+GEMM_SIZES_THIN_N96 = [(1500, 91, 600), (600, 70, 700), (130, 91, 256)]
+GEMM_SIZES_THIN_M = [(46, 1444, 1444), (69, 800, 700), (12, 300, 500)]
+GEMM_SIZES_THIN_M80 = [(80, 900, 400), (77, 300, 300)]
 
 
-@pytest.mark.parametrize("sizes", [GEMM_SIZES_SMALL, GEMM_SIZES_LARGE], ids=["tile32", "tile64"])
+@pytest.mark.parametrize(
+    "sizes",
+    [GEMM_SIZES_SMALL, GEMM_SIZES_LARGE, GEMM_SIZES_THIN_N, GEMM_SIZES_THIN_N96, GEMM_SIZES_THIN_M, GEMM_SIZES_THIN_M80],
+    ids=["tile32", "tile64", "thin_n48", "thin_n96", "thin_m80", "thin_m80b"],
+)
 def test_mfma_gemm_layouts_and_uneven_groups(gpu, sizes):
     rng = np.random.default_rng(0)
     for a_f in (False, True):
