@@ -22,9 +22,10 @@ Pinned by (tests/test_oracle_su2.py):
   which test_su2_tensor.py:36-79 asserts equals the reference's structural
   permute; norm preservation;
 * basis conventions inside a block for arbitrary reps (elementary block order,
-  internal/external degeneracy order, spin-reversal signs) follow the
-  reference's algorithm; beyond the known-answer cases above they are
-  parity unpinned (no reference outputs exist for them).
+  internal/external degeneracy order, spin-reversal signs): the blocks and
+  permutes the reference itself wrote for representations with external
+  degeneracies and mixed signatures (tests/golden/ref_su2_layouts.npz, checked
+  in tests/test_reference_fixtures.py).
 """
 
 from functools import lru_cache
