@@ -662,7 +662,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--workload", default="ctmrg", choices=["contract_permute", "ctmrg", "serve", "hbm_copy", "batch_ctmrg"])
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
-    ap.add_argument("--slots", type=int, default=4, help="serve: concurrent request slots")
+    ap.add_argument("--slots", type=int, default=None,
+                    help="serve / batch_ctmrg: concurrent slots (default 8 for serve, 4 for batch_ctmrg)")
     ap.add_argument("--serve-iters", type=int, default=10, help="serve: CTMRG iterations per request")
     ap.add_argument("--seed", type=int, default=0, help="serve: request stream seed")
     ap.add_argument("--no-shared-plans", dest="share_plans", action="store_false",
@@ -682,6 +683,8 @@ def main():
     ap.add_argument("--profile-markers", action="store_true",
                     help="bracket the timed iterations with two long spin kernels (kernel-trace cuts)")
     args = ap.parse_args()
+    if args.slots is None:
+        args.slots = 8 if args.workload == "serve" else 4
     if args.gpus != 1:
         print(
             "bench.py measures one GPU only: the hot path is single-device "
