@@ -6,6 +6,7 @@ host fallback for any device operation.
 
 import ctypes
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -291,7 +292,10 @@ def _load():
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
-        fn.argtypes = args
+        # pointer parameters are declared as void pointers: a plain address
+        # (int) then passes without building a typed pointer object, while
+        # typed pointers, ctypes arrays, byref() and None still pass as before
+        fn.argtypes = [c_vp if isinstance(a, type) and issubclass(a, ctypes._Pointer) else a for a in args]
     return lib
 
 
@@ -311,24 +315,64 @@ def check(status):
 # ----------------------------------------------------------------------------
 # small helpers to pass numpy arrays
 # ----------------------------------------------------------------------------
+# Addresses of contiguous numpy arrays for the void-pointer parameters of the
+# entry points. An address does not keep its array alive: the caller must hold
+# the array (a name, an attribute, a list element) until the call returns. A
+# temporary passed straight in (`i64p(x.copy())`) would be freed before the
+# library reads it. Its reference count gives it away: inside the helper only
+# the call's own references remain, as many as _TEMP_REFS measures on a
+# temporary once at import; such an argument is refused.
+def _refs(a):
+    return sys.getrefcount(a)
+
+
+def _refs_via(a):  # the helpers' own call depth: i64p -> _held
+    return _refs(a)
+
+
+_TEMP_REFS = _refs(np.empty(0))
+_TEMP_REFS_VIA = _refs_via(np.empty(0))
+
+
+def _held(a, dtype):
+    if sys.getrefcount(a) <= _TEMP_REFS_VIA:
+        msg = "array passed to the library by address without a reference held by the caller"
+        raise RuntimeError(msg)
+    if a.dtype != dtype or not a.flags.c_contiguous:
+        msg = f"library argument must be a contiguous {np.dtype(dtype)} array, got {a.dtype}"
+        raise TypeError(msg)
+
+
+def _addr(a):
+    if sys.getrefcount(a) <= _TEMP_REFS:
+        msg = "array passed to the library by address without a reference held by the caller"
+        raise RuntimeError(msg)
+    return a.__array_interface__["data"][0]
+
+
 def i64p(a):
-    return a.ctypes.data_as(c_i64p)
+    _held(a, np.int64)
+    return a.__array_interface__["data"][0]
 
 
 def i32p(a):
-    return a.ctypes.data_as(c_i32p)
+    _held(a, np.int32)
+    return a.__array_interface__["data"][0]
 
 
 def i8p(a):
-    return a.ctypes.data_as(c_i8p)
+    _held(a, np.int8)
+    return a.__array_interface__["data"][0]
 
 
 def u8p(a):
-    return a.ctypes.data_as(c_u8p)
+    _held(a, np.uint8)
+    return a.__array_interface__["data"][0]
 
 
 def dblp(a):
-    return a.ctypes.data_as(c_dblp)
+    _held(a, np.float64)
+    return a.__array_interface__["data"][0]
 
 
 def device_count():
@@ -343,7 +387,7 @@ def u1_fuse(reps, flips):
     sizes = np.array([r.size for r in reps], dtype=np.int64)
     flips = np.ascontiguousarray(flips, dtype=np.uint8)
     out = np.empty((int(np.prod(sizes)),), dtype=np.int8)
-    ptrs = (c_i8p * len(reps))(*[i8p(r) for r in reps])
+    ptrs = (c_vp * len(reps))(*[i8p(r) for r in reps])
     check(lib.fs_u1_fuse(len(reps), ptrs, i64p(sizes), u8p(flips), i8p(out)))
     return out
 
@@ -356,7 +400,7 @@ def find_block_cuts_native(block_values, cut_target, dims, cutoff, degen_ratio):
     cuts = np.zeros((n,), dtype=np.int64)
     if n == 0:
         return cuts
-    ptrs = (c_dblp * n)(*[dblp(v) for v in vals])
+    ptrs = (c_vp * n)(*[dblp(v) for v in vals])
     check(
         lib.fs_find_block_cuts(
             n, ptrs, i64p(sizes), int(cut_target), i64p(dims), float(cutoff),

@@ -196,7 +196,7 @@ def _ptr(t):
     if not t.is_cuda:
         msg = "frostspin_amd kernels only accept device tensors"
         raise RuntimeError(msg)
-    return ctypes.c_void_p(t.data_ptr())
+    return t.data_ptr()  # an address for the library's void-pointer parameters
 
 
 def empty(n, dtype=DTYPE):
@@ -343,7 +343,7 @@ def abelian_permute_plan(
         if plan is not None:
             _PLAN_CACHE.move_to_end(key)
             return plan
-    ptrs = (_lib.c_i8p * len(charges))(*[_lib.i8p(c) for c in charges])
+    ptrs = (_lib.c_vp * len(charges))(*[_lib.i8p(c) for c in charges])
     handle = ctypes.c_void_p()
     check(
         lib.fs_abelian_permute_plan_create(
@@ -542,19 +542,22 @@ def grouped_gemm(groups, A, B, C, *, alpha=1.0, beta=0.0, split=False):
             return
     # fs_gemm_group layout: 11 int64 per group
     garr = np.ascontiguousarray(groups, dtype=np.int64).reshape(n, 11)
-    arr = garr.ctypes.data_as(ctypes.POINTER(_lib.GemmGroup))
+    arr = _lib._addr(garr)
     def flop():
         return 2.0 * float((garr[:, 0].astype(np.float64) * garr[:, 1] * garr[:, 2]).sum())
 
     # the library launches one kernel per 32 groups that hold a non-empty group
     if multi:
+        # one device address per group and operand, as uint64 arrays read
+        # during the call (the address of each distinct tensor looked up once)
         ptrs = {}
-        bases = [
-            (_lib.c_vp * n)(*[ptrs[id(t)] if id(t) in ptrs else ptrs.setdefault(id(t), _ptr(t)) for t in lst])
-            for lst in (As, Bs, Cs)
-        ]
+        for t in (*As, *Bs, *Cs):
+            if id(t) not in ptrs:
+                ptrs[id(t)] = _ptr(t)
+        bases = [np.fromiter((ptrs[id(t)] for t in lst), dtype=np.uint64, count=n) for lst in (As, Bs, Cs)]
         with _timed("gemm", flop, _gemm_dispatches):
-            check(lib.fs_grouped_gemm_f64_multi(n, arr, *bases, float(alpha), float(beta), _stream()))
+            check(lib.fs_grouped_gemm_f64_multi(n, arr, *(_lib._addr(b) for b in bases), float(alpha), float(beta),
+                                                _stream()))
         return
     fn = lib.fs_grouped_gemm_f64_split if split else lib.fs_grouped_gemm_f64
     with _timed("gemm_split" if split else "gemm", flop, _gemm_dispatches):
@@ -588,7 +591,7 @@ def _desc_table(ctype, descs):
         table = np.empty((len(descs),), dtype=dt)
         for name, col in zip(dt.names, cols, strict=True):
             table[name] = col
-    return table, table.ctypes.data_as(ctypes.POINTER(ctype))
+    return table, _lib._addr(table)  # the table stays alive with the caller's reference
 
 
 def _ndesc(descs):
@@ -792,10 +795,12 @@ def weighted_trace(x, offs, ns, lds, weights):
     total = zeros(1)
     for s in range(0, max(len(w), 1), 64):
         sl = slice(s, s + 64)
+        # the chunks stay bound to names until the call returns: the library
+        # receives bare addresses
+        o_s, n_s, l_s, w_s = (np.ascontiguousarray(a[sl]) for a in (offs, ns, lds, w))
         check(
             lib.fs_weighted_trace_f64(
-                int(w[sl].size), _lib.i64p(offs[sl].copy()), _lib.i64p(ns[sl].copy()),
-                _lib.i64p(lds[sl].copy()), _lib.dblp(w[sl].copy()),
+                int(w_s.size), _lib.i64p(o_s), _lib.i64p(n_s), _lib.i64p(l_s), _lib.dblp(w_s),
                 _ptr(x) if x.numel() else None, _ptr(out), _stream(),
             )
         )

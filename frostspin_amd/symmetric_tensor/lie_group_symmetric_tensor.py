@@ -30,6 +30,7 @@ from .. import _lib, _ops
 from . import plan_store
 from .abelian_symmetric_tensor import _dense_source
 from .non_abelian_symmetric_tensor import NonAbelianSymmetricTensor
+from .tools import config as _debug_config
 
 # Overlap coefficients below this magnitude are exact zeros polluted by
 # rounding; they are cleared as in overlap_cg_trees (lie:720-726).
@@ -108,7 +109,7 @@ class _PermuteStructure:
         self._keep = arrs
         self.pattern = _lib.LiePattern(
             ndim, nrr, nrr_out, len(self.n_in), len(self.in_targets), len(self.out_targets),
-            *(_lib.i64p(arrs[f]) for f, _ in _lib.LiePattern._fields_[6:]),
+            *(arrs[f].ctypes.data_as(_lib.c_i64p) for f, _ in _lib.LiePattern._fields_[6:]),
         )
         self.weights = _ops.LieWeights(self.pattern, self.W)
 
@@ -317,7 +318,9 @@ class LieGroupSymmetricTensor(NonAbelianSymmetricTensor):
         pos = np.full((len(targets),), -1, dtype=np.int64)
         if self._nblocks:
             ti = np.searchsorted(targets, self._block_irreps)
-            assert (ti < len(targets)).all() and (targets[ti] == self._block_irreps).all()
+            assert not _debug_config["debug_checks"] or (
+                (ti < len(targets)).all() and (targets[ti] == self._block_irreps).all()
+            )
             pos[ti] = np.arange(self._nblocks)
         return pos
 

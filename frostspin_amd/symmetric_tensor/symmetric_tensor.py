@@ -371,7 +371,7 @@ class SymmetricTensor:
             return self._affine(other)
         if isinstance(other, DiagonalTensor):
             assert self._symmetry == other.symmetry()
-            assert (self._col_reps[-1] == other.representation).all()
+            assert not _debug_config["debug_checks"] or (self._col_reps[-1] == other.representation).all()
             return self._diagonal_mul(other, left=False)
         raise TypeError("unsupported operation")
 
@@ -380,7 +380,7 @@ class SymmetricTensor:
             return self._affine(other)
         if isinstance(other, DiagonalTensor):
             assert self._symmetry == other.symmetry()
-            assert (self._row_reps[0] == other.representation).all()
+            assert not _debug_config["debug_checks"] or (self._row_reps[0] == other.representation).all()
             return self._diagonal_mul(other, left=True)
         raise TypeError("unsupported operation")
 
@@ -414,7 +414,7 @@ class SymmetricTensor:
         GEMM over every irrep present on both sides (symmetric_tensor.py:453-489)."""
         assert type(self) is type(other)
         assert self._shape[self._nrr :] == other._shape[: other._nrr]
-        assert (self._signature[self._nrr :] ^ other._signature[: other._nrr]).all()
+        assert not _debug_config["debug_checks"] or (self._signature[self._nrr :] ^ other._signature[: other._nrr]).all()
         key = (
             type(self), self._fortran, other._fortran,
             self._block_irreps.tobytes(), self._bshapes.tobytes(), self._boffs.tobytes(),
@@ -453,7 +453,7 @@ class SymmetricTensor:
         irreps = ia[i_idx]
         offs, total = _packed_offsets(shapes)
         kb = other._bshapes[j_idx, 0]
-        assert (kb == kk).all()
+        assert not _debug_config["debug_checks"] or (kb == kk).all()
         one = np.ones_like(m)
         ars, acs = (one, m) if self._fortran else (kk, one)
         brs, bcs = (one, kb) if other._fortran else (n, one)

@@ -56,3 +56,23 @@ def test_no_host_fallback_for_device_ops():
         _ops.empty(4)
     with pytest.raises(RuntimeError, match="device tensors"):
         _ops.scale_(torch.ones(3, dtype=torch.float64), 2.0)
+
+
+def test_address_helpers_refuse_temporaries():
+    """Pointer parameters take bare addresses: an array the caller does not
+    hold would be freed before the library reads it, so the helpers refuse it;
+    held arrays (names, attributes, list elements) pass, wrong dtypes raise."""
+    import numpy as np
+    import pytest as _pytest
+
+    from frostspin_amd import _lib
+
+    held = np.arange(4, dtype=np.int64)
+    assert _lib.i64p(held) == held.__array_interface__["data"][0]
+    assert [_lib.dblp(v) for v in [np.zeros(2)]][0] != 0
+    with _pytest.raises(RuntimeError):
+        _lib.i64p(held.copy())
+    with _pytest.raises(RuntimeError):
+        _lib._addr(np.zeros(3))
+    with _pytest.raises(TypeError):
+        _lib.i32p(held)
