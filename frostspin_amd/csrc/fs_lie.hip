@@ -348,7 +348,7 @@ lie_transform_kernel(const TileDev *__restrict__ tiles, const TaskDev *__restric
 // sharing one copy.
 struct fs_lie_weights {
   int device = -1;
-  mutable fs::DeviceTables tables;  // note_stream() from the plans' launches
+  mutable fs::DeviceTables tables;  // prepare() from the plans' launches
   int64_t nw = 0;                // length of the unpadded W it was built from
   std::vector<int64_t> off;      // per pattern task: offset of its padded copy
   // per pattern task: (w_off, n_in, n_out_pad) the copy was packed with; a
@@ -718,8 +718,9 @@ int fs_lie_plan_execute_f64(const fs_lie_plan *plan, const double *src, double *
       const char *e = std::getenv("FS_LIE_KERNEL");
       return e ? std::atoi(e) : 0;
     }();
-    plan->tables.note_stream(fs::as_stream(stream));
-    if (plan->weights) plan->weights->tables.note_stream(fs::as_stream(stream));  // for its release
+    int st = plan->tables.prepare(fs::as_stream(stream));
+    if (st == FS_OK && plan->weights) st = plan->weights->tables.prepare(fs::as_stream(stream));
+    if (st != FS_OK) return st;
     if (mode == 1) {
       const size_t lds = static_cast<size_t>(plan->lds_rows) * kLanes * sizeof(double);
       hipLaunchKernelGGL(lie_transform_kernel, dim3(static_cast<unsigned>(plan->ntiles)),

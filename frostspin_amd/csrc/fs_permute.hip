@@ -201,8 +201,8 @@ int execute(fs_permute_plan *plan, const T *src, T *dst, void *stream) {
   const int64_t ntiles = plan->ntiles;
   FS_CHECK_ARG(ntiles < (int64_t(1) << 31), "permute execute: too many tiles");
   int st = upload(plan);
+  if (st == FS_OK) st = static_cast<fs::DeviceTables *>(plan->d_tables)->prepare(fs::as_stream(stream));
   if (st != FS_OK) return st;
-  static_cast<fs::DeviceTables *>(plan->d_tables)->note_stream(fs::as_stream(stream));
   hipLaunchKernelGGL(permute_box_kernel<T>, dim3(static_cast<unsigned>(ntiles)), dim3(kPermThreads), 0,
                      fs::as_stream(stream), src, dst, plan->d_boxes, plan->d_tile0,
                      plan->d_tile_box);
