@@ -24,6 +24,7 @@
   } while (0)
 
 #include <mutex>
+#include <vector>
 
 namespace fs {
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
@@ -55,6 +56,11 @@ class DeviceTables {
   ~DeviceTables() { (void)release(); }
 
   int upload(const void *host, size_t bytes);  // on the current device; `host` may go away after
+  // The same in two steps, without a host image of the caller's: begin()
+  // hands out `bytes` of host memory to fill (the pinned staging block), and
+  // commit() makes it the tables' contents. data() is valid after commit().
+  int begin(size_t bytes, char **host);
+  int commit();
   int prepare(hipStream_t s);                  // before every launch that reads the tables
   int release();                               // idempotent
   const char *data() const { return static_cast<const char *>(ptr_); }
@@ -66,6 +72,7 @@ class DeviceTables {
   int cls_ = -1;              // arena size class of ptr_; -1: pool allocation
   size_t bytes_ = 0;
   void *staging_ = nullptr;   // pinned image until the copy is queued
+  std::vector<char> image_;   // begin() above the arena's classes: filled, uploaded by commit()
   bool queued_ = false;       // copy queued (or done at upload)
   bool ready_ = false;        // copy known to be complete
   uint64_t ticket_ = 0;       // the arena's record of the queued copy

@@ -509,7 +509,6 @@ int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const f
                  "lie plan: tables too large");
     auto p = std::make_unique<fs_lie_plan>();
     std::vector<TaskDev> tdev;
-    std::vector<TileDev> tiles;
     std::vector<int32_t> reg_ids, wide_ids;
     for (int64_t t = 0; t < ntasks; ++t) {
       int st = check_task(tasks[t], nrows, rows, ndigits, digits, nw, src_size, dst_size);
@@ -558,46 +557,48 @@ int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const f
     std::stable_sort(reg_ids.begin(), reg_ids.end(), [&](int32_t a, int32_t b) {
       return int64_t(tdev[a].n_in) * tdev[a].n_out > int64_t(tdev[b].n_in) * tdev[b].n_out;
     });
-    auto add_tiles = [&](const std::vector<int32_t> &ids) {
-      for (int32_t id : ids)
-        for (int64_t e0 = 0; e0 < tdev[id].e_size; e0 += kLanes)
-          tiles.push_back({id, static_cast<int32_t>(e0)});
-    };
-    add_tiles(reg_ids);
-    const size_t n_reg_tiles = tiles.size();
-    add_tiles(wide_ids);
-    FS_CHECK_ARG(tiles.size() < (size_t(1) << 31), "lie plan: too many tiles");
-    std::vector<RowDev> rdev(static_cast<size_t>(nrows));
-    for (int64_t i = 0; i < nrows; ++i)
-      rdev[i] = {rows[i].base, static_cast<int32_t>(rows[i].rs), static_cast<int32_t>(rows[i].cs)};
-    std::vector<DigitDev> ddev(static_cast<size_t>(ndigits));
-    for (int64_t i = 0; i < ndigits; ++i) {
-      const fs_lie_digit &d = digits[i];
-      ddev[i] = {1.0 / static_cast<double>(d.radix), static_cast<int32_t>(d.radix),
-                 static_cast<int32_t>(d.in_r), static_cast<int32_t>(d.in_c),
-                 static_cast<int32_t>(d.out_r), static_cast<int32_t>(d.out_c), 0};
-    }
-    // one host image, 256-byte aligned parts, uploaded as one allocation
+    // one image, 256-byte aligned parts (tiles, tasks, rows, digits, W^T),
+    // written straight into the tables' staging memory
+    size_t n_tiles = 0, n_reg_tiles = 0;
+    for (int32_t id : reg_ids) n_reg_tiles += static_cast<size_t>((tdev[id].e_size + kLanes - 1) / kLanes);
+    n_tiles = n_reg_tiles;
+    for (int32_t id : wide_ids) n_tiles += static_cast<size_t>((tdev[id].e_size + kLanes - 1) / kLanes);
+    FS_CHECK_ARG(n_tiles < (size_t(1) << 31), "lie plan: too many tiles");
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t b_tiles = tiles.size() * sizeof(TileDev), b_tasks = tdev.size() * sizeof(TaskDev);
-    const size_t b_rows = rdev.size() * sizeof(RowDev), b_dig = ddev.size() * sizeof(DigitDev);
+    const size_t b_tiles = n_tiles * sizeof(TileDev), b_tasks = tdev.size() * sizeof(TaskDev);
+    const size_t b_rows = static_cast<size_t>(nrows) * sizeof(RowDev);
+    const size_t b_dig = static_cast<size_t>(ndigits) * sizeof(DigitDev);
     const size_t b_w = wdev.size() * sizeof(double);
     const size_t o_tasks = al(b_tiles), o_rows = o_tasks + al(b_tasks), o_dig = o_rows + al(b_rows);
     const size_t o_w = o_dig + al(b_dig), total = o_w + al(b_w) + 256;
-    std::vector<char> image(total, 0);
-    auto put = [&](size_t off, const void *h, size_t bytes) {
-      if (bytes) std::memcpy(image.data() + off, h, bytes);
-    };
-    put(0, tiles.data(), b_tiles);
-    put(o_tasks, tdev.data(), b_tasks);
-    put(o_rows, rdev.data(), b_rows);
-    put(o_dig, ddev.data(), b_dig);
-    put(o_w, wdev.data(), b_w);
     FS_HIP_CHECK(hipGetDevice(&p->device));
     if (shared)
       FS_CHECK_ARG(shared->device == p->device, "lie plan: weights on device %d, plan on device %d",
                    shared->device, p->device);
-    int st = p->tables.upload(image.data(), total);
+    char *img = nullptr;
+    int st = p->tables.begin(total, &img);
+    if (st != FS_OK) return st;
+    TileDev *tp = reinterpret_cast<TileDev *>(img);
+    for (const std::vector<int32_t> *ids : {&reg_ids, &wide_ids})
+      for (int32_t id : *ids)
+        for (int32_t e0 = 0; e0 < tdev[id].e_size; e0 += kLanes) *tp++ = {id, e0};
+    if (b_tasks) std::memcpy(img + o_tasks, tdev.data(), b_tasks);
+    RowDev *rp = reinterpret_cast<RowDev *>(img + o_rows);
+    for (int64_t i = 0; i < nrows; ++i)
+      rp[i] = {rows[i].base, static_cast<int32_t>(rows[i].rs), static_cast<int32_t>(rows[i].cs)};
+    DigitDev *dp = reinterpret_cast<DigitDev *>(img + o_dig);
+    for (int64_t i = 0; i < ndigits; ++i) {
+      const fs_lie_digit &d = digits[i];
+      dp[i] = {1.0 / static_cast<double>(d.radix), static_cast<int32_t>(d.radix),
+               static_cast<int32_t>(d.in_r), static_cast<int32_t>(d.in_c),
+               static_cast<int32_t>(d.out_r), static_cast<int32_t>(d.out_c), 0};
+    }
+    if (b_w) std::memcpy(img + o_w, wdev.data(), b_w);
+    // the alignment gaps and the tail stay zero, as in a zero-filled image
+    const size_t part_end[5] = {b_tiles, o_tasks + b_tasks, o_rows + b_rows, o_dig + b_dig, o_w + b_w};
+    const size_t next[5] = {o_tasks, o_rows, o_dig, o_w, total};
+    for (int q = 0; q < 5; ++q) std::memset(img + part_end[q], 0, next[q] - part_end[q]);
+    st = p->tables.commit();
     if (st != FS_OK) return st;
     const char *mem = p->tables.data();
     const char *d_tiles = b_tiles ? mem : nullptr;
@@ -612,8 +613,8 @@ int create_impl(int64_t ntasks, const fs_lie_task *tasks, int64_t nrows, const f
     p->w = shared ? reinterpret_cast<const double *>(shared->tables.data())
                   : reinterpret_cast<const double *>(d_w);
     p->weights = shared;
-    p->ntiles = static_cast<int64_t>(tiles.size());
-    p->ntiles_wide = static_cast<int64_t>(tiles.size() - n_reg_tiles);
+    p->ntiles = static_cast<int64_t>(n_tiles);
+    p->ntiles_wide = static_cast<int64_t>(n_tiles - n_reg_tiles);
     p->wide_rows = static_cast<int32_t>(max_wide_in);
     p->lds_rows = static_cast<int32_t>(std::min<int64_t>(max_in, kMaxLdsRows));
     p->src_size = src_size;

@@ -178,11 +178,14 @@ int upload(fs_permute_plan *p) {
   const size_t off_t0 = fs::align_up(nb * sizeof(PermBox), 256);
   const size_t off_tb = off_t0 + fs::align_up(p->tile0.size() * sizeof(int32_t), 256);
   const size_t bytes = off_tb + p->tile_box.size() * sizeof(int32_t);
-  std::vector<char> host(bytes, 0);
-  if (nb) std::memcpy(host.data(), p->boxes.data(), nb * sizeof(PermBox));
-  std::memcpy(host.data() + off_t0, p->tile0.data(), p->tile0.size() * sizeof(int32_t));
-  std::memcpy(host.data() + off_tb, p->tile_box.data(), p->tile_box.size() * sizeof(int32_t));
-  int st = tables->upload(host.data(), bytes);
+  char *host = nullptr;  // staging memory, filled in place
+  int st = tables->begin(bytes, &host);
+  if (st != FS_OK) return st;
+  std::memset(host, 0, bytes);
+  if (nb) std::memcpy(host, p->boxes.data(), nb * sizeof(PermBox));
+  std::memcpy(host + off_t0, p->tile0.data(), p->tile0.size() * sizeof(int32_t));
+  std::memcpy(host + off_tb, p->tile_box.data(), p->tile_box.size() * sizeof(int32_t));
+  st = tables->commit();
   if (st != FS_OK) return st;
   p->d_boxes = reinterpret_cast<const PermBox *>(tables->data());
   p->d_tile0 = reinterpret_cast<const int32_t *>(tables->data() + off_t0);

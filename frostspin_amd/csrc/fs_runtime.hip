@@ -150,52 +150,71 @@ int take_block(Arena &a, int cls, bool host, void **out) {
 
 } // namespace
 
-int DeviceTables::upload(const void *host, size_t bytes) {
+int DeviceTables::begin(size_t bytes, char **host) {
   std::lock_guard<std::mutex> lk(mu_);
-  FS_CHECK_ARG(ptr_ == nullptr, "plan tables uploaded twice");
+  FS_CHECK_ARG(ptr_ == nullptr && image_.empty(), "plan tables uploaded twice");
+  *host = nullptr;
   int dev = 0;
   FS_HIP_CHECK(hipGetDevice(&dev));
   const int cls = class_of(bytes);
-  if (cls >= 0) {
-    Arena &a = arena_of(dev);
-    void *d = nullptr, *h = nullptr;
-    {
-      std::lock_guard<std::mutex> al(a.mu);
-      poll(a);
-      int st = take_block(a, cls, false, &d);
-      if (st == FS_OK && bytes) st = take_block(a, cls, true, &h);
-      if (st != FS_OK) {
-        if (d) a.free_dev[cls].push_back(d);
-        return st;
-      }
-    }
-    if (bytes) std::memcpy(h, host, bytes);
-    ptr_ = d;
-    staging_ = h;
-    cls_ = cls;
-    bytes_ = bytes;
+  if (cls < 0) {
+    // above the arena: filled in host memory, copied through the pool by commit()
+    image_.resize(bytes);
+    *host = image_.data();
     device_ = dev;
-    queued_ = ready_ = bytes == 0;
+    bytes_ = bytes;
     return FS_OK;
   }
+  Arena &a = arena_of(dev);
+  void *d = nullptr, *h = nullptr;
+  {
+    std::lock_guard<std::mutex> al(a.mu);
+    poll(a);
+    int st = take_block(a, cls, false, &d);
+    if (st == FS_OK && bytes) st = take_block(a, cls, true, &h);
+    if (st != FS_OK) {
+      if (d) a.free_dev[cls].push_back(d);
+      return st;
+    }
+  }
+  ptr_ = d;
+  staging_ = h;
+  cls_ = cls;
+  bytes_ = bytes;
+  device_ = dev;
+  queued_ = ready_ = bytes == 0;
+  *host = static_cast<char *>(h);
+  return FS_OK;
+}
+
+int DeviceTables::commit() {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (image_.empty()) return FS_OK;  // arena block: the copy is queued by the first prepare()
   hipStream_t us = nullptr;
-  int st = upload_stream(dev, &us);
+  int st = upload_stream(device_, &us);
   if (st != FS_OK) return st;
   void *p = nullptr;
-  FS_HIP_CHECK(hipMallocAsync(&p, bytes, us));
-  hipError_t e = hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, us);
-  if (e == hipSuccess) e = hipStreamSynchronize(us);  // the copy only: `host` may go away
+  FS_HIP_CHECK(hipMallocAsync(&p, bytes_, us));
+  hipError_t e = hipMemcpyAsync(p, image_.data(), bytes_, hipMemcpyHostToDevice, us);
+  if (e == hipSuccess) e = hipStreamSynchronize(us);  // the copy only: the image goes away
   if (e != hipSuccess) {
     (void)hipFreeAsync(p, us);
     set_error("upload of plan tables failed: %s", hipGetErrorString(e));
     return FS_ERR_HIP;
   }
+  std::vector<char>().swap(image_);
   ptr_ = p;
   cls_ = -1;
-  bytes_ = bytes;
-  device_ = dev;
   queued_ = ready_ = true;
   return FS_OK;
+}
+
+int DeviceTables::upload(const void *host, size_t bytes) {
+  char *h = nullptr;
+  int st = begin(bytes, &h);
+  if (st != FS_OK) return st;
+  if (bytes) std::memcpy(h, host, bytes);
+  return commit();
 }
 
 int DeviceTables::prepare(hipStream_t s) {
