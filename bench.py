@@ -403,6 +403,7 @@ def run_ctmrg(args):
     import torch
 
     from frostspin_amd import _ops
+    from frostspin_amd.config import config
     from frostspin_amd.ctmrg import models
     from frostspin_amd.misc_tools import block_linalg as bl
 
@@ -412,6 +413,7 @@ def run_ctmrg(args):
     else:
         h = models.xxz_bond(0.5, 1.0)
         model = "U(1) XXZ (Jz=0.5, Jperp=1)"
+    config["ctmrg_graphs"] = not args.no_graphs
     t0 = time.perf_counter()
     su, ctm = models.thermal_ctmrg(h, args.D, args.beta, args.chi, cutoff=args.cutoff)
     torch.cuda.synchronize()
@@ -441,13 +443,20 @@ def run_ctmrg(args):
     if args.profile_markers:
         torch.cuda._sleep(PROFILE_MARKER_CYCLES)
         torch.cuda.synchronize()
+    graph_stats = dict(getattr(ctm, "_graphs", None).stats) if getattr(ctm, "_graphs", None) else None
     # one extra iteration under the kernel timer for the roofline of each kernel class
     with _ops.KernelTimer(pad_cycles=200_000) as kt:
         ctm.iterate()
     kr = kernel_roofline(kt.summary())
     # and one more with its grouped GEMMs replayed back to back (device time of
-    # the dominant kernel class without per-launch event overhead)
-    g_flop, g_ms, g_disp = gemm_replay(ctm.iterate)
+    # the dominant kernel class without per-launch event overhead); both run
+    # eagerly, every launch from the host
+    graphs_on = config["ctmrg_graphs"]
+    config["ctmrg_graphs"] = False
+    try:
+        g_flop, g_ms, g_disp = gemm_replay(ctm.iterate)
+    finally:
+        config["ctmrg_graphs"] = graphs_on
     energy = models.bond_energy(ctm, h)
     step_ms = np.array(step_ms)
     ms = wall * 1000.0 / args.steps
@@ -480,6 +489,7 @@ def run_ctmrg(args):
             "simple_update_s": t_su,
             "parallelism": "single GPU",
             "rsvd_certificates": dict(bl.RSVD_STATS),
+            "hip_graphs": graph_stats,
         },
         "roofline": {
             "bound": "mfma",
@@ -660,6 +670,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="CTMRG workloads: every iteration eager (no recorded HIP graphs of settled iterations)")
     ap.add_argument("--workload", default="ctmrg", choices=["contract_permute", "ctmrg", "serve", "hbm_copy", "batch_ctmrg"])
     ap.add_argument("--requests", type=int, default=64, help="serve: number of requests")
     ap.add_argument("--slots", type=int, default=None,

@@ -226,6 +226,12 @@ def download_start(t):
     t = t.detach()
     if not t.is_cuda:
         return t.clone()
+    if recording():
+        # a graph node writing a pinned buffer of the recorder's on every replay
+        t = t.contiguous()
+        h = GraphHooks.active.pinned(t.numel(), t.dtype).view(t.shape)
+        memcpy_async(h, t, t.numel() * t.element_size())
+        return h
     h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
     h.copy_(t, non_blocking=True)
     return h
@@ -237,14 +243,73 @@ def wait_current():
         torch.cuda.current_stream().synchronize()
 
 
-def upload_async(arr, dtype=DTYPE):
+# ---------------------------------------------------------------------------
+# hooks of the CTMRG graph recorder (frostspin_amd.ctmrg.graphs)
+# ---------------------------------------------------------------------------
+class _GraphHooks(threading.local):
+    """`active` is the graph recorder of the CTMRG iteration running in this
+    thread, if any: it is told where the host waits for read-back values and
+    where it resumes launching, and keeps the native plans a recorded segment
+    executes (their device tables must outlive the graph)."""
+
+    active = None
+
+
+GraphHooks = _GraphHooks()
+
+
+PLAN_BUILDS = [0]  # native permute / structural plans built in this process
+
+
+def host_wait(decision=None):
+    """The host needs the values read back so far (singular values, status
+    words): wait for the current stream. Under a recorder, the segment
+    recorded since the previous wait ends here and runs; `decision` is the
+    host step the recorder re-evaluates when it replays that segment."""
+    rec = GraphHooks.active
+    if rec is None:
+        wait_current()
+    else:
+        rec.end_segment(decision)
+
+
+def host_resume(decision=None):
+    """The host decision after a host_wait is taken; launches resume."""
+    rec = GraphHooks.active
+    if rec is not None:
+        rec.begin_segment(decision)
+
+
+def recording():
+    """True while a recorder captures launches into a graph."""
+    rec = GraphHooks.active
+    return rec is not None and rec.capturing
+
+
+def memcpy_async(dst, src, nbytes):
+    """Stream-ordered copy between a device tensor and a pinned host tensor
+    (fs_memcpy_async): a node of the graph when recorded."""
+    check(lib.fs_memcpy_async(dst.data_ptr(), src.data_ptr(), int(nbytes), _stream()))
+
+
+def upload_async(arr, dtype=DTYPE, *, staging=None):
     """Host array -> new 1-D device tensor without a host wait: staged through
     pinned memory, copied on the current stream (ordered before the kernels
-    launched after it)."""
+    launched after it). staging: a list that receives the pinned buffer when
+    the copy is recorded into a graph (a replay copies whatever the buffer
+    then holds)."""
     a = torch.from_numpy(np.ascontiguousarray(arr).reshape(-1))
     dev = device()
     if dev.type != "cuda":
         return a.to(dtype=dtype, copy=True)
+    if recording():
+        h = GraphHooks.active.pinned(a.numel(), dtype)
+        h.copy_(a)
+        d = torch.empty(a.shape, dtype=dtype, device=dev)
+        memcpy_async(d, h, h.numel() * h.element_size())
+        if staging is not None:
+            staging.append(h)
+        return d
     h = torch.empty(a.shape, dtype=dtype, pin_memory=True)
     h.copy_(a)
     return h.to(dev, non_blocking=True)
@@ -283,6 +348,8 @@ class PermutePlan:
         return out, nb.value
 
     def execute(self, src, dst):
+        if GraphHooks.active is not None:
+            GraphHooks.active.keep(self)
         with _timed("permute", 2.0 * dst.element_size() * self.new_ncoeff):
             if dst.dtype == torch.float64:
                 check(lib.fs_permute_plan_execute_f64(self._handle, _ptr(src), _ptr(dst), _stream()))
@@ -363,6 +430,7 @@ def abelian_permute_plan(
             ctypes.byref(handle),
         )
     )
+    PLAN_BUILDS[0] += 1
     plan = PermutePlan(handle.value)
     with _PLAN_LOCK:
         _PLAN_CACHE[key] = plan
@@ -442,6 +510,7 @@ def lie_plan_build(pattern, degen, in_block, block_off, block_rs, block_cs, weig
         _lib.i64p(block_off), _lib.i64p(block_rs), _lib.i64p(block_cs), weights.handle(),
         int(src_size), _lib.i64p(shapes), ctypes.byref(handle),
     ))
+    PLAN_BUILDS[0] += 1
     dst = int((shapes[:, 0] * shapes[:, 1]).sum())
     plan = LiePlan._from_handle(handle.value, src_size, dst)
     plan.w = weights  # the plan reads the weights' device copy: keep it alive
@@ -500,6 +569,8 @@ class LiePlan:
     def execute(self, src, dst):
         assert dst.dtype == torch.float64 and dst.numel() >= self.dst_size
         assert src is None or src.numel() >= self.src_size
+        if GraphHooks.active is not None:
+            GraphHooks.active.keep(self)
         with _timed("lie", 8.0 * self.coeffs):
             check(lib.fs_lie_plan_execute_f64(
                 self._handle, None if src is None else _ptr(src), _ptr(dst), _stream()))

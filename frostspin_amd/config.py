@@ -7,6 +7,8 @@ is not required: without it no table is read, and a table that is given takes
 precedence over the computed values for the dimensions it holds (a missing file
 is an error, as in the reference). `config["debug_checks"]` switches the norm
 and structure checks that run inside assert statements (on unless python -O).
+`config["ctmrg_graphs"]` lets SequentialCTMRG record settled iterations as HIP
+graphs and replay them (frostspin_amd.ctmrg.graphs; GPU only, on by default).
 """
 
 import argparse
@@ -24,7 +26,7 @@ _parser.add_argument(
 )
 _args, _ = _parser.parse_known_args()
 
-config = {"SU2_CG_file": _args.frostspin_SU2_CG_file, "debug_checks": __debug__}
+config = {"SU2_CG_file": _args.frostspin_SU2_CG_file, "debug_checks": __debug__, "ctmrg_graphs": True}
 
 if not __debug__:
     print("\nInfo: assert statements are disabled")

@@ -511,6 +511,16 @@ int fs_convert_f64_to_f32(int64_t n, const double *x, float *y, void *stream) {
   });
 }
 
+int fs_memcpy_async(void *dst, const void *src, int64_t bytes, void *stream) {
+  return fs::guarded([&]() -> int {
+    if (bytes <= 0) return FS_OK;
+    FS_CHECK_ARG(dst && src, "memcpy: null pointer");
+    FS_HIP_CHECK(hipMemcpyAsync(dst, src, static_cast<size_t>(bytes), hipMemcpyDefault,
+                                fs::as_stream(stream)));
+    return FS_OK;
+  });
+}
+
 int fs_convert_f32_to_f64(int64_t n, const float *x, double *y, void *stream) {
   return fs::guarded([&]() -> int {
     if (n <= 0) return FS_OK;

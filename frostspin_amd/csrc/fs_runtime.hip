@@ -222,6 +222,14 @@ int DeviceTables::prepare(hipStream_t s) {
   FS_CHECK_ARG(ptr_ != nullptr, "plan tables used after release");
   for (int i = 0; i < nstreams_; ++i)
     if (streams_[i] == s) return FS_OK;  // already ordered after the copy
+  if (!queued_ || !ready_) {
+    // a graph recorded on `s` would hold this copy or wait, and the arena's
+    // events behind it: plans are built and first used outside a capture
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FS_HIP_CHECK(hipStreamIsCapturing(s, &cs));
+    FS_CHECK_ARG(cs == hipStreamCaptureStatusNone,
+                 "plan tables: first use of a plan on a stream that is being captured");
+  }
   if (!queued_) {
     // first use: the copy goes on this stream, ahead of the plan's kernels
     Arena &a = arena_of(device_);

@@ -49,6 +49,47 @@ def construct_projectors(
     )[0]
 
 
+class _CutDecision:
+    """The host step of a projector batch between its one wait and the
+    projector launches: singular values read back, redone decompositions,
+    cuts. A CTMRG graph recorder re-evaluates it after replaying the recorded
+    launches before the wait, compares the cuts with the recorded ones and
+    writes the new s^-1/2 weights into the pinned buffer the recorded upload
+    reads (staging)."""
+
+    def __init__(self):
+        self.evaluate = None
+        self.cuts = None
+        self.non_empty = None
+        self.redone = []
+        self.staging = []
+
+    def replay(self):
+        """After a replay of the launches before the wait: True when the cuts
+        equal the recorded ones (the weights are then written for the recorded
+        upload), False when the eager path would decide otherwise."""
+        res = self.evaluate(True)
+        if res is None:
+            return False
+        cuts, s_of = res[0], res[1]
+        if not np.array_equal(cuts, self.cuts):
+            return False
+        if self.staging:
+            w = _inv_sqrt_kept(s_of, cuts, self.non_empty)
+            buf = self.staging[0]
+            if buf.numel() != w.size:
+                return False
+            buf.copy_(torch.from_numpy(w))
+        return True
+
+
+def _inv_sqrt_kept(s_of, cuts, non_empty):
+    """s^-1/2 of the kept singular values of every non-empty sector, packed."""
+    if not len(non_empty):
+        return np.zeros((0,))
+    return np.concatenate([1.0 / np.sqrt(s_of[i][: cuts[i]]) for i in non_empty])
+
+
 def construct_projectors_batch(
     problems,
     chi_target,
@@ -127,10 +168,9 @@ def construct_projectors_batch(
         R = bl.gemm(cmats[0], cmats[1], split=split)
         Rt = bl.gemm(cmats[2], cmats[3], split=split)
         M = bl.gemm(R, Rt, split=split)
-    U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
-    # every singular value the truncation needs reaches the host in one wait per
-    # stream: the readbacks are started where the values are produced and read
-    # after both streams have been waited for
+    # every singular value the truncation needs reaches the host in one wait:
+    # the readbacks are started where the values are produced, and the side
+    # stream joins the current stream before the wait
     dense_done = rand_done = None
     if dense_ids:
         # the exact SVDs of the small sectors are independent of the randomized
@@ -144,96 +184,113 @@ def construct_projectors_batch(
                 s_h = _ops.download_start(S)
             for t in (U.buf, S, VT.buf):
                 t.record_stream(main)
+            main.wait_stream(side)
         else:
             U, S, s_off, VT, finish = bl.svd_async(M.select(dense_ids))
             s_h = _ops.download_start(S)
-        dense_done = (U, VT, finish, s_h, side)
+        dense_done = (U, VT, finish, s_h)
     if rand_ids:
         rand_done = bl.rsvd_async(M.select(rand_ids), k_need[rand_ids], seed=rsvd_seed)
-    _ops.wait_current()
-    if rand_done is not None:
-        U, S, s_off, VT, finish = rand_done
-        s_list = finish()
-        for j, i in enumerate(rand_ids):
-            U_of[i] = (U, j)
-            VT_of[i] = (VT, j)
-            s_of[i] = s_list[j]
-    if dense_done is not None:
-        U, VT, finish, s_h, side = dense_done
-        if side is not None:
-            side.synchronize()
-            torch.cuda.current_stream().wait_stream(side)
-        s_list = finish(s_h)
-        for j, i in enumerate(dense_ids):
-            U_of[i] = (U, j)
-            VT_of[i] = (VT, j)
-            s_of[i] = s_list[j]
-    # the randomized SVD's oversampled tail is not accurate: only the k_need
-    # leading values enter the cut (block_chi on the reference's ARPACK branch)
-    for i in rand_ids:
-        s_of[i] = s_of[i][: k_need[i]]
 
     # The reference hands find_block_cuts every singular value of a sector on
     # its dense branch and block_chi values on its ARPACK branch (:96-103); the
     # default relative cutoff counts them all (svd_tools.py:45-48).
     ref_len = np.where(sparse_ref, np.minimum(block_chi, mins), mins)
     by_problem = [[i for i in range(n) if sectors[i][0] == pi] for pi in range(len(problems))]
+    decision = _CutDecision()
 
-    def cut_problem(pi):
-        ids = by_problem[pi]
-        if not ids:
-            return
-        dims = [problems[pi][1].irrep_dimension(sectors[i][1]) for i in ids]
-        rt = rtol if rtol is not None else float(ref_len[ids].sum()) * np.finfo(np.float64).eps
-        cuts[ids] = find_block_cuts([s_of[i] for i in ids], chi_target, dims=dims, atol=atol, rtol=rt,
-                                    degen_ratio=degen_ratio)
+    def decide(replay):
+        """Singular values read back -> per-sector cuts. replay: the readback
+        buffers were written again by a recorded graph; returns None where the
+        eager path would launch more work (a redone decomposition)."""
+        U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
+        if rand_done is not None:
+            U, S, s_off, VT, finish = rand_done
+            s_list = finish(replay=replay, redone=decision.redone)
+            if s_list is None:
+                return None
+            for j, i in enumerate(rand_ids):
+                U_of[i] = (U, j)
+                VT_of[i] = (VT, j)
+                s_of[i] = s_list[j]
+        if dense_done is not None:
+            U, VT, finish, s_h = dense_done
+            s_list = finish(s_h)
+            for j, i in enumerate(dense_ids):
+                U_of[i] = (U, j)
+                VT_of[i] = (VT, j)
+                s_of[i] = s_list[j]
+        # the randomized SVD's oversampled tail is not accurate: only the k_need
+        # leading values enter the cut (block_chi on the reference's ARPACK branch)
+        for i in rand_ids:
+            s_of[i] = s_of[i][: k_need[i]]
 
-    # multiplet-aware cut, per problem
-    cuts = np.zeros((n,), dtype=np.int64)
-    for pi in range(len(problems)):
-        cut_problem(pi)
-    # A dense-branch sector served by the randomized SVD holds only its leading
-    # values; while the cut takes all of them (a sector growing faster than
-    # block_chi allows), the decision may depend on the values not computed:
-    # such sectors are decomposed again (exact Jacobi for small sectors, a
-    # randomized SVD of twice as many values above) and their problem is cut
-    # again, until every such cut stops inside the computed values (the values
-    # beyond the cut do not change the decision; the default cutoff already
-    # counts every value of the sector).
-    while True:
-        short = [i for i in range(n)
-                 if not sparse_ref[i] and len(s_of[i]) < mins[i] and cuts[i] >= len(s_of[i])]
-        if not short:
-            break
-        exact = [i for i in short if mins[i] <= bl.EXACT_REDO_MAX]
-        wider = [i for i in short if mins[i] > bl.EXACT_REDO_MAX]
-        if exact:
-            U, S, s_off, VT, finish = bl.svd_async(M.select(exact))
-            s_list = finish()
-            for j, i in enumerate(exact):
-                U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j]
-        if wider:
-            ks = np.array([min(int(mins[i]), 2 * len(s_of[i])) for i in wider], dtype=np.int64)
-            U, S, s_off, VT, finish = bl.rsvd_async(M.select(wider), ks, seed=rsvd_seed)
-            _ops.wait_current()
-            s_list = finish()
-            for j, i in enumerate(wider):
-                U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j][: ks[j]]
-        for pi in sorted({sectors[i][0] for i in short}):
+        def cut_problem(pi):
+            ids = by_problem[pi]
+            if not ids:
+                return
+            dims = [problems[pi][1].irrep_dimension(sectors[i][1]) for i in ids]
+            rt = rtol if rtol is not None else float(ref_len[ids].sum()) * np.finfo(np.float64).eps
+            cuts[ids] = find_block_cuts([s_of[i] for i in ids], chi_target, dims=dims, atol=atol, rtol=rt,
+                                        degen_ratio=degen_ratio)
+
+        # multiplet-aware cut, per problem
+        cuts = np.zeros((n,), dtype=np.int64)
+        for pi in range(len(problems)):
             cut_problem(pi)
+        # A dense-branch sector served by the randomized SVD holds only its leading
+        # values; while the cut takes all of them (a sector growing faster than
+        # block_chi allows), the decision may depend on the values not computed:
+        # such sectors are decomposed again (exact Jacobi for small sectors, a
+        # randomized SVD of twice as many values above) and their problem is cut
+        # again, until every such cut stops inside the computed values (the values
+        # beyond the cut do not change the decision; the default cutoff already
+        # counts every value of the sector).
+        while True:
+            short = [i for i in range(n)
+                     if not sparse_ref[i] and len(s_of[i]) < mins[i] and cuts[i] >= len(s_of[i])]
+            if not short:
+                break
+            if replay:
+                return None
+            decision.redone.append("short")
+            exact = [i for i in short if mins[i] <= bl.EXACT_REDO_MAX]
+            wider = [i for i in short if mins[i] > bl.EXACT_REDO_MAX]
+            if exact:
+                U, S, s_off, VT, finish = bl.svd_async(M.select(exact))
+                s_list = finish()
+                for j, i in enumerate(exact):
+                    U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j]
+            if wider:
+                ks = np.array([min(int(mins[i]), 2 * len(s_of[i])) for i in wider], dtype=np.int64)
+                U, S, s_off, VT, finish = bl.rsvd_async(M.select(wider), ks, seed=rsvd_seed)
+                _ops.wait_current()
+                s_list = finish()
+                for j, i in enumerate(wider):
+                    U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j][: ks[j]]
+            for pi in sorted({sectors[i][0] for i in short}):
+                cut_problem(pi)
+        return cuts, s_of, U_of, VT_of
+
+    decision.evaluate = decide
+    _ops.host_wait(decision)
+    cuts, s_of, U_of, VT_of = decide(False)
+    decision.cuts = cuts.copy()
     if diagnostics is not None:
         diagnostics.update(
             sectors=[(sec[0], sec[1], sec[2]) for sec in sectors], cuts=cuts.copy(),
             values=[np.array(v, copy=True) for v in s_of], sparse_ref=sparse_ref.copy(), mins=mins.copy(),
         )
     non_empty = cuts.nonzero()[0]
+    decision.non_empty = non_empty
 
     # s^-1/2 of the kept values, one device vector for all sectors
-    w_host = np.concatenate([1.0 / np.sqrt(s_of[i][: cuts[i]]) for i in non_empty]) if len(non_empty) else np.zeros((0,))
+    w_host = _inv_sqrt_kept(s_of, cuts, non_empty)
     w_off = np.zeros((len(non_empty),), dtype=np.int64)
     if len(non_empty) > 1:
         w_off[1:] = np.cumsum(cuts[non_empty])[:-1]
-    w_dev = _ops.upload_async(w_host)
+    _ops.host_resume(decision)
+    w_dev = _ops.upload_async(w_host, staging=decision.staging)
 
     # Us = U[:, :cut] * s^-1/2 and Vs = s^-1/2 * VT[:cut, :] per sector (grouped by source buffer)
     kept = [int(cuts[i]) for i in non_empty]

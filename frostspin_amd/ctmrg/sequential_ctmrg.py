@@ -1,6 +1,7 @@
 """Sequential CTMRG schedule (ctmrg/sequential_ctmrg.py:19-211).
 
-One iterate() = up, right, down, left moves. Each move builds the projectors
+One iterate() = up, right, down, left moves (on a GPU, settled iterations are
+recorded once as HIP graphs and replayed: frostspin_amd.ctmrg.graphs). Each move builds the projectors
 of every inequivalent site from the four enlarged corners (freeing the two
 corners it renormalizes) -- the sites' projector computations are independent
 and share their device launches -- then renormalizes C, T, C of every site,
@@ -11,6 +12,7 @@ leg.
 """
 
 from .. import _ops
+from ..config import config
 from .abstract_ctmrg import AbstractCTMRG
 from .ctm_renormalize import (
     renormalize_C1_left,
@@ -26,6 +28,7 @@ from .ctm_renormalize import (
     renormalize_T3,
     renormalize_T4,
 )
+from .graphs import GraphRunner
 
 
 class SequentialCTMRG(AbstractCTMRG):
@@ -36,10 +39,27 @@ class SequentialCTMRG(AbstractCTMRG):
         )
 
     def iterate(self):
-        self.up_move()
-        self.right_move()
-        self.down_move()
-        self.left_move()
+        runner = self._graph_runner()
+        if runner is None:
+            self.up_move()
+            self.right_move()
+            self.down_move()
+            self.left_move()
+        else:
+            runner.iterate()
+
+    def _graph_runner(self):
+        """The recorder/replayer of settled iterations (frostspin_amd.ctmrg.graphs)
+        on a GPU with config["ctmrg_graphs"]; None for plain eager moves (also
+        while a KernelTimer times every launch on its own)."""
+        runner = getattr(self, "_graphs", None)
+        if not config["ctmrg_graphs"] or _ops.KernelTimer.active is not None or _ops.device().type != "cuda":
+            if runner is not None:
+                runner._drop()
+            return None
+        if runner is None:
+            runner = self._graphs = GraphRunner(self)
+        return runner
 
     def _move_projectors(self, store, problems):
         """problems: ((x, y) of the stored projectors, corner1..4, last corner)

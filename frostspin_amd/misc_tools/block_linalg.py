@@ -490,12 +490,21 @@ def rsvd_async(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False,
     res = _residual_norms2(A, U, S, s_off, VT, kc) if certify else None
     r_h = _ops.download_start(res[0]) if res is not None and res[0].numel() else None
 
-    def finish():
+    def finish(replay=False, redone=None):
+        """Host singular values per matrix, once the stream has passed the
+        readbacks. replay: the readback buffers were written again by a
+        recorded graph; returns None where the eager path would decompose a
+        matrix again (it never launches work then). redone: a list that
+        records whether the eager path did."""
         s_list = finish_small(s_h)
         failed = np.zeros((0,), dtype=np.int64)
         if st_h is not None:
             failed = st_h.numpy().any(axis=0).nonzero()[0]
             if failed.size:
+                if replay:
+                    return None
+                if redone is not None:
+                    redone.append("breakdown")
                 _splice_rsvd(A, ks, failed, U, S, s_off, s_list, VT, oversampling, power_iters, seed)
         if r_h is not None:
             r2 = r_h.numpy()
@@ -504,10 +513,14 @@ def rsvd_async(A, ks, *, oversampling=None, power_iters=2, seed=0, robust=False,
                 if kc[j] and j not in failed and s_list[j][0] > 0:
                     rel[j] = np.sqrt(max(r2[res[1][j] : res[1][j] + kc[j]].max(), 0.0)) / s_list[j][0]
             bad = [j for j in range(len(A)) if rel[j] > RSVD_RESIDUAL_TOL]
+            if replay and bad:
+                return None
             RSVD_STATS["certified"] += int((kc > 0).sum())
             RSVD_STATS["refined"] += len(bad)
             RSVD_STATS["max_rel_residual"] = max(RSVD_STATS["max_rel_residual"], float(rel.max(initial=0.0)))
             if bad:
+                if redone is not None:
+                    redone.append("residual")
                 _refine_rsvd(A, ks, ls, np.array(bad, dtype=np.int64), U, S, s_off, s_list, VT, power_iters, seed)
         return s_list
 

@@ -535,6 +535,14 @@ int fs_scale_f64(int64_t n, double alpha, double *x, void *stream);
  * tensor without a host round trip (nT /= nT.norm(), ctm_renormalize.py:327). */
 int fs_scale_rsqrt_dev_f64(int64_t n, const double *d, double *x, void *stream);
 
+/* Stream-ordered copy of `bytes` between device memory and pinned host memory
+ * (either direction, hipMemcpyDefault). No counterpart in the reference: the
+ * CTMRG graph recorder reads singular values back and writes the projector
+ * weights through it, so that those copies become nodes of a recorded graph,
+ * at fixed host addresses, instead of going through the framework's host
+ * allocator. */
+int fs_memcpy_async(void *dst, const void *src, int64_t bytes, void *stream);
+
 /* y[i] = alpha * x[i] + beta * y[i]. Used by __add__/__sub__ (:349-375). */
 int fs_axpby_f64(int64_t n, double alpha, const double *x, double beta,
                  double *y, void *stream);
