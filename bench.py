@@ -595,7 +595,9 @@ def run_batch_ctmrg(args):
     still iterating. Each worker times its own iterations (stream-synchronized),
     so p50/p99 are per-iteration latencies inside that interval."""
     from frostspin_amd import serving
+    from frostspin_amd.config import config
 
+    config["ctmrg_graphs"] = not args.no_graphs
     n_iter = args.warmup + args.steps
     reqs = [
         serving.Request(rid=i, beta=args.beta, chi=args.chi, D=args.D, n_iter=n_iter, record_iters=True)

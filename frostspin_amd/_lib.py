@@ -258,6 +258,7 @@ _SIGNATURES = {
     "fs_weighted_norm2_f64": (c_int, [c_int, c_i64p, c_dblp, c_vp, c_vp, c_vp, c_vp]),
     "fs_scale_f64": (c_int, [c_i64, c_dbl, c_vp, c_vp]),
     "fs_memcpy_async": (c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "fs_plan_tables_settle": (c_int, []),
     "fs_scale_rsqrt_dev_f64": (c_int, [c_i64, c_vp, c_vp, c_vp]),
     "fs_axpby_f64": (c_int, [c_i64, c_dbl, c_vp, c_dbl, c_vp, c_vp]),
     "fs_batched_copy2d_f64": (

@@ -59,9 +59,11 @@ def side_stream(k=0):
 # Independent chains of small launches (the enlarged corners and the
 # renormalizations of different sites) can run on separate streams; results
 # are identical either way (same kernels on the same inputs).
-# Measured on the SU(2) chi=256 workload: 2-3 % slower than one stream (the
-# chains are already issued faster than they run), so off by default.
-CONCURRENCY = {"enabled": False}
+# Measured on the SU(2) chi=256 workload with every launch from the host:
+# 2-3 % slower than one stream (the chains are already issued faster than they
+# run), so off by default there; while a CTMRG iteration is recorded as graphs
+# the chains become parallel branches of the graph, at no host cost on replay.
+CONCURRENCY = {"enabled": False, "recorded": True}
 
 
 def _device_buffers(obj):
@@ -85,7 +87,8 @@ def run_concurrently(thunks):
     Inputs must be ready on the current stream when this is called (the side
     streams wait for it first)."""
     thunks = list(thunks)
-    if len(thunks) < 2 or not CONCURRENCY["enabled"] or KernelTimer.active is not None or device().type != "cuda":
+    concurrent = CONCURRENCY["enabled"] or (CONCURRENCY["recorded"] and recording())
+    if len(thunks) < 2 or not concurrent or KernelTimer.active is not None or device().type != "cuda":
         return [t() for t in thunks]
     main = torch.cuda.current_stream()
     sides = [side_stream(1 + i) for i in range(len(thunks) - 1)]
@@ -286,6 +289,46 @@ def recording():
     return rec is not None and rec.capturing
 
 
+# Native plans dropped while some thread captures a graph are destroyed after
+# the capture: a release records events on the streams that executed the plan,
+# which must not happen on a stream being captured.
+_CAPTURES = [0]
+_DEFERRED = []
+_DEFERRED_LOCK = threading.Lock()
+
+
+def _destroy(fn, handle):
+    with _DEFERRED_LOCK:
+        if _CAPTURES[0]:
+            _DEFERRED.append((fn, handle))
+            return
+    fn(handle)
+
+
+def capture_started():
+    with _DEFERRED_LOCK:
+        _CAPTURES[0] += 1
+
+
+def capture_ended():
+    """Destroys the plans dropped during the captures, once none is left."""
+    with _DEFERRED_LOCK:
+        _CAPTURES[0] -= 1
+        if _CAPTURES[0]:
+            return
+        pending = list(_DEFERRED)
+        _DEFERRED.clear()
+    for fn, h in pending:
+        fn(h)
+
+
+def settle_plan_tables():
+    """Before a capture (after a device synchronisation): every completed plan
+    table copy is marked complete, so plans used inside the capture need no
+    event query (fs_plan_tables_settle)."""
+    check(lib.fs_plan_tables_settle())
+
+
 def memcpy_async(dst, src, nbytes):
     """Stream-ordered copy between a device tensor and a pinned host tensor
     (fs_memcpy_async): a node of the graph when recorded."""
@@ -362,7 +405,10 @@ class PermutePlan:
     def __del__(self):
         h = self._handle
         if h is not None and lib is not None:
-            lib.fs_permute_plan_destroy(h)
+            if _destroy is None:  # interpreter shutdown: module globals already cleared
+                lib.fs_permute_plan_destroy(h)
+            else:
+                _destroy(lib.fs_permute_plan_destroy, h)
             self._handle = None
 
 
@@ -491,7 +537,10 @@ class LieWeights:
     def __del__(self):
         if lib is not None:
             for h in self._handles.values():
-                lib.fs_lie_weights_destroy(h)
+                if _destroy is None:  # interpreter shutdown: module globals already cleared
+                    lib.fs_lie_weights_destroy(h)
+                else:
+                    _destroy(lib.fs_lie_weights_destroy, h)
         self._handles = {}
 
 
@@ -578,7 +627,10 @@ class LiePlan:
     def __del__(self):
         h = self._handle
         if h is not None and lib is not None:
-            lib.fs_lie_plan_destroy(h)
+            if _destroy is None:  # interpreter shutdown: module globals already cleared
+                lib.fs_lie_plan_destroy(h)
+            else:
+                _destroy(lib.fs_lie_plan_destroy, h)
             self._handle = None
 
 

@@ -614,6 +614,7 @@ struct SplitkWs {
 struct SplitkPool {
   std::mutex mu;
   std::vector<SplitkWs> ws;
+  std::vector<double *> retired;  // outgrown buffers, never freed
 };
 
 SplitkPool &splitk_pool() {
@@ -635,11 +636,13 @@ double *splitk_workspace_locked(int64_t n_doubles, hipStream_t stream) {
   }
   SplitkWs &w = *found;
   if (w.size < n_doubles) {
-    // hipFree waits for the device, so work still reading the old buffer is done
-    if (w.ptr) (void)hipFree(w.ptr);
+    // The outgrown buffer is kept, not freed: a recorded graph may hold its
+    // address and replay later. Sizes at least double, so what is kept stays
+    // below the final size.
+    const int64_t want = std::max<int64_t>({n_doubles, 2 * w.size, int64_t(1) << 20});
+    if (w.ptr) pool.retired.push_back(w.ptr);
     w.ptr = nullptr;
     w.size = 0;
-    const int64_t want = std::max<int64_t>(n_doubles, int64_t(1) << 20);
     if (hipMalloc(&w.ptr, static_cast<size_t>(want) * sizeof(double)) != hipSuccess) return nullptr;
     w.size = want;
   }

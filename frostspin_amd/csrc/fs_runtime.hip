@@ -222,20 +222,25 @@ int DeviceTables::prepare(hipStream_t s) {
   FS_CHECK_ARG(ptr_ != nullptr, "plan tables used after release");
   for (int i = 0; i < nstreams_; ++i)
     if (streams_[i] == s) return FS_OK;  // already ordered after the copy
-  if (!queued_ || !ready_) {
-    // a graph recorded on `s` would hold this copy or wait, and the arena's
-    // events behind it: plans are built and first used outside a capture
+  // A graph recorded on `s` must not hold the tables' copy, or a wait for it,
+  // with the arena's events behind them: plans are built and first used
+  // outside a capture (a later use on another stream needs neither once the
+  // copy is known to be complete).
+  auto refuse_capture = [&]() -> int {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     FS_HIP_CHECK(hipStreamIsCapturing(s, &cs));
     FS_CHECK_ARG(cs == hipStreamCaptureStatusNone,
                  "plan tables: first use of a plan on a stream that is being captured");
-  }
+    return FS_OK;
+  };
   if (!queued_) {
+    int st = refuse_capture();
+    if (st != FS_OK) return st;
     // first use: the copy goes on this stream, ahead of the plan's kernels
     Arena &a = arena_of(device_);
     std::lock_guard<std::mutex> al(a.mu);
     InFlight f;
-    int st = take_event(a, &f.ev[0]);
+    st = take_event(a, &f.ev[0]);
     if (st != FS_OK) return st;
     hipError_t e = hipMemcpyAsync(ptr_, staging_, bytes_, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
@@ -265,6 +270,10 @@ int DeviceTables::prepare(hipStream_t s) {
     if (ticket_ <= a.done_ticket) {
       ready_ = true;
     } else {
+      // not known to be complete: querying the copy's event (recorded on
+      // another stream, which may be capturing now) is refused under capture
+      int st = refuse_capture();
+      if (st != FS_OK) return st;
       const InFlight &f = a.copies[static_cast<size_t>(ticket_ - a.copies.front().ticket)];
       const int q = query(f.ev[0]);
       if (q < 0) return FS_ERR_HIP;
@@ -345,3 +354,19 @@ int DeviceTables::release() {
 }
 
 } // namespace fs
+
+extern "C" int fs_plan_tables_settle(void) {
+  return fs::guarded([&]() -> int {
+    std::vector<fs::Arena *> arenas;
+    {
+      std::lock_guard<std::mutex> lk(fs::g_upload_mu);
+      for (auto &a : fs::g_arenas)
+        if (a) arenas.push_back(a.get());
+    }
+    for (fs::Arena *a : arenas) {
+      std::lock_guard<std::mutex> al(a->mu);
+      fs::poll(*a);
+    }
+    return FS_OK;
+  });
+}

@@ -216,14 +216,25 @@ class GraphRunner:
     def _begin_capture(self):
         self._graph = torch.cuda.CUDAGraph()
         self._builds = _ops.PLAN_BUILDS[0]
-        self._graph.capture_begin(pool=self.pool, capture_error_mode="relaxed")
+        # plans whose table copies completed need no event query inside
+        self.stream.synchronize()
+        _ops.settle_plan_tables()
+        _ops.capture_started()
+        try:
+            self._graph.capture_begin(pool=self.pool, capture_error_mode="relaxed")
+        except BaseException:
+            _ops.capture_ended()
+            raise
         self.capturing = True
 
     def _end_capture(self):
         g = self._graph
         self._graph = None
         self.capturing = False
-        g.capture_end()
+        try:
+            g.capture_end()
+        finally:
+            _ops.capture_ended()
         g.replay()  # the recorded launches run now, as they would have eagerly
         self.graphs[self.move].append(g)
         if _ops.PLAN_BUILDS[0] != self._builds:
@@ -280,6 +291,8 @@ class GraphRunner:
                 self._graph.capture_end()
             except Exception:  # noqa: BLE001 -- the capture is discarded either way
                 pass
+            finally:
+                _ops.capture_ended()
             self._graph = None
             # the move's launches since its last wait did not run: run it again
             self._set_state(snapshots[m])

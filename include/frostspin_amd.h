@@ -535,6 +535,13 @@ int fs_scale_f64(int64_t n, double alpha, double *x, void *stream);
  * tensor without a host round trip (nT /= nT.norm(), ctm_renormalize.py:327). */
 int fs_scale_rsqrt_dev_f64(int64_t n, const double *d, double *x, void *stream);
 
+/* Marks every plan-table copy that has completed as such (polls the events
+ * behind them). No counterpart in the reference: the CTMRG graph recorder
+ * calls it after a device synchronisation and before it captures, so that
+ * plans executed inside the capture need no event query (events last
+ * recorded on a stream that is being captured cannot be queried). */
+int fs_plan_tables_settle(void);
+
 /* Stream-ordered copy of `bytes` between device memory and pinned host memory
  * (either direction, hipMemcpyDefault). No counterpart in the reference: the
  * CTMRG graph recorder reads singular values back and writes the projector
