@@ -70,8 +70,9 @@ class SequentialCTMRG(AbstractCTMRG):
 
     def _site_corners(self, calls):
         """The enlarged corners of every site (calls: per site, a function
-        returning its four corners); sites run on separate streams."""
-        return _ops.run_concurrently(calls)
+        returning its four corners); the corners not cached yet are built as
+        independent launch chains (streams of their own when recorded)."""
+        return self._build_corners(calls)
 
     def _renormalize_sites(self, store, calls):
         """Renormalized (C, T, C) of every site (calls: per site, a function
