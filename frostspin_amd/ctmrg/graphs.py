@@ -27,6 +27,8 @@ then be identical, else no graph is kept). Every native plan a graph executes
 is kept alive with the graphs (their device tables are read by address).
 """
 
+import warnings
+
 import numpy as np
 import torch
 
@@ -89,6 +91,7 @@ class GraphRunner:
         self.move = 0
         self.prev_cuts = [None] * len(self.moves)
         self.settings = None
+        self.failed = None  # the error that stopped recording, if any
         self.stats = {"recorded": 0, "replayed": 0, "dropped": 0}
 
     # hooks called by _ops -------------------------------------------------
@@ -189,7 +192,7 @@ class GraphRunner:
                 self._eager(m)
                 return
         builds = _ops.PLAN_BUILDS[0]
-        if self.quiet >= self.settle and config["ctmrg_graphs"]:
+        if self.quiet >= self.settle and config["ctmrg_graphs"] and self.failed is None:
             self._record()
         else:
             self._eager(0)
@@ -282,10 +285,15 @@ class GraphRunner:
                     self._end_capture()
                     if not closed:
                         self.recording = False
-        except BaseException:
+        except BaseException as exc:
             if not self.capturing:
                 self._drop()
                 raise
+            # an error inside a capture: the iteration goes on eagerly and this
+            # runner records no more (a failing capture would otherwise be
+            # retried, and paid for, every iteration)
+            self.failed = repr(exc)
+            warnings.warn(f"CTMRG graph recording disabled: {exc!r}", RuntimeWarning, stacklevel=2)
             self.capturing = False
             try:
                 self._graph.capture_end()
