@@ -294,7 +294,7 @@ def recording():
 # which must not happen on a stream being captured.
 _CAPTURES = [0]
 _DEFERRED = []
-_DEFERRED_LOCK = threading.Lock()
+_DEFERRED_LOCK = threading.RLock()  # a collection inside it may run another __del__
 
 
 def _destroy(fn, handle):

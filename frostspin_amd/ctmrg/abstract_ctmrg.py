@@ -8,7 +8,6 @@ grouped MFMA GEMM, every projector a batched device SVD.
 
 import numpy as np
 
-from .. import _ops
 from . import observables, rdm
 from .ctm_contract import (
     contract_C1234,
@@ -243,36 +242,12 @@ class AbstractCTMRG:
     def _enlarged(self, getter, setter, build, x, y, free_memory):
         c = getter(x, y)
         if c is None:
-            batch = getattr(self, "_corner_batch", None)
-            if batch is not None:
-                # built later with the other corners of the batch (_build_corners)
-                c = _PendingCorner(build)
-                batch.append(c)
-            else:
-                c = build()
+            c = build()
             if not free_memory:
                 setter(x, y, c)
         elif free_memory:
             setter(x, y, None)
         return c
-
-    def _build_corners(self, calls):
-        """Results of calls (functions returning tuples of enlarged corners)
-        with every corner not cached yet built afterwards as one batch of
-        independent launch chains (_ops.run_concurrently); cache reads, stores
-        and frees happen in the calls' order, as without the batch."""
-        pending = []
-        self._corner_batch = pending
-        try:
-            out = [tuple(call()) for call in calls]
-        finally:
-            self._corner_batch = None
-        for p, c in zip(pending, _ops.run_concurrently([p.build for p in pending]), strict=True):
-            p.value = c
-        e = self._env
-        for name, lst in e._enlarged.items():
-            e._enlarged[name] = [t.value if isinstance(t, _PendingCorner) else t for t in lst]
-        return [tuple(t.value if isinstance(t, _PendingCorner) else t for t in site) for site in out]
 
     def construct_enlarged_dr(self, x, y, *, free_memory=False):
         """Enlarged down-right corner, transposed w.r.t. clockwise order
@@ -307,16 +282,6 @@ class AbstractCTMRG:
             lambda: contract_ur(e.get_T1(x + 2, y), e.get_C2(x + 3, y), e.get_A(x + 2, y + 1), e.get_T2(x + 3, y + 1)),
             x, y, free_memory,
         )
-
-
-class _PendingCorner:
-    """An enlarged corner requested inside a corner batch, built at its end."""
-
-    __slots__ = ("build", "value")
-
-    def __init__(self, build):
-        self.build = build
-        self.value = None
 
 
 def _env_property(name):

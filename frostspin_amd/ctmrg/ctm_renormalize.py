@@ -292,30 +292,21 @@ def construct_projectors_batch(
     _ops.host_resume(decision)
     w_dev = _ops.upload_async(w_host, staging=decision.staging)
 
-    # P = Us^T R (cut x n2) and Pt = Rt Vs^T (m3 x cut), one buffer per problem,
-    # with Us = U[:, :cut] * s^-1/2 and Vs = s^-1/2 * VT[:cut, :] per sector
-    # (grouped by source buffer); the two chains are independent
+    # Us = U[:, :cut] * s^-1/2 and Vs = s^-1/2 * VT[:cut, :] per sector (grouped by source buffer)
     kept = [int(cuts[i]) for i in non_empty]
+    Us = _scaled([U_of[i] for i in non_empty], kept, w_dev, w_off, cols=True)
+    Vs = _scaled([VT_of[i] for i in non_empty], kept, w_dev, w_off, cols=False)
+
+    # P = Us^T R (cut x n2) and Pt = Rt Vs^T (m3 x cut), one buffer per problem
     ne_of = [[k for k, i in enumerate(non_empty) if sectors[i][0] == pi] for pi in range(len(problems))]
     Rk = R.select(non_empty)
     Rtk = Rt.select(non_empty)
     P_out = [bl.Mats.packed(np.array([kept[k] for k in ks], np.int64), Rk.cols[ks]) for ks in ne_of]
     Pt_out = [bl.Mats.packed(Rtk.rows[ks], np.array([kept[k] for k in ks], np.int64)) for ks in ne_of]
     order = [k for ks in ne_of for k in ks]
-
-    def p_chain():
-        Us = _scaled([U_of[i] for i in non_empty], kept, w_dev, w_off, cols=True)
-        if order:
-            bl.gemm_into(Us.T().select(order), Rk.select(order), bl.Mats.concat(P_out), split=split)
-        return Us.buf
-
-    def pt_chain():
-        Vs = _scaled([VT_of[i] for i in non_empty], kept, w_dev, w_off, cols=False)
-        if order:
-            bl.gemm_into(Rtk.select(order), Vs.T().select(order), bl.Mats.concat(Pt_out), split=split)
-        return Vs.buf
-
-    _ops.run_concurrently([p_chain, pt_chain])
+    if order:
+        bl.gemm_into(Us.T().select(order), Rk.select(order), bl.Mats.concat(P_out), split=split)
+        bl.gemm_into(Rtk.select(order), Vs.T().select(order), bl.Mats.concat(Pt_out), split=split)
 
     out = []
     for pi, (_, corner2, _, _, _) in enumerate(problems):

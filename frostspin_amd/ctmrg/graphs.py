@@ -27,6 +27,7 @@ then be identical, else no graph is kept). Every native plan a graph executes
 is kept alive with the graphs (their device tables are read by address).
 """
 
+import time
 import warnings
 
 import numpy as np
@@ -205,14 +206,20 @@ class GraphRunner:
 
     def _replay(self):
         """Replays the recorded iteration; the index of the move whose decision
-        differs from the recorded one, or None."""
+        differs from the recorded one, or None. Accumulates the host time spent
+        outside the waits (graph launches, decisions) in stats."""
+        t0 = time.perf_counter()
+        waited = 0.0
         for m in range(len(self.moves)):
             a, b = self.graphs[m]
             a.replay()
+            tw = time.perf_counter()
             self.stream.synchronize()
+            waited += time.perf_counter() - tw
             if not self.decisions[m].replay():
                 return m
             b.replay()
+        self.stats["replay_host_s"] = self.stats.get("replay_host_s", 0.0) + time.perf_counter() - t0 - waited
         return None
 
     # recording --------------------------------------------------------------

@@ -50,10 +50,12 @@ class SequentialCTMRG(AbstractCTMRG):
 
     def _graph_runner(self):
         """The recorder/replayer of settled iterations (frostspin_amd.ctmrg.graphs)
-        on a GPU with config["ctmrg_graphs"]; None for plain eager moves (also
-        while a KernelTimer times every launch on its own)."""
+        on a GPU with config["ctmrg_graphs"] (and use_graphs, a per-object
+        switch, not False); None for plain eager moves (also while a
+        KernelTimer times every launch on its own)."""
         runner = getattr(self, "_graphs", None)
-        if not config["ctmrg_graphs"] or _ops.KernelTimer.active is not None or _ops.device().type != "cuda":
+        if (not config["ctmrg_graphs"] or not getattr(self, "use_graphs", True)
+                or _ops.KernelTimer.active is not None or _ops.device().type != "cuda"):
             if runner is not None:
                 runner._drop()
             return None
@@ -70,9 +72,8 @@ class SequentialCTMRG(AbstractCTMRG):
 
     def _site_corners(self, calls):
         """The enlarged corners of every site (calls: per site, a function
-        returning its four corners); the corners not cached yet are built as
-        independent launch chains (streams of their own when recorded)."""
-        return self._build_corners(calls)
+        returning its four corners); sites run on separate streams."""
+        return _ops.run_concurrently(calls)
 
     def _renormalize_sites(self, store, calls):
         """Renormalized (C, T, C) of every site (calls: per site, a function

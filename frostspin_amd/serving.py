@@ -108,6 +108,9 @@ def solve(req, iter_times=None):
         raise ValueError(f"unsupported symmetry {req.symmetry}")
     _, tensors = models.thermal_tensors(h, req.D, req.beta, tau=req.tau)
     ctm = SequentialCTMRG.from_elementary_tensors("AB\nBA", tensors, req.chi, cutoff=1e-10, degen_ratio=0.99)
+    # every iteration eager: a serving worker process died once while
+    # recording graphs in the 8-slot serving benchmark, cause not found
+    ctm.use_graphs = False
     for _ in range(req.n_iter):
         ctm.iterate()
         if iter_times is not None:
@@ -153,13 +156,11 @@ def hw_queue_share(n_slots):
     return max(1, min(4, 16 // max(1, n_slots)))
 
 
-def _worker_main(slot, q_in, q_out, debug_checks, plan_dir=None, host_threads=None, hw_queues=None,
-                 graphs=True):
+def _worker_main(slot, q_in, q_out, debug_checks, plan_dir=None, host_threads=None, hw_queues=None):
     """Slot process: serve requests from q_in until None arrives. With a
     plan_dir, the irreps-only permute structures are shared with the other
     slots through that directory; host_threads bounds its host thread pools,
-    hw_queues its hardware queues (set before the process's first HIP call);
-    graphs is the parent's config["ctmrg_graphs"]."""
+    hw_queues its hardware queues (set before the process's first HIP call)."""
     if hw_queues:
         os.environ["GPU_MAX_HW_QUEUES"] = str(int(hw_queues))
     import torch  # noqa: PLC0415 - imported in the worker process
@@ -173,7 +174,6 @@ def _worker_main(slot, q_in, q_out, debug_checks, plan_dir=None, host_threads=No
         threadpool_limits(host_threads)
         torch.set_num_threads(host_threads)
     config["debug_checks"] = debug_checks
-    config["ctmrg_graphs"] = bool(graphs)
     plan_store.set_plan_dir(plan_dir)
     try:
         total = int(torch.cuda.get_device_properties(0).total_memory)
@@ -238,8 +238,7 @@ class ServingEngine:
             self._workers = [
                 ctx.Process(target=_worker_main,
                             args=(i, self._q_in[i], self._q_out, config["debug_checks"], self.plan_dir,
-                                  host_thread_share(self.n_slots), hw_queue_share(self.n_slots),
-                                  config["ctmrg_graphs"]),
+                                  host_thread_share(self.n_slots), hw_queue_share(self.n_slots)),
                             daemon=True)
                 for i in range(self.n_slots)
             ]
