@@ -94,12 +94,16 @@ def run_concurrently(thunks):
     sides = [side_stream(1 + i) for i in range(len(thunks) - 1)]
     for st in sides:
         st.wait_stream(main)
-    results = [thunks[0]()]
-    for st, t in zip(sides, thunks[1:], strict=True):
-        with torch.cuda.stream(st):
-            results.append(t())
-    for st in sides:
-        main.wait_stream(st)
+    try:
+        results = [thunks[0]()]
+        for st, t in zip(sides, thunks[1:], strict=True):
+            with torch.cuda.stream(st):
+                results.append(t())
+    finally:
+        # joined back whatever happened (a capture must not end with a side
+        # stream's work left unjoined)
+        for st in sides:
+            main.wait_stream(st)
     for r in results[1:]:
         for buf in _device_buffers(r):
             buf.record_stream(main)
@@ -210,14 +214,27 @@ def zeros(n, dtype=DTYPE):
     return torch.zeros((int(n),), dtype=dtype, device=device())
 
 
+class NotRecordable(RuntimeError):
+    """A host round trip (synchronous copy) inside a graph recording: the
+    recorder stops recording and the iteration continues eagerly."""
+
+
+def _refuse_recording(what):
+    if recording():
+        msg = f"{what} inside a recorded CTMRG iteration"
+        raise NotRecordable(msg)
+
+
 def upload(arr, dtype=DTYPE):
     """Host array -> new 1-D device tensor (flattened in C order)."""
+    _refuse_recording("synchronous host-to-device copy")
     a = np.ascontiguousarray(arr).reshape(-1)
     return torch.from_numpy(a).to(device=device(), dtype=dtype)
 
 
 def download(t):
     """Device tensor -> host numpy array (synchronizes the stream)."""
+    _refuse_recording("synchronous device-to-host copy")
     return t.detach().to("cpu").numpy()
 
 
