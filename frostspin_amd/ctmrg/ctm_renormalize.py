@@ -52,42 +52,46 @@ def construct_projectors(
 class _CutDecision:
     """The host step of a projector batch between its one wait and the
     projector launches: singular values read back, redone decompositions,
-    cuts. A CTMRG graph recorder re-evaluates it after replaying the recorded
-    launches before the wait, compares the cuts with the recorded ones and
-    writes the new s^-1/2 weights into the pinned buffer the recorded upload
-    reads (staging)."""
+    cuts. The launches after it depend on the cuts only (the s^-1/2 weights
+    are gathered on the device), so a CTMRG graph recorder may replay them
+    before this decision is re-evaluated: replay() then tells whether the
+    cuts equal the recorded ones."""
 
     def __init__(self):
         self.evaluate = None
         self.cuts = None
         self.non_empty = None
         self.redone = []
-        self.staging = []
 
     def replay(self):
-        """After a replay of the launches before the wait: True when the cuts
-        equal the recorded ones (the weights are then written for the recorded
-        upload), False when the eager path would decide otherwise."""
+        """After a replay of the recorded launches: True when the re-evaluated
+        cuts equal the recorded ones, False when the eager path would decide
+        otherwise (other cuts, or a decomposition to redo)."""
         res = self.evaluate(True)
-        if res is None:
-            return False
-        cuts, s_of = res[0], res[1]
-        if not np.array_equal(cuts, self.cuts):
-            return False
-        if self.staging:
-            w = _inv_sqrt_kept(s_of, cuts, self.non_empty)
-            buf = self.staging[0]
-            if buf.numel() != w.size:
-                return False
-            buf.copy_(torch.from_numpy(w))
-        return True
+        return res is not None and np.array_equal(res[0], self.cuts)
 
 
-def _inv_sqrt_kept(s_of, cuts, non_empty):
-    """s^-1/2 of the kept singular values of every non-empty sector, packed."""
+def _inv_sqrt_kept(S_of, cuts, non_empty):
+    """s^-1/2 of the kept singular values of every non-empty sector, packed in
+    that order, computed on the device: S_of[i] = (device buffer, offset) of
+    sector i's values; one gather from those buffers, then 1 / sqrt."""
     if not len(non_empty):
-        return np.zeros((0,))
-    return np.concatenate([1.0 / np.sqrt(s_of[i][: cuts[i]]) for i in non_empty])
+        return _ops.empty(0)
+    srcs, pos = [], []
+    for i in non_empty:
+        t, off = S_of[i]
+        k = next((q for q, x in enumerate(srcs) if x is t), None)
+        if k is None:
+            srcs.append(t)
+            k = len(srcs) - 1
+        pos.append((k, off, int(cuts[i])))
+    base = np.zeros((len(srcs),), dtype=np.int64)
+    if len(srcs) > 1:
+        base[1:] = np.cumsum([x.numel() for x in srcs[:-1]])
+    idx = np.concatenate([base[k] + off + np.arange(c, dtype=np.int64) for k, off, c in pos])
+    S_all = srcs[0] if len(srcs) == 1 else torch.cat(srcs)
+    idx_dev = _ops.upload_async(idx, dtype=torch.int64)
+    return 1.0 / torch.sqrt(torch.index_select(S_all, 0, idx_dev))
 
 
 def construct_projectors_batch(
@@ -188,7 +192,7 @@ def construct_projectors_batch(
         else:
             U, S, s_off, VT, finish = bl.svd_async(M.select(dense_ids))
             s_h = _ops.download_start(S)
-        dense_done = (U, VT, finish, s_h)
+        dense_done = (U, VT, finish, s_h, S, s_off)
     if rand_ids:
         rand_done = bl.rsvd_async(M.select(rand_ids), k_need[rand_ids], seed=rsvd_seed)
 
@@ -203,7 +207,7 @@ def construct_projectors_batch(
         """Singular values read back -> per-sector cuts. replay: the readback
         buffers were written again by a recorded graph; returns None where the
         eager path would launch more work (a redone decomposition)."""
-        U_of, s_of, VT_of = [None] * n, [None] * n, [None] * n
+        U_of, s_of, VT_of, S_of = [None] * n, [None] * n, [None] * n, [None] * n
         if rand_done is not None:
             U, S, s_off, VT, finish = rand_done
             s_list = finish(replay=replay, redone=decision.redone)
@@ -213,13 +217,15 @@ def construct_projectors_batch(
                 U_of[i] = (U, j)
                 VT_of[i] = (VT, j)
                 s_of[i] = s_list[j]
+                S_of[i] = (S, int(s_off[j]))
         if dense_done is not None:
-            U, VT, finish, s_h = dense_done
+            U, VT, finish, s_h, S, s_off = dense_done
             s_list = finish(s_h)
             for j, i in enumerate(dense_ids):
                 U_of[i] = (U, j)
                 VT_of[i] = (VT, j)
                 s_of[i] = s_list[j]
+                S_of[i] = (S, int(s_off[j]))
         # the randomized SVD's oversampled tail is not accurate: only the k_need
         # leading values enter the cut (block_chi on the reference's ARPACK branch)
         for i in rand_ids:
@@ -261,6 +267,7 @@ def construct_projectors_batch(
                 s_list = finish()
                 for j, i in enumerate(exact):
                     U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j]
+                    S_of[i] = (S, int(s_off[j]))
             if wider:
                 ks = np.array([min(int(mins[i]), 2 * len(s_of[i])) for i in wider], dtype=np.int64)
                 U, S, s_off, VT, finish = bl.rsvd_async(M.select(wider), ks, seed=rsvd_seed)
@@ -268,13 +275,14 @@ def construct_projectors_batch(
                 s_list = finish()
                 for j, i in enumerate(wider):
                     U_of[i], VT_of[i], s_of[i] = (U, j), (VT, j), s_list[j][: ks[j]]
+                    S_of[i] = (S, int(s_off[j]))
             for pi in sorted({sectors[i][0] for i in short}):
                 cut_problem(pi)
-        return cuts, s_of, U_of, VT_of
+        return cuts, s_of, U_of, VT_of, S_of
 
     decision.evaluate = decide
     _ops.host_wait(decision)
-    cuts, s_of, U_of, VT_of = decide(False)
+    cuts, s_of, U_of, VT_of, S_of = decide(False)
     decision.cuts = cuts.copy()
     if diagnostics is not None:
         diagnostics.update(
@@ -284,13 +292,14 @@ def construct_projectors_batch(
     non_empty = cuts.nonzero()[0]
     decision.non_empty = non_empty
 
-    # s^-1/2 of the kept values, one device vector for all sectors
-    w_host = _inv_sqrt_kept(s_of, cuts, non_empty)
+    # s^-1/2 of the kept values, one device vector for all sectors, from the
+    # device copies of the singular values (the launches after the wait then
+    # depend on the cuts only, not on values read back)
     w_off = np.zeros((len(non_empty),), dtype=np.int64)
     if len(non_empty) > 1:
         w_off[1:] = np.cumsum(cuts[non_empty])[:-1]
     _ops.host_resume(decision)
-    w_dev = _ops.upload_async(w_host, staging=decision.staging)
+    w_dev = _inv_sqrt_kept(S_of, cuts, non_empty)
 
     # Us = U[:, :cut] * s^-1/2 and Vs = s^-1/2 * VT[:cut, :] per sector (grouped by source buffer)
     kept = [int(cuts[i]) for i in non_empty]
