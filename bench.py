@@ -703,8 +703,8 @@ def main():
         args.slots = 8 if args.workload == "serve" else 4
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU only: the hot path is single-device "
-            "(sharding is described in DESIGN.md, not run).",
+            "bench.py measures one GPU only: the hot path runs on a single device "
+            "(how it would shard is a design note; no multi-GPU run exists).",
             file=sys.stderr,
         )
         sys.exit(2)

@@ -352,23 +352,25 @@ def memcpy_async(dst, src, nbytes):
     check(lib.fs_memcpy_async(dst.data_ptr(), src.data_ptr(), int(nbytes), _stream()))
 
 
-def upload_async(arr, dtype=DTYPE, *, staging=None):
+def upload_async(arr, dtype=DTYPE, *, structural=False):
     """Host array -> new 1-D device tensor without a host wait: staged through
     pinned memory, copied on the current stream (ordered before the kernels
-    launched after it). staging: a list that receives the pinned buffer when
-    the copy is recorded into a graph (a replay copies whatever the buffer
-    then holds)."""
+    launched after it). Recorded into a graph, the copy reads a pinned buffer
+    of the recorder's and every replay copies the same values, so inside a
+    recorded iteration only data fixed by the recorded structure (indices)
+    may be uploaded: structural=True says so, anything else is refused."""
     a = torch.from_numpy(np.ascontiguousarray(arr).reshape(-1))
     dev = device()
     if dev.type != "cuda":
         return a.to(dtype=dtype, copy=True)
     if recording():
+        if not structural:
+            msg = "upload of values inside a recorded CTMRG iteration"
+            raise NotRecordable(msg)
         h = GraphHooks.active.pinned(a.numel(), dtype)
         h.copy_(a)
         d = torch.empty(a.shape, dtype=dtype, device=dev)
         memcpy_async(d, h, h.numel() * h.element_size())
-        if staging is not None:
-            staging.append(h)
         return d
     h = torch.empty(a.shape, dtype=dtype, pin_memory=True)
     h.copy_(a)

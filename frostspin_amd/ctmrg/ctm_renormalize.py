@@ -90,7 +90,7 @@ def _inv_sqrt_kept(S_of, cuts, non_empty):
         base[1:] = np.cumsum([x.numel() for x in srcs[:-1]])
     idx = np.concatenate([base[k] + off + np.arange(c, dtype=np.int64) for k, off, c in pos])
     S_all = srcs[0] if len(srcs) == 1 else torch.cat(srcs)
-    idx_dev = _ops.upload_async(idx, dtype=torch.int64)
+    idx_dev = _ops.upload_async(idx, dtype=torch.int64, structural=True)  # fixed by the cuts
     return 1.0 / torch.sqrt(torch.index_select(S_all, 0, idx_dev))
 
 
