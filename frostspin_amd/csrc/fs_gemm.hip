@@ -32,11 +32,12 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 struct GroupDev {
   int32_t m, n, k, tiles_n;
   int32_t a_rs, a_cs, b_rs, b_cs;
-  int32_t c_ld, kc;  // kc: K range of one split (multiple of the K panel)
+  int32_t c_ld, kc;    // kc: K range of one split (multiple of the K panel)
+  int32_t splits, pad;  // split-K: workgroups per output tile of this group
   const double *a;
   const double *b;
   double *c;
-  int64_t ws_off;  // split-K: this group's m x n partial at ws + split * ws_stride + ws_off
+  int64_t ws_off;  // split-K: partial s of this group's m x n tile at ws + ws_off + s * m * n
 };
 
 struct Batch {
@@ -45,7 +46,6 @@ struct Batch {
   int32_t tile_prefix[kMaxGroups + 1];
   GroupDev g[kMaxGroups];
   double *ws;  // null: no split-K, the epilogue writes C
-  int64_t ws_stride;
   int64_t elem_prefix[kMaxGroups + 1];
 };
 
@@ -145,6 +145,9 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK, WGM, WGN>()))) 
   const int tile = bt.xcd_remap ? xcd_logical_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const int gi = find_group(bt, tile);
   const GroupDev &G = bt.g[gi];
+  // split-K: a group with fewer splits than the launch leaves the rest idle
+  // (whole workgroup, before any barrier)
+  if (static_cast<int>(blockIdx.y) >= G.splits) return;
   const int t = tile - bt.tile_prefix[gi];
   const int tm = t / G.tiles_n;
   const int tn = t - tm * G.tiles_n;
@@ -260,7 +263,7 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK, WGM, WGN>()))) 
   // epilogue: for v_mfma_f64_16x16x4f64 lane l holds D[l/16 + 4*r][l%16] of
   // each 16x16 tile (rows interleaved by 4, measured on gfx950)
   if (bt.ws) {  // split-K partial, reduced by gemm_splitk_reduce
-    double *__restrict__ P = bt.ws + static_cast<int64_t>(blockIdx.y) * bt.ws_stride + G.ws_off;
+    double *__restrict__ P = bt.ws + G.ws_off + static_cast<int64_t>(blockIdx.y) * G.m * G.n;
 #pragma unroll
     for (int a = 0; a < TI; ++a)
 #pragma unroll
@@ -295,7 +298,7 @@ __attribute__((amdgpu_waves_per_eu(gemm_waves_per_eu<BM, BN, BK, WGM, WGN>()))) 
 // splits are summed in a fixed order, so results do not depend on scheduling.
 constexpr int kReduceThreads = 256;
 
-__global__ void __launch_bounds__(kReduceThreads) gemm_splitk_reduce(Batch bt, int splits) {
+__global__ void __launch_bounds__(kReduceThreads) gemm_splitk_reduce(Batch bt) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * kReduceThreads + threadIdx.x;
   if (e >= bt.elem_prefix[bt.ngroups]) return;
   int g = 0;
@@ -305,7 +308,8 @@ __global__ void __launch_bounds__(kReduceThreads) gemm_splitk_reduce(Batch bt, i
   const int64_t row = local / G.n, col = local - row * G.n;
   const double *p = bt.ws + G.ws_off + local;
   double sum = 0.0;
-  for (int s = 0; s < splits; ++s) sum += p[static_cast<int64_t>(s) * bt.ws_stride];
+  const int64_t stride = static_cast<int64_t>(G.m) * G.n;
+  for (int s = 0; s < G.splits; ++s) sum += p[s * stride];
   double *c = G.c + row * G.c_ld + col;
   const double v = bt.alpha * sum;
   *c = (bt.beta == 0.0) ? v : v + bt.beta * (*c);
@@ -418,6 +422,9 @@ __global__ void __launch_bounds__(kThreads) gemm_split_kernel(Batch bt) {
   const int tile = bt.xcd_remap ? xcd_logical_tile(blockIdx.x, gridDim.x) : blockIdx.x;
   const int gi = find_group(bt, tile);
   const GroupDev &G = bt.g[gi];
+  // split-K: a group with fewer splits than the launch leaves the rest idle
+  // (whole workgroup, before any barrier)
+  if (static_cast<int>(blockIdx.y) >= G.splits) return;
   const int t = tile - bt.tile_prefix[gi];
   const int tm = t / G.tiles_n;
   const int tn = t - tm * G.tiles_n;
@@ -568,14 +575,23 @@ constexpr int tile_lds_bytes(int bm, int bn) {
 
 // Split-K: a launch whose output tiles leave most workgroup slots of the chip
 // empty while its K is long (projected corners: chi x chi outputs over chi D^2;
-// range-finder products: l x n outputs over m) runs `splits` workgroups per
+// range-finder products: l x n outputs over m) runs several workgroups per
 // tile over disjoint K ranges of at least kSplitMinK; partial tiles go to a
-// per-thread workspace and a second kernel sums them in split order. Each
-// workgroup keeps one K panel in flight, so on such launches the bytes in
-// flight, and with them the bandwidth reached, grow with the number of
-// resident workgroups: the split fills the resident slots (slots_per_cu
-// workgroups per CU, set by the tile's LDS footprint out of 160 KB, at most 6).
-// FS_GEMM_SPLITK=0 disables it (tuning only).
+// workspace and a second kernel sums them in split order. Each workgroup keeps
+// a few K panels in flight, so on such launches the bytes in flight, and with
+// them the bandwidth reached, grow with the number of resident workgroups: the
+// splits fill the resident slots (slots_per_cu workgroups per CU, set by the
+// tile's LDS footprint out of 160 KB, at most 6).
+//
+// The launch's K panels (tiles x ceil(k / kBK) summed over its groups) are
+// dealt in shares of panels / slots: a group gets one split per share its K
+// holds, so a group with a long K gets more splits than one with a short K
+// and every workgroup runs about one share. With one split count for the
+// whole launch, the tiles of the longest-K group would run longest while the
+// short groups' workgroups idle their slots (the range-finder products hold
+// sectors of 164 to 1412 rows and columns in one launch).
+// FS_GEMM_SPLITK=0 disables split-K, FS_GEMM_SPLITK_BALANCE=0 gives every
+// group the launch's uniform split count (tuning only).
 constexpr int kSplitMinK = 256;
 constexpr int kMaxSplits = 16;
 
@@ -584,16 +600,55 @@ inline int slots_per_cu(const TileShape &t) {
   return std::max(1, std::min(6, (160 * 1024) / tile_lds_bytes(t.bm, t.bn)));
 }
 
-inline int pick_splits(int64_t tiles, int64_t kmax, const TileShape &tile) {
-  static const bool off = [] {
-    const char *e = std::getenv("FS_GEMM_SPLITK");
-    return e && std::atoi(e) == 0;
-  }();
+inline bool env_off(const char *name) {
+  const char *e = std::getenv(name);
+  return e && std::atoi(e) == 0;
+}
+
+// Split-K plan of one launch: sets kc, splits and ws_off of each group and the
+// element prefix of the reduction; returns the launch's largest split count
+// (grid.y, 1 without split-K) and the workspace size in doubles.
+int plan_splits(Batch &bt, int64_t tiles, const TileShape &tile, int64_t *ws_doubles) {
+  static const bool off = env_off("FS_GEMM_SPLITK");
+  static const bool uniform = env_off("FS_GEMM_SPLITK_BALANCE");
+  *ws_doubles = 0;
+  const int ng = bt.ngroups;
   const int64_t slots = static_cast<int64_t>(kCUs) * slots_per_cu(tile);
+  int64_t kmax = 0, panels = 0;
+  for (int i = 0; i < ng; ++i) {
+    const GroupDev &d = bt.g[i];
+    kmax = std::max<int64_t>(kmax, d.k);
+    panels += static_cast<int64_t>(bt.tile_prefix[i + 1] - bt.tile_prefix[i]) * fs::ceil_div(d.k, kBK);
+  }
   if (off || tiles * 2 > slots || kmax < 2 * kSplitMinK) return 1;
-  const int64_t by_slots = slots / tiles;
-  const int64_t by_k = kmax / kSplitMinK;
-  return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>({by_slots, by_k, kMaxSplits})));
+  // uniform rule: one count for every group, by free slots and by the longest K
+  const int64_t uni = std::max<int64_t>(1, std::min<int64_t>({slots / tiles, kmax / kSplitMinK, kMaxSplits}));
+  const int64_t share = std::max<int64_t>(fs::ceil_div(panels, slots), kSplitMinK / kBK);
+  int grid_y = 1;
+  int64_t ws = 0, elems = 0;
+  bt.elem_prefix[0] = 0;
+  for (int i = 0; i < ng; ++i) {
+    GroupDev &d = bt.g[i];
+    const int64_t kp = fs::ceil_div(d.k, kBK);  // K panels of the group
+    int64_t want = uniform ? uni : std::min<int64_t>(fs::ceil_div(kp, share), kMaxSplits);
+    want = std::max<int64_t>(1, std::min<int64_t>(want, std::max<int64_t>(kp, 1)));
+    d.kc = static_cast<int32_t>(std::max<int64_t>(fs::ceil_div(kp, want), 1) * kBK);
+    d.splits = d.k > 0 ? static_cast<int32_t>(fs::ceil_div(d.k, d.kc)) : 1;
+    d.ws_off = ws;
+    ws += static_cast<int64_t>(d.splits) * d.m * d.n;
+    elems += static_cast<int64_t>(d.m) * d.n;
+    bt.elem_prefix[i + 1] = elems;
+    grid_y = std::max(grid_y, static_cast<int>(d.splits));
+  }
+  if (grid_y == 1) {  // nothing split after all: plain launch
+    for (int i = 0; i < ng; ++i) {
+      bt.g[i].kc = static_cast<int32_t>(fs::ceil_div(bt.g[i].k, kBK) * kBK);
+      bt.g[i].ws_off = 0;
+    }
+    return 1;
+  }
+  *ws_doubles = ws;
+  return grid_y;
 }
 
 // Split-K workspaces: one per (device, stream), grow-only, process-wide, so
@@ -785,6 +840,8 @@ int fill_group(GroupDev &d, const fs_gemm_group &q, const double *a_base, const 
   d.b = b_base + q.b_off;
   d.c = c_base + q.c_off;
   d.kc = static_cast<int32_t>(fs::ceil_div(q.k, kBK) * kBK);
+  d.splits = 1;
+  d.pad = 0;
   d.ws_off = 0;
   return FS_OK;
 }
@@ -862,35 +919,25 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     bt.xcd_remap = tiles >= 64 ? 1 : 0;
     int splits = 1;
     if (!SPLIT) {
-      int64_t kmax = 0, elems = 0;
-      for (int i = 0; i < ng; ++i) kmax = std::max<int64_t>(kmax, bt.g[i].k);
-      splits = pick_splits(tiles, kmax, tile);
+      int64_t ws_doubles = 0;
+      splits = plan_splits(bt, tiles, tile, &ws_doubles);
       if (splits > 1) {
-        bt.elem_prefix[0] = 0;
-        for (int i = 0; i < ng; ++i) {
-          GroupDev &d = bt.g[i];
-          d.kc = static_cast<int32_t>(fs::ceil_div(fs::ceil_div(d.k, splits), kBK) * kBK);
-          d.ws_off = elems;
-          elems += static_cast<int64_t>(d.m) * d.n;
-          bt.elem_prefix[i + 1] = elems;
-        }
         if (noalloc) {
-          FS_CHECK_ARG(elems * splits <= kScratchDoubles,
+          FS_CHECK_ARG(ws_doubles <= kScratchDoubles,
                        "grouped gemm (no-alloc): split-K needs %lld doubles, beyond the static "
                        "scratch of %lld",
-                       static_cast<long long>(elems * splits), static_cast<long long>(kScratchDoubles));
+                       static_cast<long long>(ws_doubles), static_cast<long long>(kScratchDoubles));
           ws_lock = std::unique_lock<std::mutex>(scratch_table().mu);
           const int st = scratch_ptr_locked(&bt.ws);
           if (st != FS_OK) return st;
         } else {
           ws_lock = std::unique_lock<std::mutex>(splitk_pool().mu);
-          bt.ws = splitk_workspace_locked(elems * splits, fs::as_stream(stream));
+          bt.ws = splitk_workspace_locked(ws_doubles, fs::as_stream(stream));
           if (bt.ws == nullptr) {
             fs::set_error("grouped gemm: split-K workspace allocation failed");
             return FS_ERR_HIP;
           }
         }
-        bt.ws_stride = elems;
       }
     }
     const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(splits));
@@ -904,7 +951,7 @@ int launch_grouped(int ngroups, const fs_gemm_group *groups, const double *const
     if (splits > 1) {
       const int64_t elems = bt.elem_prefix[ng];
       hipLaunchKernelGGL(gemm_splitk_reduce, dim3(static_cast<unsigned>(fs::ceil_div(elems, kReduceThreads))),
-                         dim3(kReduceThreads), 0, fs::as_stream(stream), bt, splits);
+                         dim3(kReduceThreads), 0, fs::as_stream(stream), bt);
       FS_LAUNCH_CHECK("gemm_splitk_reduce");
       ++dispatches;
     }
