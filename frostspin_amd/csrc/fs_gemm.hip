@@ -706,9 +706,10 @@ double *splitk_workspace_locked(int64_t n_doubles, hipStream_t stream) {
 
 // Allocation-free entry point (fs_grouped_gemm_f64_noalloc): the split-K
 // partial tiles live in this file-scope device array instead of a pool
-// buffer. Its size bounds every split-K launch of the default tile rules
-// (at most ~2 * 256 partial 64 x 64 tiles); a launch that would need more is
-// refused. Each device holds its own copy at its own address, looked up once
+// buffer. The split rule keeps a launch's partial tiles within its resident
+// workgroup slots plus its output tiles (at most 2304 partial 32 x 32 tiles,
+// 18 MiB; 64 x 64 tiles over a long K can need more); a launch that would
+// need more than the array is refused. Each device holds its own copy at its own address, looked up once
 // per device with hipGetSymbolAddress; host code uses only that address.
 constexpr int64_t kScratchDoubles = int64_t(1) << 21;  // 16 MiB per device
 static __device__ __align__(256) double fs_gemm_scratch[kScratchDoubles];

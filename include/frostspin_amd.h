@@ -338,7 +338,9 @@ int fs_grouped_gemm_f64_multi(int ngroups, const fs_gemm_group *groups,
  * per-device scratch array of 2^21 doubles (16 MiB, held by each device the
  * library runs on until the process ends) instead of a pooled buffer. A call
  * whose split-K partials would exceed it fails with FS_ERR_INVALID (the
- * default tile rules need at most ~13 MiB). Every partial is written before
+ * split rule keeps them within the resident workgroup slots plus the output
+ * tiles: up to ~18 MiB with 32 x 32 tiles, more with 64 x 64 tiles over a
+ * long K, which are then refused). Every partial is written before
  * it is read, so nothing needs clearing between calls. Calls that overlap on
  * one device (other streams, other threads' streams) share the array and
  * race; serialize them.
