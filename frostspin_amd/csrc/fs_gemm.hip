@@ -591,7 +591,8 @@ constexpr int tile_lds_bytes(int bm, int bn) {
 // short groups' workgroups idle their slots (the range-finder products hold
 // sectors of 164 to 1412 rows and columns in one launch).
 // FS_GEMM_SPLITK=0 disables split-K, FS_GEMM_SPLITK_BALANCE=0 gives every
-// group the launch's uniform split count (tuning only).
+// group the launch's uniform split count, FS_GEMM_SPLITK_FILL=P sizes the
+// shares for P percent of the resident slots (tuning only).
 constexpr int kSplitMinK = 256;
 constexpr int kMaxSplits = 16;
 
@@ -611,6 +612,11 @@ inline bool env_off(const char *name) {
 int plan_splits(Batch &bt, int64_t tiles, const TileShape &tile, int64_t *ws_doubles) {
   static const bool off = env_off("FS_GEMM_SPLITK");
   static const bool uniform = env_off("FS_GEMM_SPLITK_BALANCE");
+  static const int64_t fill = [] {  // percent of the resident slots the shares target
+    const char *e = std::getenv("FS_GEMM_SPLITK_FILL");
+    const int v = e ? std::atoi(e) : 100;
+    return static_cast<int64_t>(v >= 25 && v <= 400 ? v : 100);
+  }();
   *ws_doubles = 0;
   const int ng = bt.ngroups;
   const int64_t slots = static_cast<int64_t>(kCUs) * slots_per_cu(tile);
@@ -623,7 +629,7 @@ int plan_splits(Batch &bt, int64_t tiles, const TileShape &tile, int64_t *ws_dou
   if (off || tiles * 2 > slots || kmax < 2 * kSplitMinK) return 1;
   // uniform rule: one count for every group, by free slots and by the longest K
   const int64_t uni = std::max<int64_t>(1, std::min<int64_t>({slots / tiles, kmax / kSplitMinK, kMaxSplits}));
-  const int64_t share = std::max<int64_t>(fs::ceil_div(panels, slots), kSplitMinK / kBK);
+  const int64_t share = std::max<int64_t>(fs::ceil_div(panels * 100, slots * fill), kSplitMinK / kBK);
   int grid_y = 1;
   int64_t ws = 0, elems = 0;
   bt.elem_prefix[0] = 0;
