@@ -345,6 +345,33 @@ def test_gemm_split_k_fills_slots(gpu):
     assert _ops._gemm_dispatches() == 2
 
 
+def test_gemm_split_k_shares_per_group(gpu):
+    """Range-finder launch of ten sectors (m = k from 164 to 1412, 20 to 66
+    columns): the K panels are dealt per group, so the long-K sectors run more
+    splits than the short ones in the same launch (one tile kernel and one
+    reduction); against numpy for two layouts, and bit-identical when the same
+    call is repeated."""
+    rng = np.random.default_rng(16)
+    ms = [164, 300, 420, 560, 700, 840, 980, 1120, 1260, 1412]
+    ns = [20, 26, 30, 36, 40, 46, 50, 56, 60, 66]
+    shapes = [(m, n, m) for m, n in zip(ms, ns, strict=True)]
+    _check_groups(shapes, [(False, False)] * 10, rng)
+    assert _ops._gemm_dispatches() == 2
+    _check_groups(shapes, [(i % 2 == 1, False) for i in range(10)], rng)
+    groups, ao, bo, co = [], 0, 0, 0
+    for m, n, k in shapes:
+        groups.append((m, n, k, ao, k, 1, bo, n, 1, co, n))
+        ao, bo, co = ao + m * k, bo + k * n, co + m * n
+    a = _dev(rng.standard_normal(ao))
+    b = _dev(rng.standard_normal(bo))
+    outs = []
+    for _ in range(2):
+        C = _ops.empty(co)
+        _ops.grouped_gemm(groups, a, b, C)
+        outs.append(_ops.download(C))
+    assert np.array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize(
     "shapes",
     [
