@@ -39,7 +39,7 @@ PEAK_HBM = 8.0e12
 # chi=256, beta=4, FP64), from rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled
 # by the same counters on a 1 GiB device copy (factors 2.0 and 1.0); the
 # counters cannot be read live, so the figure applies to that workload only.
-GEMM_TRAFFIC_SU2_D8_CHI256 = 33.10e6
+GEMM_TRAFFIC_SU2_D8_CHI256 = 33.98e6
 GEMM_TRAFFIC_SOURCE = "profiles/ctmrg_su2_heisenberg_d8_chi256_beta4_hbm_traffic.md"
 PROFILE_MARKER_CYCLES = 20_000_000
 REPLAY_SPIN_CYCLES = 100_000_000  # longer than the host takes to enqueue one replay window
