@@ -583,8 +583,9 @@ constexpr int tile_lds_bytes(int bm, int bn) {
 // splits fill the resident slots (slots_per_cu workgroups per CU, set by the
 // tile's LDS footprint out of 160 KB, at most 6). The 32 x 32 instance takes
 // 124 registers, so 4 of its workgroups are resident per CU, not the 6 its LDS
-// allows; shares sized for 4 were 2 % slower (FS_GEMM_SPLITK_FILL=67), and
-// holding the kernel to 94 registers for 5 per CU 3 % slower.
+// allows; shares sized for half the LDS-bound slots were 2 % slower
+// (FS_GEMM_SPLITK_FILL=50), and holding the kernel to 94 registers for 5 per
+// CU 3 % slower.
 //
 // The launch's K panels (tiles x ceil(k / kBK) summed over its groups) are
 // dealt in shares of panels / slots: a group gets one split per share its K
